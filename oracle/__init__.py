@@ -1,0 +1,347 @@
+"""ctypes wrapper of the FP64 oracle (gcoracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path. Only tests/, the smoke
+check and bench.py's cpu_baseline leg load it; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+_HIER = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HIER, "libgcoracle.so")
+_lib = None
+
+D = C.POINTER(C.c_double)
+U32 = C.POINTER(C.c_uint32)
+I32 = C.POINTER(C.c_int32)
+
+
+class Szene_c(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("x", D), ("v", D), ("masse", D), ("w", D), ("kollisionsabstand", D), ("reibung", D),
+        ("n_abstand", C.c_int32), ("abstand_punkte", U32), ("abstand_ruhe", D), ("abstand_nachgiebigkeit", D),
+        ("membran_kante", C.POINTER(C.c_uint8)),
+        ("n_naht", C.c_int32), ("naht_punkte", U32), ("naht_ruhe", D),
+        ("n_biegung", C.c_int32), ("biegung_punkte", U32), ("biegung_ruhe", D), ("biegung_nachgiebigkeit", D),
+        ("n_membran", C.c_int32), ("membran_punkte", U32), ("membran_daten", D),
+        ("n_dreiecke", C.c_int32), ("dreiecke", U32), ("teil_von_punkt", U32),
+        ("bild_dt", C.c_double), ("schwerkraft", C.c_double * 3), ("daempfung", C.c_double),
+    ]
+
+
+class Param_c(C.Structure):
+    _fields_ = [
+        ("schritte_je_bild", C.c_int32), ("cg_max", C.c_int32), ("cg_toleranz", C.c_double), ("naht_k", C.c_double),
+        ("kontakt_rand", C.c_double), ("normalkraft_aus_last", C.c_int32), ("selbstkollision", C.c_int32),
+        ("selbst_zusatzdicke", C.c_double), ("selbst_steifigkeit", C.c_double),
+        ("selbst_stoss_iterationen", C.c_int32), ("selbst_schnitte_an_naehten", C.c_int32),
+    ]
+
+
+class Statistik_c(C.Structure):
+    _fields_ = [
+        ("cg_iterationen", C.c_int64), ("loesungen", C.c_int64), ("nicht_konvergiert", C.c_int64),
+        ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
+        ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
+    ]
+
+
+def bauen_falls_noetig() -> None:
+    src = [os.path.join(_HIER, f) for f in ("gcoracle.cpp", "gcoracle.h", "selbstkollision.cpp")]
+    if not os.path.exists(_SO) or any(os.path.getmtime(s) > os.path.getmtime(_SO) for s in src):
+        subprocess.run(["make", "-C", _HIER, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        bauen_falls_noetig()
+        _lib = C.CDLL(_SO)
+        L = _lib
+        L.or_sim_neu.restype = C.c_void_p
+        L.or_sim_neu.argtypes = [C.POINTER(Szene_c), C.POINTER(Param_c)]
+        for name in ("or_sim_frei", "or_sim_bild", "or_sim_reset_statistik"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.or_sim_schritt.argtypes = [C.c_void_p, C.c_double]
+        L.or_sys_bauen.argtypes = [C.c_void_p, C.c_double]
+        L.or_koerper_sdf.restype = C.c_double
+        L.or_winkel_fehler.restype = C.c_double
+        L.or_steifigkeit.restype = C.c_double
+        L.or_steifigkeit_aus.restype = C.c_double
+        L.or_membran_energie.restype = C.c_double
+        L.or_sys_n_kanten.restype = C.c_int32
+        L.or_sys_n_biegungen.restype = C.c_int32
+        L.or_sys_n_kanten.argtypes = [C.c_void_p]
+        L.or_sys_n_biegungen.argtypes = [C.c_void_p]
+        L.or_sim_koerper_zylinder.argtypes = [C.c_void_p, D, D, C.c_double, D]
+        L.or_sim_koerper_sdf.argtypes = [C.c_void_p, C.POINTER(C.c_float), I32, D, C.c_double, D, D]
+        for name in ("or_sim_lage", "or_sim_setze_lage"):
+            getattr(L, name).argtypes = [C.c_void_p, D, D]
+        L.or_sim_setze_naht_ruhe.argtypes = [C.c_void_p, D]
+        L.or_sim_setze_w.argtypes = [C.c_void_p, D]
+        L.or_sim_setze_dv_vorher.argtypes = [C.c_void_p, D, C.c_int32]
+        L.or_sim_statistik.argtypes = [C.c_void_p, C.POINTER(Statistik_c)]
+        L.or_sim_param.argtypes = [C.c_void_p, C.POINTER(Param_c)]
+        L.or_sys_kanten.argtypes = [C.c_void_p, U32, D, D, D, D]
+        L.or_sys_biegungen.argtypes = [C.c_void_p, U32, D, D]
+        L.or_sys_membran_h.argtypes = [C.c_void_p, D]
+        L.or_sys_vektoren.argtypes = [C.c_void_p, D, D, D, D]
+        L.or_sys_diag.argtypes = [C.c_void_p, D]
+        L.or_sys_fest.argtypes = [C.c_void_p, I32, D, D, D]
+        L.or_sys_setze_fest.argtypes = [C.c_void_p, I32, D, D, D]
+        L.or_sys_mal_h.argtypes = [C.c_void_p, D, D]
+        L.or_sys_mal_a.argtypes = [C.c_void_p, D, D]
+        L.or_sys_loese.argtypes = [C.c_void_p, D, I32, I32, D]
+        L.or_sys_rest.argtypes = [C.c_void_p, D, D]
+    return _lib
+
+
+def _p(a, t=D):
+    return a.ctypes.data_as(t)
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def param(**kw) -> Param_c:
+    p = Param_c()
+    lib().or_param_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class OrakelSim:
+    """The oracle's simulation handle for one scene (a ``garmentcraft_amd.szene.Szene``)."""
+
+    def __init__(self, s, p: Optional[Param_c] = None):
+        L = lib()
+        self._keep = []
+        k = self._keep
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            k.append(a)
+            return a
+
+        c = Szene_c()
+        c.n = s.n
+        c.x = _p(arr(s.x, np.float64)); c.v = _p(arr(s.v, np.float64))
+        c.masse = _p(arr(s.masse, np.float64)); c.w = _p(arr(s.w, np.float64))
+        c.kollisionsabstand = _p(arr(s.kollisionsabstand, np.float64)); c.reibung = _p(arr(s.reibung, np.float64))
+        c.n_abstand = s.abstand_punkte.shape[0]
+        c.abstand_punkte = _p(arr(s.abstand_punkte, np.uint32), U32)
+        c.abstand_ruhe = _p(arr(s.abstand_ruhe, np.float64))
+        c.abstand_nachgiebigkeit = _p(arr(s.abstand_nachgiebigkeit, np.float64))
+        c.membran_kante = _p(arr(s.membran_kante, np.uint8), C.POINTER(C.c_uint8))
+        c.n_naht = s.naht_punkte.shape[0]
+        c.naht_punkte = _p(arr(s.naht_punkte, np.uint32), U32); c.naht_ruhe = _p(arr(s.naht_ruhe, np.float64))
+        c.n_biegung = s.biegung_punkte.shape[0]
+        c.biegung_punkte = _p(arr(s.biegung_punkte, np.uint32), U32)
+        c.biegung_ruhe = _p(arr(s.biegung_ruhe, np.float64))
+        c.biegung_nachgiebigkeit = _p(arr(s.biegung_nachgiebigkeit, np.float64))
+        c.n_membran = s.membran_punkte.shape[0]
+        c.membran_punkte = _p(arr(s.membran_punkte, np.uint32), U32)
+        c.membran_daten = _p(arr(s.membran_daten, np.float64))
+        c.n_dreiecke = s.dreiecke.shape[0]
+        c.dreiecke = _p(arr(s.dreiecke, np.uint32), U32)
+        c.teil_von_punkt = _p(arr(s.teil_von_punkt, np.uint32), U32)
+        c.bild_dt = s.param.bild_dt
+        for q in range(3):
+            c.schwerkraft[q] = s.param.schwerkraft[q]
+        c.daempfung = s.param.daempfung
+        self.n = s.n
+        self.h = C.c_void_p(L.or_sim_neu(C.byref(c), C.byref(p) if p is not None else None))
+        self._keep = []
+        from garmentcraft_amd.szene import GitterSdf, Zylinder
+        for kb in s.koerper:
+            if isinstance(kb, Zylinder):
+                L.or_sim_koerper_zylinder(self.h, _p(f64(kb.punkt)), _p(f64(kb.achse)), kb.radius, _p(f64(kb.geschwindigkeit)))
+            elif isinstance(kb, GitterSdf):
+                w = np.ascontiguousarray(kb.werte, dtype=np.float32)
+                dims = np.array(kb.dims, dtype=np.int32)
+                L.or_sim_koerper_sdf(self.h, _p(w, C.POINTER(C.c_float)), _p(dims, I32), _p(f64(kb.origin)), kb.spacing,
+                                     _p(f64(kb.versatz)), _p(f64(kb.geschwindigkeit)))
+            else:
+                raise TypeError(kb)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.or_sim_frei(self.h)
+            self.h = None
+
+    def set_param(self, p: Param_c):
+        lib().or_sim_param(self.h, C.byref(p))
+
+    def lage(self):
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3))
+        lib().or_sim_lage(self.h, _p(x), _p(v))
+        return x, v
+
+    def setze_lage(self, x=None, v=None):
+        lib().or_sim_setze_lage(self.h, _p(f64(x)) if x is not None else None, _p(f64(v)) if v is not None else None)
+
+    def setze_naht_ruhe(self, r):
+        lib().or_sim_setze_naht_ruhe(self.h, _p(f64(r)))
+
+    def setze_w(self, w):
+        lib().or_sim_setze_w(self.h, _p(f64(w)))
+
+    def setze_dv_vorher(self, dv):
+        if dv is None:
+            lib().or_sim_setze_dv_vorher(self.h, None, 0)
+        else:
+            d = f64(dv)
+            lib().or_sim_setze_dv_vorher(self.h, _p(d), d.shape[0])
+
+    def bild(self):
+        lib().or_sim_bild(self.h)
+
+    def schritt(self, h: float):
+        lib().or_sim_schritt(self.h, h)
+
+    def statistik(self) -> dict:
+        st = Statistik_c()
+        lib().or_sim_statistik(self.h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in Statistik_c._fields_}
+
+    # system probes
+    def sys_bauen(self, h: float):
+        lib().or_sys_bauen(self.h, h)
+
+    def sys_vektoren(self):
+        n = self.n
+        rhs, f, last, pinv = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3))
+        lib().or_sys_vektoren(self.h, _p(rhs), _p(f), _p(last), _p(pinv))
+        return {"rhs": rhs, "f": f, "last": last, "pinv": pinv}
+
+    def sys_diag(self):
+        d = np.zeros((self.n, 3, 3))
+        lib().or_sys_diag(self.h, _p(d))
+        return d
+
+    def sys_kanten(self):
+        m = lib().or_sys_n_kanten(self.h)
+        ij = np.zeros((m, 2), np.uint32); k = np.zeros(m); r = np.zeros((m, 3)); q = np.zeros(m); ru = np.zeros(m)
+        lib().or_sys_kanten(self.h, _p(ij, U32), _p(k), _p(r), _p(q), _p(ru))
+        return {"ij": ij, "k": k, "richtung": r, "quer": q, "ruhe": ru}
+
+    def sys_biegungen(self):
+        m = lib().or_sys_n_biegungen(self.h)
+        pk = np.zeros((m, 4), np.uint32); g = np.zeros((m, 4, 3)); k = np.zeros(m)
+        lib().or_sys_biegungen(self.h, _p(pk, U32), _p(g), _p(k))
+        return {"punkte": pk, "grad": g, "k": k}
+
+    def sys_membran_h(self, t: int):
+        h = np.zeros((t, 9))
+        lib().or_sys_membran_h(self.h, _p(h))
+        return h
+
+    def sys_fest(self):
+        n = self.n
+        fl = np.zeros(n, np.int32); nr = np.zeros((n, 3)); vn = np.zeros(n); vk = np.zeros((n, 3))
+        lib().or_sys_fest(self.h, _p(fl, I32), _p(nr), _p(vn), _p(vk))
+        return {"flag": fl, "normale": nr, "vn_soll": vn, "v_koerper": vk}
+
+    def sys_setze_fest(self, fl, nr, vn, vk):
+        lib().or_sys_setze_fest(self.h, _p(np.ascontiguousarray(fl, np.int32), I32), _p(f64(nr)), _p(f64(vn)), _p(f64(vk)))
+
+    def sys_mal_h(self, p):
+        o = np.zeros((self.n, 3)); lib().or_sys_mal_h(self.h, _p(f64(p)), _p(o)); return o
+
+    def sys_mal_a(self, p):
+        o = np.zeros((self.n, 3)); lib().or_sys_mal_a(self.h, _p(f64(p)), _p(o)); return o
+
+    def sys_loese(self, dv):
+        d = f64(dv).copy()
+        it = C.c_int32(); nk = C.c_int32(); rest = C.c_double()
+        lib().or_sys_loese(self.h, _p(d), C.byref(it), C.byref(nk), C.byref(rest))
+        return d, {"iterationen": it.value, "nicht_konvergiert": bool(nk.value), "rest": rest.value}
+
+    def sys_rest(self, dv):
+        o = np.zeros((self.n, 3)); lib().or_sys_rest(self.h, _p(f64(dv)), _p(o)); return o
+
+
+# element-level helpers -------------------------------------------------------
+
+def membran_bauen(ruhe, kanten, dir2, material):
+    out = np.zeros(12)
+    ok = lib().or_membran_bauen(_p(f64(ruhe)), _p(f64(kanten)), _p(f64(dir2)), _p(f64(material)), _p(out))
+    return out if ok else None
+
+
+def _elem_fn(name, nout):
+    def f(elem, *args):
+        out = np.zeros(nout)
+        getattr(lib(), name)(_p(f64(elem)), *[_p(f64(a)) for a in args], _p(out))
+        return out
+    return f
+
+
+membran_dehnung = _elem_fn("or_membran_dehnung", 3)
+membran_spannung = _elem_fn("or_membran_spannung", 3)
+membran_krafte = _elem_fn("or_membran_krafte", 9)
+membran_krafte_zug = _elem_fn("or_membran_krafte_zug", 9)
+membran_hessen = _elem_fn("or_membran_hessen", 9)
+membran_mal_h_aus = _elem_fn("or_membran_mal_h_aus", 9)
+membran_diag_bloecke_aus = _elem_fn("or_membran_diag_bloecke_aus", 27)
+
+
+def membran_energie(elem, p):
+    lib().or_membran_energie.argtypes = [D, D]
+    return lib().or_membran_energie(_p(f64(elem)), _p(f64(p)))
+
+
+def psd2(a, b, c):
+    out = np.zeros(3); lib().or_psd2(C.c_double(a), C.c_double(b), C.c_double(c), _p(out)); return out
+
+
+def dieder(q):
+    t = C.c_double(); g = np.zeros(12)
+    ok = lib().or_dieder(_p(f64(q)), C.byref(t), _p(g))
+    return (t.value, g.reshape(4, 3)) if ok else None
+
+
+def winkel_fehler(theta, ruhe):
+    return lib().or_winkel_fehler(C.c_double(theta), C.c_double(ruhe))
+
+
+def steifigkeit(weft, warp, bias, theta):
+    return lib().or_steifigkeit(C.c_double(weft), C.c_double(warp), C.c_double(bias), C.c_double(theta))
+
+
+def steifigkeit_aus(c2, s2, weft, warp, bias):
+    return lib().or_steifigkeit_aus(*[C.c_double(v) for v in (c2, s2, weft, warp, bias)])
+
+
+def cos2_sin2(e, warp):
+    out = np.zeros(2)
+    ok = lib().or_cos2_sin2(_p(f64(e)), _p(f64(warp)), _p(out))
+    return (out[0], out[1]) if ok else None
+
+
+def m3_inv(m):
+    out = np.zeros(9); lib().or_m3_inv(_p(f64(m).ravel()), _p(out)); return out.reshape(3, 3)
+
+
+def koerper_sdf(form, x):
+    L = lib()
+    L.or_koerper_sdf.argtypes = [D, D, D, C.c_double, C.c_double, C.c_double, D]
+    return L.or_koerper_sdf(_p(f64(form.ell_c)), _p(f64(form.ell_r)), _p(f64(form.cyl_c)), form.cyl_r,
+                            form.cyl_z0, form.cyl_z1, _p(f64(x)))
+
+
+def sdf_sample(gitter, q):
+    L = lib()
+    d = C.c_double(); g = np.zeros(3)
+    w = np.ascontiguousarray(gitter.werte, dtype=np.float32)
+    dims = np.array(gitter.dims, dtype=np.int32)
+    ok = L.or_sdf_sample(_p(w, C.POINTER(C.c_float)), _p(dims, I32), _p(f64(gitter.origin)), C.c_double(gitter.spacing),
+                         _p(f64(q)), C.byref(d), _p(g))
+    return (d.value, g) if ok else None

@@ -1,0 +1,147 @@
+/*
+ * gcoracle — FP64 CPU restatement of klon3d-sim's implicit backward-Euler step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP path is
+ * compared against; it is never linked into, loaded by, or called from the
+ * product library. Only tests/, the smoke check and bench.py's cpu_baseline leg
+ * may load it.
+ *
+ * Every function restates a reference function with the same loop order,
+ * summation order and branch order; the citation is given at each definition in
+ * gcoracle.cpp (paths relative to the reference checkout, crates/klon3d-sim/src).
+ *
+ * Layout conventions (same as the product C ABI):
+ *   points:   double[3*n] interleaved (x0,y0,z0,x1,...) like Vec<[f64;3]>
+ *   membrane: double[12] per element =
+ *             {flaeche, ruhe_inv[4], kette[2], schuss[2], kette_n_m, dehnung_n_m, schub_n_m}
+ *   hessen:   double[9]  per element = {f1[3], f2[3], w[3]}
+ */
+#ifndef GCORACLE_H
+#define GCORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- element-level restatements (kontinuum.rs, cpu.rs, biegung.rs, implizit.rs) ---- */
+int    or_membran_bauen(const double ruhe[6], const double kanten[3], const double dir2[2],
+                        const double material[3], double out_elem[12]);
+void   or_membran_dehnung(const double elem[12], const double p[9], double out[3]);
+void   or_membran_spannung(const double elem[12], const double p[9], double out[3]);
+double or_membran_energie(const double elem[12], const double p[9]);
+void   or_membran_krafte(const double elem[12], const double p[9], double out[9]);
+void   or_membran_krafte_zug(const double elem[12], const double p[9], double out[9]);
+void   or_membran_hessen(const double elem[12], const double p[9], double out_h[9]);
+void   or_membran_mal_h_aus(const double elem[12], const double h[9], const double delta[9], double out[9]);
+void   or_membran_diag_bloecke_aus(const double elem[12], const double h[9], double out[27]);
+void   or_psd2(double a, double b, double c, double out[3]);
+int    or_dieder(const double q[12], double* theta, double grad[12]);
+double or_winkel_fehler(double theta, double ruhe);
+double or_steifigkeit(double weft, double warp, double bias, double theta);
+double or_steifigkeit_aus(double c2, double s2, double weft, double warp, double bias);
+int    or_cos2_sin2(const double e[2], const double warp[2], double out[2]);
+void   or_m3_inv(const double m[9], double out[9]);
+
+/* ---- bodies ---- */
+/* Exact signed distance of the ellipsoid-cylinder union used by the drape scenes
+ * (ellipsoid by Eberly's bisection on the Lagrange multiplier, capped cylinder
+ * closed form, union by min). Used to check the sampled grid. */
+double or_koerper_sdf(const double ell_c[3], const double ell_r[3], const double cyl_c[2], double cyl_r,
+                      double cyl_z0, double cyl_z1, const double x[3]);
+/* Trilinear sample of a grid SDF and its gradient; returns 0 outside the grid. */
+int    or_sdf_sample(const float* werte, const int dims[3], const double origin[3], double spacing,
+                     const double q[3], double* d, double grad[3]);
+
+/* ---- simulation ---- */
+typedef struct or_szene {
+  int32_t n;
+  const double* x;                 /* 3n */
+  const double* v;                 /* 3n */
+  const double* masse;             /* n  */
+  const double* w;                 /* n  */
+  const double* kollisionsabstand; /* n  */
+  const double* reibung;           /* n  */
+  int32_t n_abstand;
+  const uint32_t* abstand_punkte;  /* 2 per constraint */
+  const double* abstand_ruhe;
+  const double* abstand_nachgiebigkeit;
+  const uint8_t* membran_kante;    /* 1 = carried by a membrane (k = 0) */
+  int32_t n_naht;
+  const uint32_t* naht_punkte;     /* 2 per seam spring */
+  const double* naht_ruhe;         /* may be +inf (open seam) */
+  int32_t n_biegung;
+  const uint32_t* biegung_punkte;  /* 4 per constraint */
+  const double* biegung_ruhe;
+  const double* biegung_nachgiebigkeit;
+  int32_t n_membran;
+  const uint32_t* membran_punkte;  /* 3 per element */
+  const double* membran_daten;     /* 12 per element */
+  int32_t n_dreiecke;              /* cloth triangles (self-collision); may be 0 */
+  const uint32_t* dreiecke;
+  const uint32_t* teil_von_punkt;  /* piece index per point (self-collision); may be NULL */
+  double bild_dt;
+  double schwerkraft[3];
+  double daempfung;
+} or_szene;
+
+typedef struct or_param {
+  int32_t schritte_je_bild;
+  int32_t cg_max;
+  double cg_toleranz;
+  double naht_k;
+  double kontakt_rand;
+  int32_t normalkraft_aus_last;
+  int32_t selbstkollision;
+  double selbst_zusatzdicke;
+  double selbst_steifigkeit;
+  int32_t selbst_stoss_iterationen;
+  int32_t selbst_schnitte_an_naehten;
+} or_param;
+
+typedef struct or_statistik {
+  int64_t cg_iterationen, loesungen, nicht_konvergiert, kontakte, haftend, freigegeben;
+  double rest_max;
+  int64_t naehe_punkt_dreieck, naehe_kante_kante, selbst_stoesse;
+} or_statistik;
+
+typedef struct or_sim or_sim;
+
+void    or_param_default(or_param* p);
+or_sim* or_sim_neu(const or_szene* s, const or_param* p);
+void    or_sim_frei(or_sim* s);
+int     or_sim_koerper_zylinder(or_sim* s, const double punkt[3], const double achse[3], double radius, const double geschw[3]);
+int     or_sim_koerper_sdf(or_sim* s, const float* werte, const int dims[3], const double origin[3], double spacing,
+                           const double versatz[3], const double geschw[3]);
+void    or_sim_param(or_sim* s, const or_param* p);
+void    or_sim_lage(const or_sim* s, double* x, double* v);
+void    or_sim_setze_lage(or_sim* s, const double* x, const double* v);
+void    or_sim_setze_naht_ruhe(or_sim* s, const double* ruhe);
+void    or_sim_setze_w(or_sim* s, const double* w);
+void    or_sim_setze_dv_vorher(or_sim* s, const double* dv, int32_t n); /* n = 0 clears the warm start */
+void    or_sim_bild(or_sim* s);
+void    or_sim_schritt(or_sim* s, double h);
+void    or_sim_statistik(const or_sim* s, or_statistik* out);
+void    or_sim_reset_statistik(or_sim* s);
+
+/* System of the next substep at the current state (implizit.rs system_bauen),
+ * kept inside the handle for the probes below. */
+void    or_sys_bauen(or_sim* s, double h);
+int32_t or_sys_n_kanten(const or_sim* s);   /* edges followed by seams */
+int32_t or_sys_n_biegungen(const or_sim* s);
+void    or_sys_kanten(const or_sim* s, uint32_t* ij, double* k, double* richtung, double* quer, double* ruhe);
+void    or_sys_biegungen(const or_sim* s, uint32_t* punkte, double* grad, double* k);
+void    or_sys_membran_h(const or_sim* s, double* out_h9);
+void    or_sys_vektoren(const or_sim* s, double* rhs, double* f, double* last, double* pinv9);
+void    or_sys_diag(const or_sim* s, double* diag9);
+void    or_sys_fest(const or_sim* s, int32_t* flag, double* normale, double* vn_soll, double* v_koerper);
+void    or_sys_setze_fest(or_sim* s, const int32_t* flag, const double* normale, const double* vn_soll, const double* v_koerper);
+void    or_sys_mal_h(const or_sim* s, const double* p, double* out);
+void    or_sys_mal_a(const or_sim* s, const double* p, double* out);
+/* CpuLoeser::loese on the built system and contact set; dv in = warm start. */
+void    or_sys_loese(const or_sim* s, double* dv, int32_t* iterationen, int32_t* nicht_konvergiert, double* rest);
+void    or_sys_rest(const or_sim* s, const double* dv, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
