@@ -1,0 +1,107 @@
+"""ctypes binding of libgcsim.so (the C ABI declared in include/gcsim.h).
+
+Loading fails loudly: there is no CPU fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HIER = os.path.dirname(os.path.abspath(__file__))
+SO_PFAD = os.path.join(_HIER, "libgcsim.so")
+
+GC_FP64 = 0
+GC_FP32 = 1
+
+D = C.POINTER(C.c_double)
+U32 = C.POINTER(C.c_uint32)
+I32 = C.POINTER(C.c_int32)
+F32 = C.POINTER(C.c_float)
+
+
+class GcFehler(RuntimeError):
+    """An entry point returned a nonzero status."""
+
+
+class Optionen(C.Structure):
+    _fields_ = [("praezision", C.c_int32), ("geraet", C.c_int32), ("bloecke", C.c_int32), ("threads", C.c_int32)]
+
+
+class Topologie(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("masse", D), ("w", D), ("kollisionsabstand", D), ("reibung", D),
+        ("n_kanten", C.c_int32), ("kanten_punkte", U32), ("kanten_k", D), ("kanten_ruhe", D),
+        ("n_naehte", C.c_int32), ("naht_punkte", U32), ("naht_ruhe", D),
+        ("n_biegungen", C.c_int32), ("biegung_punkte", U32), ("biegung_k", D), ("biegung_ruhe", D),
+        ("n_membran", C.c_int32), ("membran_punkte", U32), ("membran_daten", D),
+    ]
+
+
+class Param(C.Structure):
+    _fields_ = [
+        ("schwerkraft", C.c_double * 3), ("daempfung", C.c_double), ("cg_max", C.c_int32),
+        ("cg_toleranz", C.c_double), ("naht_k", C.c_double), ("kontakt_rand", C.c_double),
+        ("normalkraft_aus_last", C.c_int32),
+    ]
+
+
+class CgLauf_c(C.Structure):
+    _fields_ = [("iterationen", C.c_int32), ("nicht_konvergiert", C.c_int32), ("rest", C.c_double)]
+
+
+class Statistik_c(C.Structure):
+    _fields_ = [
+        ("cg_iterationen", C.c_int64), ("loesungen", C.c_int64), ("nicht_konvergiert", C.c_int64),
+        ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
+    ]
+
+
+# (name, restype, argtypes) for every entry point of include/gcsim.h
+SIGNATUREN = [
+    ("gc_create", C.c_void_p, [C.POINTER(Optionen), C.c_char_p, C.c_int32]),
+    ("gc_destroy", None, [C.c_void_p]),
+    ("gc_last_error", C.c_char_p, [C.c_void_p]),
+    ("gc_name", C.c_char_p, [C.c_void_p]),
+    ("gc_set_topology", C.c_int, [C.c_void_p, C.POINTER(Topologie)]),
+    ("gc_set_param", C.c_int, [C.c_void_p, C.POINTER(Param)]),
+    ("gc_set_naht_ruhe", C.c_int, [C.c_void_p, D]),
+    ("gc_set_w", C.c_int, [C.c_void_p, D]),
+    ("gc_add_zylinder", C.c_int, [C.c_void_p, D, D, C.c_double, D]),
+    ("gc_add_sdf", C.c_int, [C.c_void_p, F32, I32, D, C.c_double, D, D]),
+    ("gc_set_koerper_lage", C.c_int, [C.c_void_p, C.c_int32, D]),
+    ("gc_set_state", C.c_int, [C.c_void_p, D, D]),
+    ("gc_get_state", C.c_int, [C.c_void_p, D, D]),
+    ("gc_set_dv_vorher", C.c_int, [C.c_void_p, D, C.c_int32]),
+    ("gc_build", C.c_int, [C.c_void_p, C.c_double]),
+    ("gc_set_fest", C.c_int, [C.c_void_p, I32, D, D, D]),
+    ("gc_get_fest", C.c_int, [C.c_void_p, I32, D, D, D]),
+    ("gc_solve", C.c_int, [C.c_void_p, D, C.POINTER(CgLauf_c)]),
+    ("gc_apply_A", C.c_int, [C.c_void_p, D, D]),
+    ("gc_apply_H", C.c_int, [C.c_void_p, D, D]),
+    ("gc_residual", C.c_int, [C.c_void_p, D, D]),
+    ("gc_read_load", C.c_int, [C.c_void_p, D]),
+    ("gc_read_system", C.c_int, [C.c_void_p, D, D, D, D]),
+    ("gc_read_records", C.c_int, [C.c_void_p, D, D, D]),
+    ("gc_step", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    ("gc_bild", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    ("gc_step_graph", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    ("gc_read_statistik", C.c_int, [C.c_void_p, C.POINTER(Statistik_c), C.c_int32]),
+    ("gc_zeiten", C.c_int, [C.c_void_p, D, D]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libgcsim.so (built by __graft_entry__.build() or `make -C garmentcraft_amd/csrc`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO_PFAD):
+            raise GcFehler(f"{SO_PFAD} is not built; run `make -C garmentcraft_amd/csrc`")
+        L = C.CDLL(SO_PFAD)
+        for name, res, args in SIGNATUREN:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

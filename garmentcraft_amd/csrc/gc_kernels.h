@@ -1,0 +1,622 @@
+// gc_kernels.h — device kernels of the implicit step for CDNA4 (gfx950).
+//
+// Layout: structure of arrays in HBM, one writer per constraint, one gather per
+// point over a CSR list of the constraints that touch it (no atomics on data).
+// The CSR list of a point holds its constraints by category — distance
+// constraints (edges with k != 0, then seams), bending, membranes — each in
+// constraint order, which is the order in which the reference's sequential
+// loops add into that point (implizit.rs:185-221, 232-279, 725-827). The FP64
+// build therefore reproduces the reference's per-point sums.
+#pragma once
+#include "gc_math.h"
+
+// CSR entry: constraint index << 2 | slot. Kante: slot 0 = first point (+), 1 = second (-).
+#define GC_KAT 4  // categories per point in the offset table: kanten, biege, membran, (end)
+
+template <typename T>
+struct SysDev {
+  int32_t n, nk, nb, nm;
+  double h, h2, c, naht_k, kontakt_rand;
+  double g[3];
+  int32_t cg_max;
+  double cg_tol2, boden;
+  int32_t aus_last;
+  // CSR
+  const uint32_t* off;   // 4 per point + 1
+  const uint32_t* csr;
+  // distance constraints (edges with k != 0 in input order, then seams)
+  const uint32_t* k_ij;
+  const double* k_k;
+  const double* k_ruhe;
+  const uint8_t* k_naht;
+  V3<T>* k_n;
+  T* k_q;
+  T* k_kk;
+  V3<double>* k_f;
+  uint8_t* k_zug;
+  // bending
+  const uint32_t* b_p;
+  const double* b_k;
+  const double* b_ruhe;
+  T* b_kt;
+  V3<T>* b_grad;  // 4 per constraint
+  double* b_kf;   // k * wrapped angle error
+  // membranes
+  const uint32_t* m_p;
+  const Membran<T>* m_el;         // operator records (per iteration)
+  const Membran<double>* m_el64;  // the same elements for the per-step build
+  Hessen<T>* m_h;
+  V3<double>* m_f;  // 3 per element: force
+  V3<double>* m_z;  // 3 per element: tension-only force
+  // points
+  const double* masse;
+  const double* w;
+  const double* koll;
+  const double* reib;
+  V3<double>* x;
+  V3<double>* v;
+  T* mdiag;
+  V3<double>* f;
+  V3<double>* last;
+  V3<T>* b;
+  T* pinv;  // 9 per point
+  // contacts: 0 free, 1 held along a normal, 2 held completely (w <= 0)
+  uint8_t* fflag;
+  V3<T>* fnorm;
+  V3<double>* fnorm64;
+  double* vn_soll;
+  V3<double>* vk;
+  double* fn_kraft;
+  // PCG
+  V3<T>* dv;
+  V3<T>* r;
+  V3<T>* z;
+  V3<T>* p[2];
+  V3<T>* ap;
+  V3<double>* dv_vorher;
+  int32_t hat_vorher;
+  // bodies
+  int32_t n_koerper;
+  Koerper koerper[GC_MAX_KOERPER];
+};
+
+// Shared device state of the grid-wide reductions and the step's counters.
+struct GridState {
+  unsigned int zaehler;      // barrier arrivals (monotonic within a launch)
+  unsigned int fehler;       // spin limit hit
+  unsigned int frei[2];      // a point was released in pass k
+  unsigned int n_fest;       // points held along a normal
+  unsigned int pad0[3];
+  // statistics of the last step (device side)
+  long long cg_iterationen, loesungen, nicht_konvergiert, kontakte, haftend, freigegeben;
+  double rest_max;
+  // result of the last solve
+  int iterationen;
+  int nicht_konv;
+  double rest;
+  double delta, ziel, bb;
+};
+
+#define GC_MAX_BLOECKE 1024
+struct alignas(64) Slot {
+  double v[4];
+};
+
+// ---------------------------------------------------------------------------
+// Grid-wide all-reduce: block partials into per-block slots, one arrival
+// counter, relaxed polling, then every block sums all slots in the same fixed
+// order (so every block holds the identical value — deterministic and equal to
+// the bit across blocks). Slots are double-buffered by epoch parity: a block
+// can only overwrite parity p after every block has published epoch e+1,
+// i.e. after everyone has read epoch e-1.
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
+  // sh: blockDim.x * NV doubles of LDS
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int k = 0; k < NV; ++k) sh[k * nt + t] = v[k];
+  __syncthreads();
+  for (int o = nt / 2; o > 0; o >>= 1) {
+    if (t < o)
+      for (int k = 0; k < NV; ++k) sh[k * nt + t] += sh[k * nt + t + o];
+    __syncthreads();
+  }
+  for (int k = 0; k < NV; ++k) v[k] = sh[k * nt];
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch) {
+  block_sum<NV>(v, sh);
+  ++epoch;
+  const unsigned nb = gridDim.x;
+  Slot* buf = slots + (epoch & 1u) * GC_MAX_BLOECKE;
+  __shared__ int s_fehler;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; ++k) buf[blockIdx.x].v[k] = v[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&gs->zaehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned ziel = nb * epoch;
+    unsigned spins = 0;
+    int fehler = 0;
+    while (__hip_atomic_load(&gs->zaehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ziel) {
+      if (++spins > (1u << 22)) { fehler = 1; break; }
+      if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { fehler = 1; break; }
+    }
+    if (fehler) __hip_atomic_store(&gs->fehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_fehler = fehler;
+  }
+  __syncthreads();
+  if (s_fehler) return false;
+  // every block sums the nb partials in the same order
+  double acc[NV];
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int k = 0; k < NV; ++k) acc[k] += __hip_atomic_load(&buf[b].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<NV>(acc, sh);
+  for (int k = 0; k < NV; ++k) v[k] = acc[k];
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Operator rows
+// ---------------------------------------------------------------------------
+// H*p at row i (implizit.rs:185-221), p given by a functor j -> V3<T>.
+template <typename T, typename PF>
+__device__ __forceinline__ V3<T> mal_h_zeile(const SysDev<T>& s, uint32_t i, PF p) {
+  V3<T> out = zero3<T>();
+  const uint32_t* o = s.off + GC_KAT * i;
+  const uint32_t e0 = o[0], e1 = o[1], e2 = o[2], e3 = o[3];
+  for (uint32_t e = e0; e < e1; ++e) {
+    const uint32_t ent = s.csr[e], c = ent >> 2;
+    const V3<T> d = p(s.k_ij[2 * c]) - p(s.k_ij[2 * c + 1]);
+    const V3<T> n = s.k_n[c];
+    const T q = s.k_q[c];
+    const V3<T> bd = (n * (dot(n, d) * (T(1) - q)) + d * q) * s.k_kk[c];
+    out = (ent & 3u) == 0u ? out + bd : out - bd;
+  }
+  for (uint32_t e = e1; e < e2; ++e) {
+    const uint32_t ent = s.csr[e], c = ent >> 2, m = ent & 3u;
+    const uint32_t* bp = s.b_p + 4 * c;
+    const V3<T>* gr = s.b_grad + 4 * c;
+    T gp = T(0);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) gp += dot(gr[l], p(bp[l]));
+    out = out + gr[m] * (s.b_kt[c] * gp);
+  }
+  for (uint32_t e = e2; e < e3; ++e) {
+    const uint32_t ent = s.csr[e], c = ent >> 2, k = ent & 3u;
+    const uint32_t* mp = s.m_p + 3 * c;
+    V3<T> d3[3] = {p(mp[0]), p(mp[1]), p(mp[2])};
+    out = out + mal_h_aus_k(s.m_el[c], s.m_h[c], d3, (int)k);
+  }
+  return out;
+}
+
+// A*p at row i (implizit.rs:224-229).
+template <typename T, typename PF>
+__device__ __forceinline__ V3<T> mal_a_zeile(const SysDev<T>& s, uint32_t i, PF p) {
+  V3<T> hp = mal_h_zeile(s, i, p);
+  return hp * T(s.h2) + p(i) * s.mdiag[i];
+}
+
+// implizit.rs:283-288 with the held-point rule of CpuLoeser::loese.
+template <typename T>
+__device__ __forceinline__ V3<T> filtern(const SysDev<T>& s, uint32_t i, V3<T> v) {
+  const uint8_t fl = s.fflag[i];
+  if (fl == 0) return v;
+  if (fl == 2) return zero3<T>();
+  const V3<T> n = s.fnorm[i];
+  return v - n * dot(v, n);
+}
+
+// ---------------------------------------------------------------------------
+// Build kernels (implizit.rs:725-844; device form of gpu.rs aufbau_k/aufbau_b)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_bau_kanten(SysDev<T> s) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.nk; c += gridDim.x * blockDim.x) {
+    const uint32_t i = s.k_ij[2 * c], j = s.k_ij[2 * c + 1];
+    const double k = s.k_k[c];
+    double l0 = s.k_ruhe[c];
+    const V3<double> d = s.x[j] - s.x[i];
+    V3<double> nr, fi;
+    double quer;
+    uint8_t zug = 0;
+    if (!s.k_naht[c]) {  // implizit.rs:742-752
+      const double l = gmax(norm(d), 1e-12);
+      nr = d * (1.0 / l);
+      fi = nr * (k * (l - l0));
+      zug = l > l0 ? 1 : 0;
+      quer = gmax(1.0 - l0 / l, 0.0);
+    } else {  // implizit.rs:763-771
+      const double l = norm(d);
+      l0 = isfinite(l0) ? l0 : l;
+      const bool lang = l0 > 0.0 && l > 1e-12;
+      fi = lang ? d * (k * (l - l0) / l) : d * k;
+      nr = lang ? d * (1.0 / l) : zero3<double>();
+      quer = lang ? gmax(1.0 - l0 / l, 0.0) : 1.0;
+    }
+    s.k_n[c] = cast3<T>(nr);
+    s.k_q[c] = T(quer);
+    s.k_kk[c] = T(k);
+    s.k_f[c] = fi;
+    s.k_zug[c] = zug;
+  }
+}
+
+// Bending geometry in FP64 in both modes (the angle and its gradients are a
+// per-step cost); only the gradients the operator streams are stored as T.
+template <typename T>
+__global__ void k_bau_biegung(SysDev<T> s) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.nb; c += gridDim.x * blockDim.x) {
+    const uint32_t* bp = s.b_p + 4 * c;
+    V3<double> q[4];
+    for (int m = 0; m < 4; ++m) q[m] = s.x[bp[m]];
+    double theta = 0.0;
+    V3<double> g[4];
+    if (!dieder(q, &theta, g)) {  // implizit.rs:780-788: degenerate keeps its slot with no stiffness
+      theta = 0.0;
+      for (int m = 0; m < 4; ++m) g[m] = zero3<double>();
+    }
+    const double k = s.b_k[c];
+    const double fehler = winkel_fehler(theta, s.b_ruhe[c]);
+    for (int m = 0; m < 4; ++m) s.b_grad[4 * c + m] = cast3<T>(g[m]);
+    s.b_kf[c] = k * fehler;
+    s.b_kt[c] = T(k);
+  }
+}
+
+// Membrane strain, forces and tangent in FP64 in both modes: the Green strain
+// 0.5*(F^T F - 1) cancels at small stretch, which in FP32 costs ~1e-4 relative
+// in the force. Only the tangent the operator streams is stored as T.
+template <typename T>
+__global__ void k_bau_membran(SysDev<T> s) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.nm; c += gridDim.x * blockDim.x) {
+    const uint32_t* mp = s.m_p + 3 * c;
+    const V3<double> x0 = s.x[mp[0]];
+    const V3<double> e1 = s.x[mp[1]] - x0, e2 = s.x[mp[2]] - x0;
+    const Membran<double> el = s.m_el64[c];
+    V3<double> f1, f2;
+    spalten(el, e1, e2, &f1, &f2);
+    V3<double> fk[3], zk[3];
+    krafte(el, f1, f2, fk, zk);
+    for (int k = 0; k < 3; ++k) {
+      s.m_f[3 * c + k] = fk[k];
+      s.m_z[3 * c + k] = zk[k];
+    }
+    Hessen<T> h;
+    h.f1 = cast3<T>(f1);
+    h.f2 = cast3<T>(f2);
+    h.w = cast3<T>(geo_w_aus(el, f1, f2));
+    s.m_h[c] = h;
+  }
+}
+
+// Contacts of the substep (implizit.rs:872-883) and the held-point flags.
+template <typename T>
+__global__ void k_fest(SysDev<T> s, GridState* gs) {
+  unsigned lokal = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    uint8_t fl = 0;
+    V3<double> nn = zero3<double>(), vkk = zero3<double>();
+    double soll = 0.0;
+    if (s.w[i] <= 0.0) {
+      fl = 2;
+    } else {
+      for (int b = 0; b < s.n_koerper; ++b) {
+        V3<double> nor, vk;
+        double tiefe;
+        if (kontakt(s.koerper[b], s.x[i], s.koll[i] + s.kontakt_rand, &nor, &tiefe, &vk)) {
+          const double t = tiefe - s.kontakt_rand;
+          fl = 1;
+          nn = nor;
+          soll = t / s.h + dot(vk, nor);
+          vkk = vk;
+        }
+      }
+    }
+    s.fflag[i] = fl;
+    s.fnorm[i] = cast3<T>(nn);
+    s.fnorm64[i] = nn;
+    s.vn_soll[i] = soll;
+    s.vk[i] = vkk;
+    s.fn_kraft[i] = 0.0;
+    if (fl == 1) ++lokal;
+  }
+  if (lokal) atomicAdd(&gs->n_fest, lokal);
+}
+
+// Per point: force, load, H*v, right-hand side, diagonal block and its inverse
+// (implizit.rs:722-723, 745-751, 767-768, 793-795, 819-825, 887-894; the device
+// form of gpu.rs sammeln).
+template <typename T>
+__global__ void k_sammeln(SysDev<T> s) {
+  for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < s.n; ii += gridDim.x * blockDim.x) {
+    const uint32_t i = (uint32_t)ii;
+    const double m = s.masse[i];
+    const V3<double> g = {s.g[0], s.g[1], s.g[2]};
+    V3<double> f = g * m - s.v[i] * (s.c * m);
+    V3<double> last = g * m;
+    const uint32_t* o = s.off + GC_KAT * i;
+    for (uint32_t e = o[0]; e < o[1]; ++e) {
+      const uint32_t ent = s.csr[e], c = ent >> 2;
+      const V3<double> fi = s.k_f[c];
+      if ((ent & 3u) == 0u) {
+        f = f + fi;
+        if (s.k_zug[c]) last = last + fi;
+      } else {
+        f = f - fi;
+        if (s.k_zug[c]) last = last - fi;
+      }
+    }
+    for (uint32_t e = o[1]; e < o[2]; ++e) {
+      const uint32_t ent = s.csr[e], c = ent >> 2, mm = ent & 3u;
+      f = f - cast3<double>(s.b_grad[4 * c + mm]) * s.b_kf[c];
+    }
+    for (uint32_t e = o[2]; e < o[3]; ++e) {
+      const uint32_t ent = s.csr[e], c = ent >> 2, k = ent & 3u;
+      f = f + s.m_f[3 * c + k];
+      last = last + s.m_z[3 * c + k];
+    }
+    const V3<T> hv = mal_h_zeile(s, i, [&](uint32_t j) { return cast3<T>(s.v[j]); });
+    const V3<double> rhs = (f - cast3<double>(hv) * s.h) * s.h;
+    s.f[i] = f;
+    s.last[i] = last;
+    s.b[i] = cast3<T>(rhs);
+    const T md = T(m * (1.0 + s.h * s.c));
+    s.mdiag[i] = md;
+    // diagonal block (implizit.rs:232-279)
+    T d[9] = {md, T(0), T(0), T(0), md, T(0), T(0), T(0), md};
+    const T h2 = T(s.h2);
+    for (uint32_t e = o[0]; e < o[1]; ++e) {
+      const uint32_t c = s.csr[e] >> 2;
+      const T kk = s.k_kk[c], q = s.k_q[c];
+      const V3<T> n = s.k_n[c];
+      const T sk = kk * h2 * q;
+      d[0] += sk; d[4] += sk; d[8] += sk;
+      const T sa = kk * h2 * (T(1) - q);
+      const T nv[3] = {n.x, n.y, n.z};
+      for (int a = 0; a < 3; ++a)
+        for (int bb = 0; bb < 3; ++bb) d[3 * a + bb] += sa * nv[a] * nv[bb];
+    }
+    for (uint32_t e = o[1]; e < o[2]; ++e) {
+      const uint32_t ent = s.csr[e], c = ent >> 2, mm = ent & 3u;
+      const V3<T> gr = s.b_grad[4 * c + mm];
+      const T sb = s.b_kt[c] * h2;
+      const T gv[3] = {gr.x, gr.y, gr.z};
+      for (int a = 0; a < 3; ++a)
+        for (int bb = 0; bb < 3; ++bb) d[3 * a + bb] += sb * gv[a] * gv[bb];
+    }
+    for (uint32_t e = o[2]; e < o[3]; ++e) {
+      const uint32_t ent = s.csr[e], c = ent >> 2, k = ent & 3u;
+      T bl[9];
+      diag_block_k(s.m_el[c], s.m_h[c], (int)k, bl);
+      for (int q = 0; q < 9; ++q) d[q] += h2 * bl[q];
+    }
+    T inv[9];
+    m3_inv(d, inv);
+    for (int q = 0; q < 9; ++q) s.pinv[9 * i + q] = inv[q];
+  }
+}
+
+// Per-step reset of the shared counters (a fill kernel, so a captured step
+// records no memset node).
+__global__ void k_reset(GridState* gs) {
+  if (threadIdx.x == 0) {
+    gs->zaehler = 0;
+    gs->fehler = 0;
+    gs->frei[0] = 0;
+    gs->frei[1] = 0;
+    gs->n_fest = 0;
+  }
+}
+
+// Warm start (implizit.rs:906): dv of the previous step, h*g on the first.
+template <typename T>
+__global__ void k_warm(SysDev<T> s) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    const V3<double> gh = V3<double>{s.g[0], s.g[1], s.g[2]} * s.h;
+    s.dv[i] = cast3<T>(s.hat_vorher ? s.dv_vorher[i] : gh);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent modified PCG (CpuLoeser::loese, implizit.rs:370-426).
+// One launch per solve; blocks own contiguous point ranges; two grid
+// all-reduces and one grid barrier per iteration.
+// durchgang 1 is the second solve after a release and returns at once
+// unless a point was released.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(1024) k_pcg(SysDev<T> s, GridState* gs, Slot* slots, int durchgang) {
+  extern __shared__ double sh[];  // blockDim.x * 2 doubles
+  if (durchgang == 1) {
+    if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  }
+  const int nb = gridDim.x;
+  const int per = (s.n + nb - 1) / nb;
+  const int lo = blockIdx.x * per;
+  const int hi = min(s.n, lo + per);
+  unsigned epoch = 0;
+
+  // start: held along the normal get the target normal velocity (implizit.rs:375-384)
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const uint8_t fl = s.fflag[i];
+    if (fl == 2) {
+      s.dv[i] = zero3<T>();
+    } else if (fl == 1) {
+      const V3<double> n = s.fnorm64[i];
+      const double soll = s.vn_soll[i] - dot(s.v[i], n);
+      const V3<T> dvf = filtern(s, i, s.dv[i]);
+      s.dv[i] = dvf + s.fnorm[i] * T(soll);
+    }
+  }
+  {
+    double nix[1] = {0.0};
+    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch)) return;
+  }
+  // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z)  (implizit.rs:385-391)
+  double part[2] = {0.0, 0.0};
+  const V3<T>* dvp = s.dv;
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return dvp[j]; });
+    const uint8_t fl = s.fflag[i];
+    const V3<T> bi = s.b[i];
+    const V3<T> ri = fl == 2 ? zero3<T>() : filtern(s, i, bi - ap);
+    const T* P = s.pinv + 9 * i;
+    if (fl != 2) {
+      const V3<T> fb = filtern(s, i, bi);
+      part[0] += (double)dot(fb, m3_mal(P, fb));
+    }
+    const V3<T> zi = m3_mal(P, ri);
+    const V3<T> cd = filtern(s, i, zi);
+    part[1] += (double)dot(ri, cd);
+    s.r[i] = ri;
+    s.z[i] = zi;
+  }
+  if (!grid_allreduce<2>(part, sh, slots, gs, epoch)) return;
+  const double bb = part[0];
+  double delta = part[1];
+  const double ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
+  int it = 0;
+  double beta = 0.0;
+  int cur = 0;
+  while (delta > ziel && it < s.cg_max) {
+    // new direction p = S(z + p_old * beta), held completely: 0 (implizit.rs:420-422)
+    V3<T>* pn = s.p[cur];
+    const V3<T>* po = s.p[cur ^ 1];
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      const V3<T> zz = it == 0 ? s.z[i] : s.z[i] + po[i] * T(beta);
+      pn[i] = s.fflag[i] == 2 ? zero3<T>() : filtern(s, i, zz);
+    }
+    {
+      double nix[1] = {0.0};
+      if (!grid_allreduce<1>(nix, sh, slots, gs, epoch)) return;
+    }
+    // ap = S(A p), ca = p . ap (implizit.rs:401-405)
+    double ca[1] = {0.0};
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return pn[j]; });
+      ap = s.fflag[i] == 2 ? zero3<T>() : filtern(s, i, ap);
+      s.ap[i] = ap;
+      ca[0] += (double)dot(pn[i], ap);
+    }
+    if (!grid_allreduce<1>(ca, sh, slots, gs, epoch)) return;
+    if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
+    const double alpha = delta / ca[0];
+    // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
+    double neu[1] = {0.0};
+    const T ta = T(alpha);
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      s.dv[i] = s.dv[i] + pn[i] * ta;
+      const V3<T> ri = s.r[i] - s.ap[i] * ta;
+      s.r[i] = ri;
+      const V3<T> zi = m3_mal(s.pinv + 9 * i, ri);
+      s.z[i] = zi;
+      neu[0] += (double)dot(ri, zi);
+    }
+    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch)) return;
+    beta = neu[0] / delta;
+    delta = neu[0];
+    cur ^= 1;
+    ++it;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    gs->iterationen = it;
+    gs->nicht_konv = delta > ziel ? 1 : 0;
+    gs->rest = sqrt(delta / gmax(bb, s.boden));
+    gs->delta = delta;
+    gs->ziel = ziel;
+    gs->bb = bb;
+    gs->loesungen += 1;
+    gs->cg_iterationen += it;
+    if (delta > ziel) gs->nicht_konvergiert += 1;
+    gs->rest_max = gmax(gs->rest_max, gs->rest);
+  }
+}
+
+// Release pass (implizit.rs:916-942): normal force from the residual
+// A*dv - b; a held point that pulls is released.
+template <typename T>
+__global__ void k_freigabe(SysDev<T> s, GridState* gs, int durchgang) {
+  if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  if (__hip_atomic_load(&gs->n_fest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) gs->zaehler = 0;
+  unsigned frei = 0, weg = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    if (s.fflag[i] != 1) continue;
+    const V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return s.dv[j]; });
+    const V3<double> rest = cast3<double>(ap) - cast3<double>(s.b[i]);
+    const double fn = dot(rest, s.fnorm64[i]) / s.h;
+    if (fn < 0.0) {
+      s.fflag[i] = 0;
+      ++frei;
+      ++weg;
+    } else {
+      s.fn_kraft[i] = fn;
+    }
+  }
+  if (frei) {
+    atomicAdd(&gs->frei[durchgang], frei);
+    atomicAdd((unsigned long long*)&gs->freigegeben, (unsigned long long)frei);
+    atomicSub(&gs->n_fest, weg);
+  }
+}
+
+// Friction, velocity and position update, push-out (implizit.rs:945-1013).
+template <typename T>
+__global__ void k_update(SysDev<T> s, GridState* gs) {
+  unsigned kont = 0, haft = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    const V3<double> dvi = cast3<double>(s.dv[i]);
+    s.dv_vorher[i] = dvi;
+    if (s.w[i] <= 0.0) continue;
+    V3<double> v = s.v[i] + dvi;
+    if (s.fflag[i] == 1) {
+      ++kont;
+      const V3<double> n = s.fnorm64[i];
+      const V3<double> rel = v - s.vk[i];
+      const V3<double> t = rel - n * dot(rel, n);
+      const double lt = norm(t);
+      const double normal = s.aus_last ? gmax(-dot(s.last[i], n), 0.0) : s.fn_kraft[i];
+      const double grenze = s.reib[i] * normal * s.h * s.w[i];
+      if (lt <= grenze) {
+        ++haft;
+        v = v - t;
+      } else if (lt > 0.0) {
+        v = v - t * (grenze / lt);
+      }
+    }
+    s.v[i] = v;
+    V3<double> x = s.x[i] + v * s.h;
+    for (int b = 0; b < s.n_koerper; ++b) {
+      V3<double> nor, vk;
+      double tiefe;
+      if (kontakt(s.koerper[b], x, s.koll[i], &nor, &tiefe, &vk)) {
+        x = x + nor * tiefe;
+        const double vn = dot(s.v[i] - vk, nor);
+        if (vn < 0.0) s.v[i] = s.v[i] - nor * vn;
+      }
+    }
+    s.x[i] = x;
+  }
+  if (kont) atomicAdd((unsigned long long*)&gs->kontakte, (unsigned long long)kont);
+  if (haft) atomicAdd((unsigned long long*)&gs->haftend, (unsigned long long)haft);
+}
+
+// Utility kernels for the Loeser-level entry points.
+template <typename T>
+__global__ void k_mal(SysDev<T> s, const V3<T>* p, V3<T>* out, int nur_h) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+    auto pf = [&](uint32_t j) { return p[j]; };
+    out[i] = nur_h ? mal_h_zeile(s, (uint32_t)i, pf) : mal_a_zeile(s, (uint32_t)i, pf);
+  }
+}
+
+__global__ void k_fest_zaehlen(const uint8_t* fl, int n, GridState* gs) {
+  unsigned lokal = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) lokal += fl[i] == 1 ? 1u : 0u;
+  if (lokal) atomicAdd(&gs->n_fest, lokal);
+}

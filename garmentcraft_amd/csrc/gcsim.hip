@@ -1,0 +1,905 @@
+// gcsim.hip — host runtime and C ABI of the MI355X implicit step.
+//
+// One context = one garment system resident in HBM. The host builds the
+// per-point CSR adjacency once per topology, then every substep is a fixed
+// sequence of kernel launches on one stream (capturable as a HIP graph):
+//   reset, build edges / bending / membranes, contacts, per-point gather,
+//   warm start, PCG, release, PCG (only after a release), release, update.
+#include "../../include/gcsim.h"
+#include "gc_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+const int BUILD_THREADS = 256;
+
+struct Koerperhost {
+  Koerper k;
+  float* werte_dev = nullptr;
+};
+
+}  // namespace
+
+struct gc_ctx {
+  gc_optionen opt{};
+  std::string err;
+  std::string name;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  // host copy of the topology
+  int n = 0;
+  std::vector<double> masse, w, koll, reib;
+  std::vector<uint32_t> k_ij;
+  std::vector<double> k_k, k_ruhe;
+  std::vector<uint8_t> k_naht;
+  std::vector<int32_t> k_quelle;  // device kante -> input edge index, or -(1 + seam index)
+  int n_kanten_in = 0, n_naehte = 0, nk = 0;
+  std::vector<uint32_t> b_p;
+  std::vector<double> b_k, b_ruhe;
+  std::vector<uint32_t> m_p;
+  std::vector<double> m_d;
+  double msum = 0.0;
+  gc_param param{};
+  std::vector<Koerperhost> koerper;
+  // device
+  std::vector<void*> allocs;
+  SysDev<double> sd{};
+  SysDev<float> sf{};
+  GridState* gs = nullptr;
+  Slot* slots = nullptr;
+  bool topo = false, hat_param = false, built = false;
+  bool hat_vorher = false;
+  int pcg_blocks = 0, pcg_threads = 256, pcg_max_blocks = 0;
+  // timing
+  std::vector<hipEvent_t> ev;
+  double ms_schritt = 0.0, ms_pcg = 0.0;
+  // graph of one substep
+  hipGraphExec_t graph = nullptr;
+  double graph_h = -1.0;
+};
+
+#define HIPCK(ctx, call)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+      return 1;                                                                             \
+    }                                                                                       \
+  } while (0)
+
+static int fehler(gc_ctx* c, const std::string& m) {
+  c->err = m;
+  return 1;
+}
+
+template <typename X>
+static X* dalloc(gc_ctx* c, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(1, count) * sizeof(X);
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMemset(p, 0, bytes) != hipSuccess) return nullptr;
+  c->allocs.push_back(p);
+  return (X*)p;
+}
+
+template <typename X>
+static int hoch(gc_ctx* c, X* dst, const X* src, size_t count) {
+  if (count == 0) return 0;
+  HIPCK(c, hipMemcpyAsync(dst, src, count * sizeof(X), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+static void frei_alles(gc_ctx* c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  for (auto& k : c->koerper)
+    if (k.werte_dev) (void)hipFree(k.werte_dev);
+}
+
+template <typename T> static SysDev<T>& sys(gc_ctx* c);
+template <> SysDev<double>& sys<double>(gc_ctx* c) { return c->sd; }
+template <> SysDev<float>& sys<float>(gc_ctx* c) { return c->sf; }
+
+static unsigned gitter(long count) {
+  long b = (count + BUILD_THREADS - 1) / BUILD_THREADS;
+  return (unsigned)std::max(1L, std::min(b, 8192L));
+}
+
+// ---------------------------------------------------------------------------
+// topology upload
+// ---------------------------------------------------------------------------
+template <typename T>
+static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const std::vector<uint32_t>& csr) {
+  SysDev<T>& s = sys<T>(c);
+  s = SysDev<T>{};
+  const int n = c->n, nk = c->nk, nb = (int)c->b_k.size(), nm = (int)(c->m_p.size() / 3);
+  s.n = n; s.nk = nk; s.nb = nb; s.nm = nm;
+  uint32_t* d_off = dalloc<uint32_t>(c, off.size());
+  uint32_t* d_csr = dalloc<uint32_t>(c, csr.size());
+  uint32_t* d_kij = dalloc<uint32_t>(c, 2 * (size_t)nk);
+  double* d_kk = dalloc<double>(c, nk);
+  double* d_kr = dalloc<double>(c, nk);
+  uint8_t* d_kn = dalloc<uint8_t>(c, nk);
+  uint32_t* d_bp = dalloc<uint32_t>(c, 4 * (size_t)nb);
+  double* d_bk = dalloc<double>(c, nb);
+  double* d_br = dalloc<double>(c, nb);
+  uint32_t* d_mp = dalloc<uint32_t>(c, 3 * (size_t)nm);
+  Membran<T>* d_mel = dalloc<Membran<T>>(c, nm);
+  Membran<double>* d_mel64 = sizeof(T) == sizeof(double) ? (Membran<double>*)d_mel : dalloc<Membran<double>>(c, nm);
+  double* d_masse = dalloc<double>(c, n);
+  double* d_w = dalloc<double>(c, n);
+  double* d_koll = dalloc<double>(c, n);
+  double* d_reib = dalloc<double>(c, n);
+  if (!d_off || !d_csr || !d_kij || !d_kk || !d_kr || !d_kn || !d_bp || !d_bk || !d_br || !d_mp || !d_mel || !d_mel64 || !d_masse ||
+      !d_w || !d_koll || !d_reib)
+    return fehler(c, "hipMalloc failed (topology)");
+  std::vector<Membran<T>> mel(nm);
+  std::vector<Membran<double>> mel64(nm);
+  for (int e = 0; e < nm; ++e) {
+    const double* d = &c->m_d[12 * (size_t)e];
+    Membran<double>& m = mel64[e];
+    m.flaeche = d[0];
+    for (int q = 0; q < 4; ++q) m.r[q] = d[1 + q];
+    m.kette[0] = d[5]; m.kette[1] = d[6];
+    m.schuss[0] = d[7]; m.schuss[1] = d[8];
+    m.ek = d[9]; m.es = d[10]; m.g = d[11];
+    Membran<T>& t = mel[e];
+    t.flaeche = T(m.flaeche);
+    for (int q = 0; q < 4; ++q) t.r[q] = T(m.r[q]);
+    t.kette[0] = T(m.kette[0]); t.kette[1] = T(m.kette[1]);
+    t.schuss[0] = T(m.schuss[0]); t.schuss[1] = T(m.schuss[1]);
+    t.ek = T(m.ek); t.es = T(m.es); t.g = T(m.g);
+  }
+  if (sizeof(T) != sizeof(double) && hoch(c, d_mel64, mel64.data(), mel64.size())) return 1;
+  if (hoch(c, d_off, off.data(), off.size()) || hoch(c, d_csr, csr.data(), csr.size()) ||
+      hoch(c, d_kij, c->k_ij.data(), c->k_ij.size()) || hoch(c, d_kk, c->k_k.data(), c->k_k.size()) ||
+      hoch(c, d_kr, c->k_ruhe.data(), c->k_ruhe.size()) || hoch(c, d_kn, c->k_naht.data(), c->k_naht.size()) ||
+      hoch(c, d_bp, c->b_p.data(), c->b_p.size()) || hoch(c, d_bk, c->b_k.data(), c->b_k.size()) ||
+      hoch(c, d_br, c->b_ruhe.data(), c->b_ruhe.size()) || hoch(c, d_mp, c->m_p.data(), c->m_p.size()) ||
+      hoch(c, d_mel, mel.data(), mel.size()) || hoch(c, d_masse, c->masse.data(), c->masse.size()) ||
+      hoch(c, d_w, c->w.data(), c->w.size()) || hoch(c, d_koll, c->koll.data(), c->koll.size()) ||
+      hoch(c, d_reib, c->reib.data(), c->reib.size()))
+    return 1;
+  s.off = d_off; s.csr = d_csr;
+  s.k_ij = d_kij; s.k_k = d_kk; s.k_ruhe = d_kr; s.k_naht = d_kn;
+  s.b_p = d_bp; s.b_k = d_bk; s.b_ruhe = d_br;
+  s.m_p = d_mp; s.m_el = d_mel; s.m_el64 = d_mel64;
+  s.masse = d_masse; s.w = d_w; s.koll = d_koll; s.reib = d_reib;
+  // per-step and state buffers
+  s.k_n = dalloc<V3<T>>(c, nk); s.k_q = dalloc<T>(c, nk); s.k_kk = dalloc<T>(c, nk);
+  s.k_f = dalloc<V3<double>>(c, nk); s.k_zug = dalloc<uint8_t>(c, nk);
+  s.b_kt = dalloc<T>(c, nb); s.b_grad = dalloc<V3<T>>(c, 4 * (size_t)nb); s.b_kf = dalloc<double>(c, nb);
+  s.m_h = dalloc<Hessen<T>>(c, nm); s.m_f = dalloc<V3<double>>(c, 3 * (size_t)nm); s.m_z = dalloc<V3<double>>(c, 3 * (size_t)nm);
+  s.x = dalloc<V3<double>>(c, n); s.v = dalloc<V3<double>>(c, n);
+  s.mdiag = dalloc<T>(c, n); s.f = dalloc<V3<double>>(c, n); s.last = dalloc<V3<double>>(c, n);
+  s.b = dalloc<V3<T>>(c, n); s.pinv = dalloc<T>(c, 9 * (size_t)n);
+  s.fflag = dalloc<uint8_t>(c, n); s.fnorm = dalloc<V3<T>>(c, n); s.fnorm64 = dalloc<V3<double>>(c, n);
+  s.vn_soll = dalloc<double>(c, n); s.vk = dalloc<V3<double>>(c, n); s.fn_kraft = dalloc<double>(c, n);
+  s.dv = dalloc<V3<T>>(c, n); s.r = dalloc<V3<T>>(c, n); s.z = dalloc<V3<T>>(c, n);
+  s.p[0] = dalloc<V3<T>>(c, n); s.p[1] = dalloc<V3<T>>(c, n); s.ap = dalloc<V3<T>>(c, n);
+  s.dv_vorher = dalloc<V3<double>>(c, n);
+  if (!s.k_n || !s.k_q || !s.k_kk || !s.k_f || !s.k_zug || !s.b_kt || !s.b_grad || !s.b_kf || !s.m_h || !s.m_f || !s.m_z ||
+      !s.x || !s.v || !s.mdiag || !s.f || !s.last || !s.b || !s.pinv || !s.fflag || !s.fnorm || !s.fnorm64 || !s.vn_soll ||
+      !s.vk || !s.fn_kraft || !s.dv || !s.r || !s.z || !s.p[0] || !s.p[1] || !s.ap || !s.dv_vorher)
+    return fehler(c, "hipMalloc failed (vectors)");
+  // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
+  int occ = 0;
+  const size_t lds = (size_t)c->pcg_threads * 2 * sizeof(double);
+  HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, lds));
+  c->pcg_max_blocks = std::min(GC_MAX_BLOECKE, std::max(1, occ) * c->cus);
+  int want = c->opt.bloecke > 0 ? c->opt.bloecke : (n + c->pcg_threads - 1) / c->pcg_threads;
+  want = std::min(want, 128);
+  c->pcg_blocks = std::max(1, std::min(want, c->pcg_max_blocks));
+  return 0;
+}
+
+template <typename T>
+static void param_setzen(gc_ctx* c) {
+  SysDev<T>& s = sys<T>(c);
+  for (int q = 0; q < 3; ++q) s.g[q] = c->param.schwerkraft[q];
+  s.c = c->param.daempfung;
+  s.cg_max = c->param.cg_max;
+  s.cg_tol2 = c->param.cg_toleranz * c->param.cg_toleranz;
+  s.naht_k = c->param.naht_k;
+  s.kontakt_rand = c->param.kontakt_rand;
+  s.aus_last = c->param.normalkraft_aus_last;
+  s.boden = c->msum * 1e-18;
+  s.hat_vorher = c->hat_vorher ? 1 : 0;
+  s.n_koerper = (int)c->koerper.size();
+  for (size_t b = 0; b < c->koerper.size() && b < GC_MAX_KOERPER; ++b) s.koerper[b] = c->koerper[b].k;
+}
+
+static void alle_param(gc_ctx* c) {
+  if (c->opt.praezision == GC_FP32) param_setzen<float>(c);
+  else param_setzen<double>(c);
+}
+
+// ---------------------------------------------------------------------------
+// launch sequences
+// ---------------------------------------------------------------------------
+template <typename T>
+static int launch_bau(gc_ctx* c, double h) {
+  SysDev<T>& s = sys<T>(c);
+  s.h = h;
+  s.h2 = h * h;
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->gs);
+  if (s.nk) hipLaunchKernelGGL(k_bau_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, s);
+  if (s.nb) hipLaunchKernelGGL(k_bau_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, s);
+  if (s.nm) hipLaunchKernelGGL(k_bau_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, s);
+  hipLaunchKernelGGL(k_fest<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs);
+  hipLaunchKernelGGL(k_sammeln<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int launch_pcg(gc_ctx* c, int durchgang) {
+  SysDev<T>& s = sys<T>(c);
+  const size_t lds = (size_t)c->pcg_threads * 2 * sizeof(double);
+  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), lds, c->stream, s, c->gs, c->slots, durchgang);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int launch_schritt(gc_ctx* c, double h, hipEvent_t e_pcg0, hipEvent_t e_pcg1) {
+  SysDev<T>& s = sys<T>(c);
+  if (launch_bau<T>(c, h)) return 1;
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s);
+  if (e_pcg0) HIPCK(c, hipEventRecord(e_pcg0, st));
+  if (launch_pcg<T>(c, 0)) return 1;
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs, 0);
+  if (launch_pcg<T>(c, 1)) return 1;
+  if (e_pcg1) HIPCK(c, hipEventRecord(e_pcg1, st));
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs, 1);
+  hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+static int pruefe_gitter(gc_ctx* c) {
+  GridState g;
+  HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (g.fehler) return fehler(c, "grid barrier wait exceeded its spin limit (blocks not co-resident?)");
+  return 0;
+}
+
+template <typename T>
+static int schritte(gc_ctx* c, double h, int n) {
+  if (!c->topo || !c->hat_param) return fehler(c, "gc_step: topology and parameters must be set first");
+  if ((int)c->ev.size() < 2 * n + 2) {
+    while ((int)c->ev.size() < 2 * n + 2) {
+      hipEvent_t e;
+      HIPCK(c, hipEventCreate(&e));
+      c->ev.push_back(e);
+    }
+  }
+  HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+  for (int k = 0; k < n; ++k) {
+    sys<T>(c).hat_vorher = c->hat_vorher ? 1 : 0;
+    if (launch_schritt<T>(c, h, c->ev[2 + 2 * k], c->ev[3 + 2 * k])) return 1;
+    c->hat_vorher = true;
+  }
+  HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCK(c, hipEventSynchronize(c->ev[1]));
+  float ms = 0.f;
+  HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->ms_schritt = ms;
+  double pcg = 0.0;
+  for (int k = 0; k < n; ++k) {
+    float m2 = 0.f;
+    HIPCK(c, hipEventElapsedTime(&m2, c->ev[2 + 2 * k], c->ev[3 + 2 * k]));
+    pcg += m2;
+  }
+  c->ms_pcg = pcg;
+  c->built = false;
+  return pruefe_gitter(c);
+}
+
+template <typename T>
+static int schritte_graph(gc_ctx* c, double h, int n) {
+  if (!c->topo || !c->hat_param) return fehler(c, "gc_step_graph: topology and parameters must be set first");
+  if (n <= 0) return 0;
+  int done = 0;
+  if (!c->hat_vorher) {  // the first substep starts from h*g; the captured one from dv_vorher
+    if (schritte<T>(c, h, 1)) return 1;
+    done = 1;
+    if (n == 1) return 0;
+  }
+  if (!c->graph || c->graph_h != h) {
+    if (c->graph) {
+      HIPCK(c, hipGraphExecDestroy(c->graph));
+      c->graph = nullptr;
+    }
+    sys<T>(c).hat_vorher = 1;
+    hipGraph_t g;
+    HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_schritt<T>(c, h, nullptr, nullptr);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc) return 1;
+    if (e2 != hipSuccess) return fehler(c, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return fehler(c, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    c->graph_h = h;
+  }
+  if (c->ev.size() < 2) {
+    while (c->ev.size() < 2) {
+      hipEvent_t e;
+      HIPCK(c, hipEventCreate(&e));
+      c->ev.push_back(e);
+    }
+  }
+  HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+  for (int k = done; k < n; ++k) HIPCK(c, hipGraphLaunch(c->graph, c->stream));
+  HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCK(c, hipEventSynchronize(c->ev[1]));
+  float ms = 0.f;
+  HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->ms_schritt = ms;
+  c->ms_pcg = std::nan("");
+  c->built = false;
+  return pruefe_gitter(c);
+}
+
+// ---------------------------------------------------------------------------
+// Loeser-level helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+static int hoch_vec(gc_ctx* c, V3<T>* dst, const double* src) {
+  std::vector<V3<T>> tmp(c->n);
+  for (int i = 0; i < c->n; ++i) tmp[i] = V3<T>{T(src[3 * i]), T(src[3 * i + 1]), T(src[3 * i + 2])};
+  if (hoch(c, dst, tmp.data(), tmp.size())) return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+template <typename T>
+static int runter_vec(gc_ctx* c, double* dst, const V3<T>* src, size_t count) {
+  std::vector<V3<T>> tmp(count);
+  if (count) HIPCK(c, hipMemcpyAsync(tmp.data(), src, count * sizeof(V3<T>), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < count; ++i) {
+    dst[3 * i] = (double)tmp[i].x;
+    dst[3 * i + 1] = (double)tmp[i].y;
+    dst[3 * i + 2] = (double)tmp[i].z;
+  }
+  return 0;
+}
+
+template <typename T>
+static int runter_skalar(gc_ctx* c, double* dst, const T* src, size_t count) {
+  std::vector<T> tmp(count);
+  if (count) HIPCK(c, hipMemcpyAsync(tmp.data(), src, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < count; ++i) dst[i] = (double)tmp[i];
+  return 0;
+}
+
+template <typename T>
+static int mal(gc_ctx* c, const double* p, double* out, int nur_h) {
+  if (!c->built) return fehler(c, "no system built (call gc_build first)");
+  SysDev<T>& s = sys<T>(c);
+  if (hoch_vec<T>(c, s.ap, p)) return 1;
+  hipLaunchKernelGGL(k_mal<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, s, (const V3<T>*)s.ap, s.r, nur_h);
+  HIPCK(c, hipGetLastError());
+  return runter_vec<T>(c, out, s.r, c->n);
+}
+
+template <typename T>
+static int loese(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
+  if (!c->built) return fehler(c, "no system built (call gc_build first)");
+  SysDev<T>& s = sys<T>(c);
+  if (hoch_vec<T>(c, s.dv, dv)) return 1;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, c->gs);
+  if (launch_pcg<T>(c, 0)) return 1;
+  GridState g;
+  HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
+  if (runter_vec<T>(c, dv, s.dv, c->n)) return 1;
+  if (g.fehler) return fehler(c, "grid barrier wait exceeded its spin limit");
+  if (lauf) {
+    lauf->iterationen = g.iterationen;
+    lauf->nicht_konvergiert = g.nicht_konv;
+    lauf->rest = g.rest;
+  }
+  return 0;
+}
+
+template <typename T>
+static int fest_setzen(gc_ctx* c, const int32_t* flag, const double* normale, const double* vn_soll, const double* vk) {
+  SysDev<T>& s = sys<T>(c);
+  const int n = c->n;
+  std::vector<uint8_t> fl(n);
+  std::vector<V3<T>> nt(n);
+  std::vector<V3<double>> nd(n), vv(n);
+  for (int i = 0; i < n; ++i) {
+    fl[i] = c->w[i] <= 0.0 ? 2 : (flag[i] ? 1 : 0);
+    nd[i] = V3<double>{normale[3 * i], normale[3 * i + 1], normale[3 * i + 2]};
+    nt[i] = cast3<T>(nd[i]);
+    vv[i] = V3<double>{vk[3 * i], vk[3 * i + 1], vk[3 * i + 2]};
+  }
+  if (hoch(c, s.fflag, fl.data(), n) || hoch(c, s.fnorm, nt.data(), n) || hoch(c, s.fnorm64, nd.data(), n) ||
+      hoch(c, s.vn_soll, vn_soll, n) || hoch(c, s.vk, vv.data(), n))
+    return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+template <typename T>
+static int fest_lesen(gc_ctx* c, int32_t* flag, double* normale, double* vn_soll, double* vk) {
+  SysDev<T>& s = sys<T>(c);
+  const int n = c->n;
+  std::vector<uint8_t> fl(n);
+  std::vector<V3<double>> nd(n), vv(n);
+  HIPCK(c, hipMemcpyAsync(fl.data(), s.fflag, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(nd.data(), s.fnorm64, n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(vv.data(), s.vk, n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  if (vn_soll) HIPCK(c, hipMemcpyAsync(vn_soll, s.vn_soll, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) {
+    if (flag) flag[i] = fl[i] == 1 ? 1 : 0;
+    if (normale) { normale[3 * i] = nd[i].x; normale[3 * i + 1] = nd[i].y; normale[3 * i + 2] = nd[i].z; }
+    if (vk) { vk[3 * i] = vv[i].x; vk[3 * i + 1] = vv[i].y; vk[3 * i + 2] = vv[i].z; }
+  }
+  return 0;
+}
+
+template <typename T>
+static int system_lesen(gc_ctx* c, double* rhs, double* f, double* last, double* pinv9) {
+  if (!c->built) return fehler(c, "no system built (call gc_build first)");
+  SysDev<T>& s = sys<T>(c);
+  const size_t n = c->n;
+  if (rhs && runter_vec<T>(c, rhs, s.b, n)) return 1;
+  if (f) HIPCK(c, hipMemcpyAsync(f, s.f, n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  if (last) HIPCK(c, hipMemcpyAsync(last, s.last, n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (pinv9 && runter_skalar<T>(c, pinv9, s.pinv, 9 * n)) return 1;
+  return 0;
+}
+
+template <typename T>
+static int records_lesen(gc_ctx* c, double* richtung, double* quer, double* grad) {
+  if (!c->built) return fehler(c, "no system built (call gc_build first)");
+  SysDev<T>& s = sys<T>(c);
+  if (richtung && runter_vec<T>(c, richtung, s.k_n, c->nk)) return 1;
+  if (quer && runter_skalar<T>(c, quer, s.k_q, c->nk)) return 1;
+  if (grad && runter_vec<T>(c, grad, s.b_grad, 4 * c->b_k.size())) return 1;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
+  auto melde = [&](const std::string& m) {
+    if (err && err_len > 0) {
+      std::snprintf(err, (size_t)err_len, "%s", m.c_str());
+    }
+  };
+  gc_optionen o{};
+  if (opt) o = *opt;
+  if (o.praezision != GC_FP64 && o.praezision != GC_FP32) {
+    melde("praezision must be GC_FP64 or GC_FP32");
+    return nullptr;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    melde("no HIP device");
+    return nullptr;
+  }
+  if (o.geraet < 0 || o.geraet >= ndev) {
+    melde("device ordinal out of range");
+    return nullptr;
+  }
+  if (hipSetDevice(o.geraet) != hipSuccess) {
+    melde("hipSetDevice failed");
+    return nullptr;
+  }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, o.geraet) != hipSuccess) {
+    melde("hipGetDeviceProperties failed");
+    return nullptr;
+  }
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+    melde(std::string("built for gfx950, device is ") + p.gcnArchName);
+    return nullptr;
+  }
+  gc_ctx* c = new gc_ctx();
+  c->opt = o;
+  c->cus = p.multiProcessorCount;
+  c->pcg_threads = o.threads > 0 ? o.threads : 256;
+  if (c->pcg_threads > 1024 || (c->pcg_threads & (c->pcg_threads - 1))) {
+    melde("threads must be a power of two <= 1024");
+    delete c;
+    return nullptr;
+  }
+  c->name = o.praezision == GC_FP32 ? "MI355X HIP (f32, f64 reductions)" : "MI355X HIP (f64)";
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    melde("hipStreamCreate failed");
+    delete c;
+    return nullptr;
+  }
+  c->gs = dalloc<GridState>(c, 1);
+  c->slots = dalloc<Slot>(c, 2 * GC_MAX_BLOECKE);
+  if (!c->gs || !c->slots) {
+    melde("hipMalloc failed");
+    frei_alles(c);
+    delete c;
+    return nullptr;
+  }
+  c->param.schwerkraft[2] = -9.81;
+  c->param.cg_max = 50;
+  c->param.cg_toleranz = 1e-5;
+  c->param.naht_k = 500.0;
+  c->param.kontakt_rand = 1e-4;
+  return c;
+}
+
+void gc_destroy(gc_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->opt.geraet);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  frei_alles(c);
+  for (auto e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* gc_last_error(const gc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* gc_name(const gc_ctx* c) { return c ? c->name.c_str() : ""; }
+
+int gc_set_topology(gc_ctx* c, const gc_topologie* t) {
+  if (!c || !t) return 1;
+  if (c->topo) return fehler(c, "topology already set (create a new context for a new garment)");
+  const int n = t->n;
+  if (n <= 0) return fehler(c, "n must be > 0");
+  auto pruefe = [&](const uint32_t* idx, size_t cnt) {
+    for (size_t q = 0; q < cnt; ++q)
+      if (idx[q] >= (uint32_t)n) return false;
+    return true;
+  };
+  if (!pruefe(t->kanten_punkte, 2 * (size_t)t->n_kanten) || !pruefe(t->naht_punkte, 2 * (size_t)t->n_naehte) ||
+      !pruefe(t->biegung_punkte, 4 * (size_t)t->n_biegungen) || !pruefe(t->membran_punkte, 3 * (size_t)t->n_membran))
+    return fehler(c, "point index out of range in topology");
+  c->n = n;
+  c->masse.assign(t->masse, t->masse + n);
+  c->w.assign(t->w, t->w + n);
+  c->koll.assign(t->kollisionsabstand, t->kollisionsabstand + n);
+  c->reib.assign(t->reibung, t->reibung + n);
+  c->msum = 0.0;
+  for (int i = 0; i < n; ++i) c->msum += c->masse[i];  // implizit.rs:397 (sequential sum)
+  // device distance list: edges with k != 0 (implizit.rs:190-192 skips k = 0), then seams
+  c->k_ij.clear(); c->k_k.clear(); c->k_ruhe.clear(); c->k_naht.clear(); c->k_quelle.clear();
+  c->n_kanten_in = t->n_kanten;
+  c->n_naehte = t->n_naehte;
+  for (int e = 0; e < t->n_kanten; ++e) {
+    if (t->kanten_k[e] == 0.0) continue;
+    c->k_ij.push_back(t->kanten_punkte[2 * e]);
+    c->k_ij.push_back(t->kanten_punkte[2 * e + 1]);
+    c->k_k.push_back(t->kanten_k[e]);
+    c->k_ruhe.push_back(t->kanten_ruhe[e]);
+    c->k_naht.push_back(0);
+    c->k_quelle.push_back(e);
+  }
+  for (int e = 0; e < t->n_naehte; ++e) {
+    c->k_ij.push_back(t->naht_punkte[2 * e]);
+    c->k_ij.push_back(t->naht_punkte[2 * e + 1]);
+    c->k_k.push_back(c->param.naht_k);
+    c->k_ruhe.push_back(t->naht_ruhe[e]);
+    c->k_naht.push_back(1);
+    c->k_quelle.push_back(-(1 + e));
+  }
+  c->nk = (int)c->k_k.size();
+  c->b_p.assign(t->biegung_punkte, t->biegung_punkte + 4 * (size_t)t->n_biegungen);
+  c->b_k.assign(t->biegung_k, t->biegung_k + t->n_biegungen);
+  c->b_ruhe.assign(t->biegung_ruhe, t->biegung_ruhe + t->n_biegungen);
+  c->m_p.assign(t->membran_punkte, t->membran_punkte + 3 * (size_t)t->n_membran);
+  c->m_d.assign(t->membran_daten, t->membran_daten + 12 * (size_t)t->n_membran);
+  // CSR by point and category, each category in constraint order
+  std::vector<uint32_t> cnt((size_t)GC_KAT * n, 0);
+  for (int k = 0; k < c->nk; ++k) {
+    cnt[GC_KAT * (size_t)c->k_ij[2 * k] + 0]++;
+    cnt[GC_KAT * (size_t)c->k_ij[2 * k + 1] + 0]++;
+  }
+  for (size_t q = 0; q < c->b_p.size(); ++q) cnt[GC_KAT * (size_t)c->b_p[q] + 1]++;
+  for (size_t q = 0; q < c->m_p.size(); ++q) cnt[GC_KAT * (size_t)c->m_p[q] + 2]++;
+  std::vector<uint32_t> off((size_t)GC_KAT * n + 1);
+  uint32_t run = 0;
+  for (size_t q = 0; q < (size_t)GC_KAT * n; ++q) {
+    off[q] = run;
+    run += cnt[q];
+  }
+  off[(size_t)GC_KAT * n] = run;
+  std::vector<uint32_t> csr(run);
+  std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+  for (int k = 0; k < c->nk; ++k) {
+    csr[cur[GC_KAT * (size_t)c->k_ij[2 * k] + 0]++] = ((uint32_t)k << 2) | 0u;
+    csr[cur[GC_KAT * (size_t)c->k_ij[2 * k + 1] + 0]++] = ((uint32_t)k << 2) | 1u;
+  }
+  for (size_t b = 0; b < c->b_k.size(); ++b)
+    for (uint32_t m = 0; m < 4; ++m) csr[cur[GC_KAT * (size_t)c->b_p[4 * b + m] + 1]++] = ((uint32_t)b << 2) | m;
+  for (size_t e = 0; e < c->m_p.size() / 3; ++e)
+    for (uint32_t k = 0; k < 3; ++k) csr[cur[GC_KAT * (size_t)c->m_p[3 * e + k] + 2]++] = ((uint32_t)e << 2) | k;
+  HIPCK(c, hipSetDevice(c->opt.geraet));
+  int rc = c->opt.praezision == GC_FP32 ? topologie_laden<float>(c, off, csr) : topologie_laden<double>(c, off, csr);
+  if (rc) return rc;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->topo = true;
+  alle_param(c);
+  return 0;
+}
+
+int gc_set_param(gc_ctx* c, const gc_param* p) {
+  if (!c || !p) return 1;
+  if (p->cg_max < 0) return fehler(c, "cg_max must be >= 0");
+  c->param = *p;
+  c->hat_param = true;
+  if (c->topo) {
+    // seam stiffness is the parameter naht_k (implizit.rs:760, 771)
+    for (int k = 0; k < c->nk; ++k)
+      if (c->k_naht[k]) c->k_k[k] = p->naht_k;
+    double* dk = const_cast<double*>(c->opt.praezision == GC_FP32 ? c->sf.k_k : c->sd.k_k);
+    if (hoch(c, dk, c->k_k.data(), c->k_k.size())) return 1;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+  }
+  alle_param(c);
+  return 0;
+}
+
+int gc_set_naht_ruhe(gc_ctx* c, const double* ruhe) {
+  if (!c || !ruhe || !c->topo) return 1;
+  for (int k = 0; k < c->nk; ++k)
+    if (c->k_naht[k]) c->k_ruhe[k] = ruhe[-(c->k_quelle[k] + 1)];
+  double* dr = const_cast<double*>(c->opt.praezision == GC_FP32 ? c->sf.k_ruhe : c->sd.k_ruhe);
+  if (hoch(c, dr, c->k_ruhe.data(), c->k_ruhe.size())) return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_set_w(gc_ctx* c, const double* w) {
+  if (!c || !w || !c->topo) return 1;
+  c->w.assign(w, w + c->n);
+  double* dw = const_cast<double*>(c->opt.praezision == GC_FP32 ? c->sf.w : c->sd.w);
+  if (hoch(c, dw, c->w.data(), c->w.size())) return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_add_zylinder(gc_ctx* c, const double punkt[3], const double achse[3], double radius, const double v[3]) {
+  if (!c) return -1;
+  if (c->koerper.size() >= GC_MAX_KOERPER) {
+    c->err = "too many bodies";
+    return -1;
+  }
+  Koerperhost kh;
+  std::memset(&kh.k, 0, sizeof(Koerper));
+  kh.k.art = GC_ZYLINDER;
+  for (int q = 0; q < 3; ++q) {
+    kh.k.punkt[q] = punkt[q];
+    kh.k.achse[q] = achse[q];
+    kh.k.geschwindigkeit[q] = v[q];
+  }
+  kh.k.radius = radius;
+  c->koerper.push_back(kh);
+  alle_param(c);
+  return (int)c->koerper.size() - 1;
+}
+
+int gc_add_sdf(gc_ctx* c, const float* werte, const int32_t dims[3], const double origin[3], double spacing,
+               const double versatz[3], const double v[3]) {
+  if (!c) return -1;
+  if (c->koerper.size() >= GC_MAX_KOERPER) {
+    c->err = "too many bodies";
+    return -1;
+  }
+  if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2 || !(spacing > 0.0)) {
+    c->err = "SDF grid needs at least 2 nodes per axis and a positive spacing";
+    return -1;
+  }
+  size_t cnt = (size_t)dims[0] * dims[1] * dims[2];
+  Koerperhost kh;
+  std::memset(&kh.k, 0, sizeof(Koerper));
+  if (hipMalloc(&kh.werte_dev, cnt * sizeof(float)) != hipSuccess) {
+    c->err = "hipMalloc failed (SDF)";
+    return -1;
+  }
+  if (hipMemcpy(kh.werte_dev, werte, cnt * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    c->err = "hipMemcpy failed (SDF)";
+    (void)hipFree(kh.werte_dev);
+    return -1;
+  }
+  kh.k.art = GC_SDF;
+  kh.k.werte = kh.werte_dev;
+  for (int q = 0; q < 3; ++q) {
+    kh.k.dims[q] = dims[q];
+    kh.k.origin[q] = origin[q];
+    kh.k.punkt[q] = versatz[q];
+    kh.k.geschwindigkeit[q] = v[q];
+  }
+  kh.k.spacing = spacing;
+  c->koerper.push_back(kh);
+  alle_param(c);
+  return (int)c->koerper.size() - 1;
+}
+
+int gc_set_koerper_lage(gc_ctx* c, int32_t k, const double lage[3]) {
+  if (!c || k < 0 || k >= (int)c->koerper.size()) return 1;
+  for (int q = 0; q < 3; ++q) c->koerper[k].k.punkt[q] = lage[q];
+  alle_param(c);
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return 0;
+}
+
+int gc_set_state(gc_ctx* c, const double* x, const double* v) {
+  if (!c || !c->topo) return 1;
+  V3<double>* dx = c->opt.praezision == GC_FP32 ? c->sf.x : c->sd.x;
+  V3<double>* dv = c->opt.praezision == GC_FP32 ? c->sf.v : c->sd.v;
+  if (x && hoch(c, dx, (const V3<double>*)x, c->n)) return 1;
+  if (v && hoch(c, dv, (const V3<double>*)v, c->n)) return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->built = false;
+  return 0;
+}
+
+int gc_get_state(gc_ctx* c, double* x, double* v) {
+  if (!c || !c->topo) return 1;
+  V3<double>* dx = c->opt.praezision == GC_FP32 ? c->sf.x : c->sd.x;
+  V3<double>* dv = c->opt.praezision == GC_FP32 ? c->sf.v : c->sd.v;
+  if (x) HIPCK(c, hipMemcpyAsync(x, dx, c->n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  if (v) HIPCK(c, hipMemcpyAsync(v, dv, c->n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_set_dv_vorher(gc_ctx* c, const double* dv, int32_t n) {
+  if (!c || !c->topo) return 1;
+  if (n == 0) {
+    c->hat_vorher = false;
+    alle_param(c);
+    return 0;
+  }
+  if (n != c->n) return fehler(c, "gc_set_dv_vorher: length must be n or 0");
+  V3<double>* d = c->opt.praezision == GC_FP32 ? c->sf.dv_vorher : c->sd.dv_vorher;
+  if (hoch(c, d, (const V3<double>*)dv, c->n)) return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->hat_vorher = true;
+  alle_param(c);
+  return 0;
+}
+
+int gc_build(gc_ctx* c, double h) {
+  if (!c || !c->topo || !c->hat_param) return c ? fehler(c, "gc_build: topology and parameters must be set first") : 1;
+  if (!(h > 0.0)) return fehler(c, "h must be > 0");
+  int rc = c->opt.praezision == GC_FP32 ? launch_bau<float>(c, h) : launch_bau<double>(c, h);
+  if (rc) return rc;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->built = true;
+  return 0;
+}
+
+int gc_set_fest(gc_ctx* c, const int32_t* flag, const double* normale, const double* vn_soll, const double* vk) {
+  if (!c || !c->topo || !flag || !normale || !vn_soll || !vk) return 1;
+  return c->opt.praezision == GC_FP32 ? fest_setzen<float>(c, flag, normale, vn_soll, vk) : fest_setzen<double>(c, flag, normale, vn_soll, vk);
+}
+
+int gc_get_fest(gc_ctx* c, int32_t* flag, double* normale, double* vn_soll, double* vk) {
+  if (!c || !c->topo) return 1;
+  return c->opt.praezision == GC_FP32 ? fest_lesen<float>(c, flag, normale, vn_soll, vk) : fest_lesen<double>(c, flag, normale, vn_soll, vk);
+}
+
+int gc_solve(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
+  if (!c || !dv) return 1;
+  return c->opt.praezision == GC_FP32 ? loese<float>(c, dv, lauf) : loese<double>(c, dv, lauf);
+}
+
+int gc_apply_A(gc_ctx* c, const double* p, double* out) {
+  if (!c || !p || !out) return 1;
+  return c->opt.praezision == GC_FP32 ? mal<float>(c, p, out, 0) : mal<double>(c, p, out, 0);
+}
+
+int gc_apply_H(gc_ctx* c, const double* p, double* out) {
+  if (!c || !p || !out) return 1;
+  return c->opt.praezision == GC_FP32 ? mal<float>(c, p, out, 1) : mal<double>(c, p, out, 1);
+}
+
+int gc_residual(gc_ctx* c, const double* dv, double* out) {
+  if (!c || !dv || !out) return 1;
+  if (gc_apply_A(c, dv, out)) return 1;
+  std::vector<double> b(3 * (size_t)c->n);
+  if (gc_read_system(c, b.data(), nullptr, nullptr, nullptr)) return 1;
+  for (size_t q = 0; q < b.size(); ++q) out[q] = out[q] - b[q];
+  return 0;
+}
+
+int gc_read_load(gc_ctx* c, double* out) {
+  if (!c || !out) return 1;
+  return gc_read_system(c, nullptr, nullptr, out, nullptr);
+}
+
+int gc_read_system(gc_ctx* c, double* rhs, double* f, double* last, double* pinv9) {
+  if (!c) return 1;
+  return c->opt.praezision == GC_FP32 ? system_lesen<float>(c, rhs, f, last, pinv9) : system_lesen<double>(c, rhs, f, last, pinv9);
+}
+
+int gc_read_records(gc_ctx* c, double* richtung, double* quer, double* grad) {
+  if (!c) return 1;
+  return c->opt.praezision == GC_FP32 ? records_lesen<float>(c, richtung, quer, grad) : records_lesen<double>(c, richtung, quer, grad);
+}
+
+int gc_step(gc_ctx* c, double h, int32_t n) {
+  if (!c) return 1;
+  if (!(h > 0.0) || n < 0) return fehler(c, "gc_step: h must be > 0 and n >= 0");
+  return c->opt.praezision == GC_FP32 ? schritte<float>(c, h, n) : schritte<double>(c, h, n);
+}
+
+int gc_step_graph(gc_ctx* c, double h, int32_t n) {
+  if (!c) return 1;
+  if (!(h > 0.0) || n < 0) return fehler(c, "gc_step_graph: h must be > 0 and n >= 0");
+  return c->opt.praezision == GC_FP32 ? schritte_graph<float>(c, h, n) : schritte_graph<double>(c, h, n);
+}
+
+int gc_bild(gc_ctx* c, double bild_dt, int32_t schritte_je_bild) {
+  if (!c) return 1;
+  const int n = std::max(1, schritte_je_bild);
+  const double h = bild_dt / (double)n;
+  std::vector<Koerper> anfang;
+  for (auto& k : c->koerper) anfang.push_back(k.k);
+  for (int k = 0; k < n; ++k) {
+    const double weg = (double)(k + 1) * h;
+    for (size_t b = 0; b < c->koerper.size(); ++b)
+      for (int q = 0; q < 3; ++q) c->koerper[b].k.punkt[q] = anfang[b].punkt[q] + anfang[b].geschwindigkeit[q] * weg;
+    alle_param(c);
+    if (gc_step(c, h, 1)) return 1;
+  }
+  return 0;
+}
+
+int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
+  if (!c) return 1;
+  GridState g;
+  HIPCK(c, hipMemcpy(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost));
+  if (out) {
+    out->cg_iterationen = g.cg_iterationen;
+    out->loesungen = g.loesungen;
+    out->nicht_konvergiert = g.nicht_konvergiert;
+    out->kontakte = g.kontakte;
+    out->haftend = g.haftend;
+    out->freigegeben = g.freigegeben;
+    out->rest_max = g.rest_max;
+  }
+  if (reset) {
+    g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
+    g.rest_max = 0.0;
+    HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int gc_zeiten(gc_ctx* c, double* ms_schritt, double* ms_pcg) {
+  if (!c) return 1;
+  if (ms_schritt) *ms_schritt = c->ms_schritt;
+  if (ms_pcg) *ms_pcg = c->ms_pcg;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+"""A garment system resident on the MI355X: the Python face of one gc_ctx.
+
+:class:`GeraetSzene` uploads a :class:`~garmentcraft_amd.szene.Szene` once
+(topology, stiffnesses, rest lengths, bodies, state) and then steps it on the
+device. The Loeser-level probes (build, solve, operator, residual, load) are
+exposed for the parity checks against the reference's solver contract
+(implizit.rs:304-351).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .szene import GitterSdf, Szene, Zylinder
+
+
+@dataclass
+class ImplizitParameter:
+    """implizit.rs:456-531 (the fields this path reads)."""
+    schritte_je_bild: int = 5
+    cg_max: int = 50
+    cg_toleranz: float = 1e-5
+    naht_k: float = 500.0
+    kontakt_rand: float = 1e-4
+    normalkraft_aus_last: bool = False
+
+
+@dataclass
+class CgLauf:
+    """implizit.rs:293-299."""
+    iterationen: int
+    nicht_konvergiert: bool
+    rest: float
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t=L.D):
+    return a.ctypes.data_as(t)
+
+
+class GeraetSzene:
+    """One scene on one device. ``praezision``: "fp64" or "fp32" (FP32 bulk,
+    FP64 reductions)."""
+
+    def __init__(self, s: Szene, param: Optional[ImplizitParameter] = None, praezision: str = "fp64",
+                 bloecke: int = 0, threads: int = 0, geraet: int = 0):
+        self.lib = L.lib()
+        opt = L.Optionen(L.GC_FP32 if praezision == "fp32" else L.GC_FP64, geraet, bloecke, threads)
+        err = C.create_string_buffer(512)
+        h = self.lib.gc_create(C.byref(opt), err, 512)
+        if not h:
+            raise L.GcFehler("gc_create: " + err.value.decode())
+        self.h = C.c_void_p(h)
+        self.n = s.n
+        self.praezision = praezision
+        self.param = param or ImplizitParameter()
+        self.bild_dt = s.param.bild_dt
+        self._szene_param = s.param
+        self.set_param(self.param)
+        self._topologie(s)
+        for kb in s.koerper:
+            self.add_koerper(kb)
+        self.set_state(s.x, s.v)
+
+    # -- plumbing -----------------------------------------------------------
+    def _ck(self, rc: int, was: str) -> None:
+        if rc != 0:
+            raise L.GcFehler(f"{was}: {self.lib.gc_last_error(self.h).decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None:
+            self.lib.gc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def name(self) -> str:
+        return self.lib.gc_name(self.h).decode()
+
+    # -- set-up -------------------------------------------------------------
+    def _topologie(self, s: Szene) -> None:
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        k = np.where(s.membran_kante, 0.0, 1.0 / s.abstand_nachgiebigkeit)  # implizit.rs:732-736
+        bk = 1.0 / s.biegung_nachgiebigkeit                                   # implizit.rs:777
+        t = L.Topologie()
+        t.n = s.n
+        t.masse = _p(arr(s.masse, np.float64)); t.w = _p(arr(s.w, np.float64))
+        t.kollisionsabstand = _p(arr(s.kollisionsabstand, np.float64)); t.reibung = _p(arr(s.reibung, np.float64))
+        t.n_kanten = s.abstand_punkte.shape[0]
+        t.kanten_punkte = _p(arr(s.abstand_punkte, np.uint32), L.U32)
+        t.kanten_k = _p(arr(k, np.float64)); t.kanten_ruhe = _p(arr(s.abstand_ruhe, np.float64))
+        t.n_naehte = s.naht_punkte.shape[0]
+        t.naht_punkte = _p(arr(s.naht_punkte, np.uint32), L.U32); t.naht_ruhe = _p(arr(s.naht_ruhe, np.float64))
+        t.n_biegungen = s.biegung_punkte.shape[0]
+        t.biegung_punkte = _p(arr(s.biegung_punkte, np.uint32), L.U32)
+        t.biegung_k = _p(arr(bk, np.float64)); t.biegung_ruhe = _p(arr(s.biegung_ruhe, np.float64))
+        t.n_membran = s.membran_punkte.shape[0]
+        t.membran_punkte = _p(arr(s.membran_punkte, np.uint32), L.U32)
+        t.membran_daten = _p(arr(s.membran_daten, np.float64))
+        self._ck(self.lib.gc_set_topology(self.h, C.byref(t)), "gc_set_topology")
+
+    def set_param(self, p: ImplizitParameter) -> None:
+        self.param = p
+        c = L.Param()
+        for q in range(3):
+            c.schwerkraft[q] = self._szene_param.schwerkraft[q]
+        c.daempfung = self._szene_param.daempfung
+        c.cg_max = p.cg_max
+        c.cg_toleranz = p.cg_toleranz
+        c.naht_k = p.naht_k
+        c.kontakt_rand = p.kontakt_rand
+        c.normalkraft_aus_last = 1 if p.normalkraft_aus_last else 0
+        self._ck(self.lib.gc_set_param(self.h, C.byref(c)), "gc_set_param")
+
+    def add_koerper(self, kb) -> int:
+        if isinstance(kb, Zylinder):
+            i = self.lib.gc_add_zylinder(self.h, _p(_f64(kb.punkt)), _p(_f64(kb.achse)), kb.radius, _p(_f64(kb.geschwindigkeit)))
+        elif isinstance(kb, GitterSdf):
+            w = np.ascontiguousarray(kb.werte, dtype=np.float32)
+            dims = np.array(kb.dims, dtype=np.int32)
+            i = self.lib.gc_add_sdf(self.h, _p(w, L.F32), _p(dims, L.I32), _p(_f64(kb.origin)), kb.spacing,
+                                    _p(_f64(kb.versatz)), _p(_f64(kb.geschwindigkeit)))
+        else:
+            raise TypeError(f"unknown body {type(kb)}")
+        if i < 0:
+            raise L.GcFehler("add body: " + self.lib.gc_last_error(self.h).decode())
+        return i
+
+    def set_state(self, x=None, v=None) -> None:
+        self._ck(self.lib.gc_set_state(self.h, _p(_f64(x)) if x is not None else None,
+                                       _p(_f64(v)) if v is not None else None), "gc_set_state")
+
+    def lage(self):
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_get_state(self.h, _p(x), _p(v)), "gc_get_state")
+        return x, v
+
+    def set_naht_ruhe(self, r) -> None:
+        self._ck(self.lib.gc_set_naht_ruhe(self.h, _p(_f64(r))), "gc_set_naht_ruhe")
+
+    def set_w(self, w) -> None:
+        self._ck(self.lib.gc_set_w(self.h, _p(_f64(w))), "gc_set_w")
+
+    def set_dv_vorher(self, dv) -> None:
+        if dv is None:
+            self._ck(self.lib.gc_set_dv_vorher(self.h, None, 0), "gc_set_dv_vorher")
+        else:
+            d = _f64(dv)
+            self._ck(self.lib.gc_set_dv_vorher(self.h, _p(d), d.shape[0]), "gc_set_dv_vorher")
+
+    def set_koerper_lage(self, k: int, lage) -> None:
+        self._ck(self.lib.gc_set_koerper_lage(self.h, k, _p(_f64(lage))), "gc_set_koerper_lage")
+
+    # -- stepping -----------------------------------------------------------
+    def schritte(self, h: float, n: int = 1) -> None:
+        self._ck(self.lib.gc_step(self.h, h, n), "gc_step")
+
+    def schritte_graph(self, h: float, n: int) -> None:
+        self._ck(self.lib.gc_step_graph(self.h, h, n), "gc_step_graph")
+
+    def bild(self, bild_dt: Optional[float] = None, schritte_je_bild: Optional[int] = None) -> None:
+        dt = self.bild_dt if bild_dt is None else bild_dt
+        n = self.param.schritte_je_bild if schritte_je_bild is None else schritte_je_bild
+        self._ck(self.lib.gc_bild(self.h, dt, n), "gc_bild")
+
+    def statistik(self, reset: bool = False) -> dict:
+        st = L.Statistik_c()
+        self._ck(self.lib.gc_read_statistik(self.h, C.byref(st), 1 if reset else 0), "gc_read_statistik")
+        return {f: getattr(st, f) for f, _ in L.Statistik_c._fields_}
+
+    def zeiten(self):
+        a = C.c_double(); b = C.c_double()
+        self._ck(self.lib.gc_zeiten(self.h, C.byref(a), C.byref(b)), "gc_zeiten")
+        return a.value, b.value
+
+    # -- Loeser-level probes ------------------------------------------------
+    def bauen(self, h: float) -> None:
+        self._ck(self.lib.gc_build(self.h, h), "gc_build")
+
+    def fest(self):
+        n = self.n
+        fl = np.zeros(n, np.int32); nr = np.zeros((n, 3)); vn = np.zeros(n); vk = np.zeros((n, 3))
+        self._ck(self.lib.gc_get_fest(self.h, _p(fl, L.I32), _p(nr), _p(vn), _p(vk)), "gc_get_fest")
+        return {"flag": fl, "normale": nr, "vn_soll": vn, "v_koerper": vk}
+
+    def setze_fest(self, flag, normale, vn_soll, v_koerper) -> None:
+        self._ck(self.lib.gc_set_fest(self.h, _p(np.ascontiguousarray(flag, np.int32), L.I32), _p(_f64(normale)),
+                                      _p(_f64(vn_soll)), _p(_f64(v_koerper))), "gc_set_fest")
+
+    def loese(self, dv):
+        d = _f64(dv).copy()
+        lauf = L.CgLauf_c()
+        self._ck(self.lib.gc_solve(self.h, _p(d), C.byref(lauf)), "gc_solve")
+        return d, CgLauf(lauf.iterationen, bool(lauf.nicht_konvergiert), lauf.rest)
+
+    def mal_a(self, p):
+        o = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_apply_A(self.h, _p(_f64(p)), _p(o)), "gc_apply_A")
+        return o
+
+    def mal_h(self, p):
+        o = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_apply_H(self.h, _p(_f64(p)), _p(o)), "gc_apply_H")
+        return o
+
+    def rest(self, dv):
+        o = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_residual(self.h, _p(_f64(dv)), _p(o)), "gc_residual")
+        return o
+
+    def last(self):
+        o = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_read_load(self.h, _p(o)), "gc_read_load")
+        return o
+
+    def system(self):
+        n = self.n
+        rhs, f, last, pinv = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3, 3))
+        self._ck(self.lib.gc_read_system(self.h, _p(rhs), _p(f), _p(last), _p(pinv)), "gc_read_system")
+        return {"rhs": rhs, "f": f, "last": last, "pinv": pinv}
+
+    def records(self, n_kanten_geraet: int, n_biegungen: int):
+        r = np.zeros((n_kanten_geraet, 3)); q = np.zeros(n_kanten_geraet); g = np.zeros((n_biegungen, 4, 3))
+        self._ck(self.lib.gc_read_records(self.h, _p(r), _p(q), _p(g)), "gc_read_records")
+        return {"richtung": r, "quer": q, "grad": g}

@@ -1,0 +1,187 @@
+/*
+ * gcsim — MI355X-native implicit backward-Euler cloth step (C ABI).
+ *
+ * Drop-in for klon3d-sim's implicit path. The reference plugs solvers into
+ * `ImplizitCg` through the trait `Loeser` (crates/klon3d-sim/src/implizit.rs:304-351)
+ * and steps frames through `Rechenweg::bild` (crates/klon3d-sim/src/cpu.rs:58-60,
+ * implemented at implizit.rs:1042-1057). Each entry point below names the
+ * reference item it replaces. All paths are relative to the reference checkout.
+ *
+ * Conventions
+ *   - Every function returns int status: 0 = ok, nonzero = error; the message is
+ *     available from gc_last_error(ctx). Non-convergence is not an error: it is
+ *     reported in gc_cg_lauf, as the reference reports it in `CgLauf`
+ *     (implizit.rs:293-299).
+ *   - Point arrays are caller-owned, interleaved double[3*n] like Vec<[f64;3]>.
+ *   - One context is not thread-safe (the reference takes &mut self); several
+ *     contexts may live side by side. A context owns its device buffers and
+ *     keeps the topology resident between calls (the reference GPU backend does
+ *     the same, gpu.rs:1356-1456).
+ *   - No torch or HIP types cross this boundary.
+ */
+#ifndef GCSIM_H
+#define GCSIM_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gc_ctx gc_ctx;
+
+/* Arithmetic of the bulk kernels. Reductions, residual norms and the
+ * convergence test are FP64 in both modes. */
+enum { GC_FP64 = 0, GC_FP32 = 1 };
+
+typedef struct gc_optionen {
+  int32_t praezision;     /* GC_FP64 or GC_FP32 */
+  int32_t geraet;         /* HIP device ordinal */
+  int32_t bloecke;        /* persistent PCG blocks; 0 = choose (autotuned table) */
+  int32_t threads;        /* threads per block; 0 = choose */
+} gc_optionen;
+
+/* Topology of the system, as `System` carries it when the backend builds the
+ * geometry itself (`System::ohne_geometrie`, implizit.rs:123-172, filled at
+ * implizit.rs:718-797): per distance constraint the two points, the
+ * stiffness (0 = carried by a membrane) and the rest length; per seam spring
+ * the two points and the rest length (stiffness = gc_param.naht_k, rest +inf =
+ * open, implizit.rs:757-765); per bending constraint four points, stiffness and
+ * rest angle; per membrane triangle three points and the 12-value element
+ * {A0, ruhe_inv[4], kette[2], schuss[2], E_k, E_s, G} (kontinuum.rs:69-99). */
+typedef struct gc_topologie {
+  int32_t n;
+  const double* masse;               /* n */
+  const double* w;                   /* n, inverse mass; <= 0 means held */
+  const double* kollisionsabstand;   /* n */
+  const double* reibung;             /* n */
+  int32_t n_kanten;
+  const uint32_t* kanten_punkte;     /* 2 per edge */
+  const double* kanten_k;
+  const double* kanten_ruhe;
+  int32_t n_naehte;
+  const uint32_t* naht_punkte;       /* 2 per seam spring */
+  const double* naht_ruhe;
+  int32_t n_biegungen;
+  const uint32_t* biegung_punkte;    /* 4 per constraint */
+  const double* biegung_k;
+  const double* biegung_ruhe;
+  int32_t n_membran;
+  const uint32_t* membran_punkte;    /* 3 per element */
+  const double* membran_daten;       /* 12 per element */
+} gc_topologie;
+
+/* ImplizitParameter (implizit.rs:456-531) plus the scene's Simparameter
+ * (szene.rs:95-126) that the step reads. */
+typedef struct gc_param {
+  double schwerkraft[3];
+  double daempfung;          /* air damping c (1/s) */
+  int32_t cg_max;
+  double cg_toleranz;
+  double naht_k;
+  double kontakt_rand;
+  int32_t normalkraft_aus_last;
+} gc_param;
+
+/* CgLauf (implizit.rs:293-299). */
+typedef struct gc_cg_lauf {
+  int32_t iterationen;
+  int32_t nicht_konvergiert;
+  double rest;
+} gc_cg_lauf;
+
+/* ImplizitStatistik (implizit.rs:534-561), the counters this path keeps. */
+typedef struct gc_statistik {
+  int64_t cg_iterationen;
+  int64_t loesungen;
+  int64_t nicht_konvergiert;
+  int64_t kontakte;
+  int64_t haftend;
+  int64_t freigegeben;
+  double rest_max;
+} gc_statistik;
+
+/* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):
+ * NULL on failure with the reason in err (if given). */
+gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len);
+void gc_destroy(gc_ctx* ctx);
+const char* gc_last_error(const gc_ctx* ctx);
+/* `Loeser::name` (implizit.rs:306). */
+const char* gc_name(const gc_ctx* ctx);
+
+/* Upload topology, stiffnesses and rest lengths; kept on the device until the
+ * next call (gpu.rs:1362-1456 re-uploads only on change). */
+int gc_set_topology(gc_ctx* ctx, const gc_topologie* topo);
+int gc_set_param(gc_ctx* ctx, const gc_param* p);
+/* Seam rest lengths change while a garment is sewn (`Szene::naht_kuerzen`,
+ * szene.rs:768-774). */
+int gc_set_naht_ruhe(gc_ctx* ctx, const double* ruhe);
+/* Held points (w = 0) are set by callers that clamp (tests/kontinuum.rs:382-386). */
+int gc_set_w(gc_ctx* ctx, const double* w);
+
+/* Bodies (koerper.rs:16-26). A sampled grid SDF stands where the reference has
+ * a triangle-mesh body; it answers `Koerper::kontakt` (koerper.rs:69-91).
+ * werte: float[nx*ny*nz], x fastest. Returns the body index or -1. */
+int gc_add_zylinder(gc_ctx* ctx, const double punkt[3], const double achse[3], double radius, const double geschwindigkeit[3]);
+int gc_add_sdf(gc_ctx* ctx, const float* werte, const int32_t dims[3], const double origin[3], double spacing,
+               const double versatz[3], const double geschwindigkeit[3]);
+/* Move a body to an absolute offset (cylinder: axis point; SDF: versatz), as
+ * `Koerper::verschoben` does per substep (koerper.rs:122-133). */
+int gc_set_koerper_lage(gc_ctx* ctx, int32_t koerper, const double lage[3]);
+
+/* State: positions and velocities (Szene::x, Szene::v). */
+int gc_set_state(gc_ctx* ctx, const double* x, const double* v);
+int gc_get_state(gc_ctx* ctx, double* x, double* v);
+/* Warm start of the next step (`ImplizitCg::dv_vorher`, implizit.rs:578, 906, 945).
+ * n = 0 clears it (the next step then starts from h*g). */
+int gc_set_dv_vorher(gc_ctx* ctx, const double* dv, int32_t n);
+
+/* --- the Loeser plug-in (implizit.rs:304-351) ------------------------------ */
+
+/* Build the system of a substep of length h at the current state: geometry,
+ * forces, load, right-hand side, preconditioner (`baut_selbst` = true:
+ * the device does what ImplizitCg::system_bauen does at implizit.rs:725-894),
+ * and the contacts against the bodies (implizit.rs:872-883). */
+int gc_build(gc_ctx* ctx, double h);
+/* Replace the contact set of the built system (`fest`, implizit.rs:593-599):
+ * flag per point (0/1), normal, target normal velocity, body velocity. */
+int gc_set_fest(gc_ctx* ctx, const int32_t* flag, const double* normale, const double* vn_soll, const double* v_koerper);
+int gc_get_fest(gc_ctx* ctx, int32_t* flag, double* normale, double* vn_soll, double* v_koerper);
+/* `Loeser::loese` (implizit.rs:308, CpuLoeser at 370-426): modified PCG;
+ * dv in = warm start, out = solution. */
+int gc_solve(gc_ctx* ctx, double* dv, gc_cg_lauf* lauf);
+/* `Loeser::mal_a` (implizit.rs:310): unfiltered A*p. */
+int gc_apply_A(gc_ctx* ctx, const double* p, double* out);
+/* `System::mal_h` (implizit.rs:185-221): H*p without mass and h^2. */
+int gc_apply_H(gc_ctx* ctx, const double* p, double* out);
+/* `Loeser::rest_geloest` (implizit.rs:337-342): A*dv - b. */
+int gc_residual(gc_ctx* ctx, const double* dv, double* out);
+/* `Loeser::last_lesen` (implizit.rs:348-350): load per point (tension + weight). */
+int gc_read_load(gc_ctx* ctx, double* out);
+/* The built system for record-by-record checks (tests/gpu.rs:356-436):
+ * rhs, f, last (3n each), pinv (9n, row-major), any may be NULL. */
+int gc_read_system(gc_ctx* ctx, double* rhs, double* f, double* last, double* pinv9);
+/* Per distance constraint (edges with k != 0 in input order, then seams):
+ * direction (3), cross factor; per bending constraint: 4 gradients (12). */
+int gc_read_records(gc_ctx* ctx, double* richtung, double* quer, double* biege_grad);
+
+/* --- the whole step on the device ----------------------------------------- */
+
+/* `ImplizitCg::schritt` (implizit.rs:899-1038) for n substeps of length h:
+ * build, warm start, solve, release pass and second solve, friction, position
+ * update, push-out. Bodies stay where they are. */
+int gc_step(gc_ctx* ctx, double h, int32_t n);
+/* `Rechenweg::bild` for ImplizitCg (implizit.rs:1042-1057): schritte substeps of
+ * bild_dt/schritte, bodies moved from their frame-start position by
+ * v*(k+1)*h before substep k. */
+int gc_bild(gc_ctx* ctx, double bild_dt, int32_t schritte);
+/* Capture one substep of length h into a HIP graph and replay it n times
+ * (same semantics as gc_step). */
+int gc_step_graph(gc_ctx* ctx, double h, int32_t n);
+int gc_read_statistik(gc_ctx* ctx, gc_statistik* out, int32_t reset);
+/* Per-kernel device timings of the last gc_step/gc_step_graph call (HIP events
+ * on the stream the kernels run on): total ms and PCG-kernel ms. */
+int gc_zeiten(gc_ctx* ctx, double* ms_schritt, double* ms_pcg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
