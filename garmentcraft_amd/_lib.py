@@ -24,7 +24,8 @@ class GcFehler(RuntimeError):
 
 
 class Optionen(C.Structure):
-    _fields_ = [("praezision", C.c_int32), ("geraet", C.c_int32), ("bloecke", C.c_int32), ("threads", C.c_int32)]
+    _fields_ = [("praezision", C.c_int32), ("geraet", C.c_int32), ("bloecke", C.c_int32), ("threads", C.c_int32),
+                ("lds_kb", C.c_int32)]
 
 
 class Topologie(C.Structure):

@@ -75,6 +75,24 @@ struct SysDev {
   V3<T>* ap;
   V3<double>* dv_vorher;
   int32_t hat_vorher;
+  // tiles of the PCG operator: row ranges whose CSR entries (and the points
+  // their constraints touch) fit in LDS
+  int32_t n_kacheln;
+  const int32_t* kachel_zeilen;   // lo, hi per tile
+  const uint32_t* kachel_el_off;  // element-list offsets (n_kacheln + 1)
+  const uint32_t* kachel_el;      // category << 30 | constraint
+  const uint16_t* kachel_lv;      // 4 tile-local point indices per listed element
+  const uint16_t* kachel_pos;     // 4 LDS slots per listed element, 0xFFFF = point not in the tile
+  const uint32_t* kachel_v_off;   // point-list offsets (n_kacheln + 1)
+  const uint32_t* kachel_v;       // points touched by the tile's constraints, global index
+  uint32_t kachel_slots_max;      // LDS slot area (entries) of the largest tile
+  // Chronopoulos-Gear recurrences: published (double-buffered) r, w, s and the
+  // block-private p, u
+  V3<T>* cr[2];
+  V3<T>* cw[2];
+  V3<T>* cs[2];
+  V3<T>* cp;
+  V3<T>* cu;
   // bodies
   int32_t n_koerper;
   Koerper koerper[GC_MAX_KOERPER];
@@ -110,18 +128,25 @@ struct alignas(64) Slot {
 // can only overwrite parity p after every block has published epoch e+1,
 // i.e. after everyone has read epoch e-1.
 // ---------------------------------------------------------------------------
+// Deterministic block sum: xor butterfly inside the wave (every lane ends
+// with the same value, IEEE addition being commutative), then the wave sums in
+// wave order. sh: 16 * NV doubles.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
-  // sh: blockDim.x * NV doubles of LDS
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int k = 0; k < NV; ++k) sh[k * nt + t] = v[k];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
   __syncthreads();
-  for (int o = nt / 2; o > 0; o >>= 1) {
-    if (t < o)
-      for (int k = 0; k < NV; ++k) sh[k * nt + t] += sh[k * nt + t + o];
-    __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < NV; ++k) sh[wave * NV + k] = v[k];
+  __syncthreads();
+  for (int k = 0; k < NV; ++k) {
+    double a = 0.0;
+    for (int w = 0; w < nw; ++w) a += sh[w * NV + k];
+    v[k] = a;
   }
-  for (int k = 0; k < NV; ++k) v[k] = sh[k * nt];
   __syncthreads();
 }
 
@@ -423,116 +448,252 @@ __global__ void k_warm(SysDev<T> s) {
 }
 
 // ---------------------------------------------------------------------------
-// Persistent modified PCG (CpuLoeser::loese, implizit.rs:370-426).
-// One launch per solve; blocks own contiguous point ranges; two grid
-// all-reduces and one grid barrier per iteration.
-// durchgang 1 is the second solve after a release and returns at once
-// unless a point was released.
+// Tiled operator for the PCG. Stage A puts the operand of every point the
+// tile's constraints touch into LDS (computed once per point); stage B
+// computes every listed constraint's contribution to the tile's own points,
+// in parallel over constraints, into one LDS slot per CSR entry; stage C sums
+// each own point's slots in CSR order — the reference's summation order
+// (implizit.rs:185-221).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ V3<T> filter_fl(uint8_t fl, V3<T> n, V3<T> v) {
+  if (fl == 0) return v;
+  if (fl == 2) return zero3<T>();
+  return v - n * dot(v, n);
+}
+
+template <typename T>
+__device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T>* slot, const V3<T>* u) {
+  const uint32_t e0 = s.kachel_el_off[t], e1 = s.kachel_el_off[t + 1];
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const uint32_t ref = s.kachel_el[e];
+    const uint32_t kat = ref >> 30, c = ref & 0x3FFFFFFFu;
+    const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
+    const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
+    const uint32_t lv[4] = {lw.x & 0xFFFFu, lw.x >> 16, lw.y & 0xFFFFu, lw.y >> 16};
+    const uint32_t pos[4] = {pw.x & 0xFFFFu, pw.x >> 16, pw.y & 0xFFFFu, pw.y >> 16};
+    if (kat == 2u) {
+      const Membran<T> el = s.m_el[c];
+      const Hessen<T> hh = s.m_h[c];
+      const V3<T> d3[3] = {u[lv[0]], u[lv[1]], u[lv[2]]};
+      V3<T> o[3];
+      mal_h_aus_alle(el, hh, d3, o);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (pos[k] != 0xFFFFu) slot[pos[k]] = o[k];
+    } else if (kat == 1u) {
+      const V3<T>* gr = s.b_grad + 4 * (size_t)c;
+      V3<T> g[4];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) g[l] = gr[l];
+      const T kt = s.b_kt[c];
+      T gp = T(0);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) gp += dot(g[l], u[lv[l]]);
+      const T kg = kt * gp;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (pos[m] != 0xFFFFu) slot[pos[m]] = g[m] * kg;
+    } else {
+      const V3<T> n = s.k_n[c];
+      const T q = s.k_q[c], kk = s.k_kk[c];
+      const V3<T> d = u[lv[0]] - u[lv[1]];
+      const V3<T> bd = (n * (dot(n, d) * (T(1) - q)) + d * q) * kk;
+      if (pos[0] != 0xFFFFu) slot[pos[0]] = bd;
+      if (pos[1] != 0xFFFFu) slot[pos[1]] = bd * T(-1);
+    }
+  }
+}
+
+// H*u of point i from its LDS slots, in CSR order.
+template <typename T>
+__device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_t basis, const V3<T>* slot) {
+  const uint32_t a = s.off[GC_KAT * i] - basis, b = s.off[GC_KAT * i + GC_KAT] - basis;
+  V3<T> hp = zero3<T>();
+  for (uint32_t q = a; q < b; ++q) hp = hp + slot[q];
+  return hp;
+}
+
+// The next preconditioned residual of point j, u = S P (r - alpha (w + beta s)),
+// from the published r, w, s of the previous iteration. The owner and every
+// tile that reads the point evaluate this same expression, so they agree to
+// the bit.
+template <typename T>
+__device__ __forceinline__ V3<T> u_naechstes(const SysDev<T>& s, uint32_t j, const V3<T>* R, const V3<T>* W,
+                                             const V3<T>* S, T ta, T tb, V3<T>* r_neu, V3<T>* s_neu) {
+  const uint8_t fl = s.fflag[j];
+  const V3<T> sn = W[j] + S[j] * tb;
+  const V3<T> rn = R[j] - sn * ta;
+  const V3<T> z = m3_mal(s.pinv + 9 * (size_t)j, rn);
+  if (r_neu) *r_neu = rn;
+  if (s_neu) *s_neu = sn;
+  return filter_fl(fl, s.fnorm[j], z);
+}
+
+template <typename T>
+__device__ __forceinline__ V3<T> u_start(const SysDev<T>& s, uint32_t j, const V3<T>* R) {
+  return filter_fl(s.fflag[j], s.fnorm[j], m3_mal(s.pinv + 9 * (size_t)j, R[j]));
+}
+
+// ---------------------------------------------------------------------------
+// Persistent modified PCG (CpuLoeser::loese, implizit.rs:370-426) in the
+// Chronopoulos-Gear form: the same iterates, with both inner products of an
+// iteration in one grid all-reduce. One launch per solve, one block per CU; a
+// block owns tiles t = b, b+nb, ...
+// durchgang 1 is the second solve after a release; it returns at once unless
+// a point was released.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(1024) k_pcg(SysDev<T> s, GridState* gs, Slot* slots, int durchgang) {
-  extern __shared__ double sh[];  // blockDim.x * 2 doubles
+  extern __shared__ __align__(16) unsigned char lds_roh[];
+  V3<T>* slot = (V3<T>*)lds_roh;
+  V3<T>* ul = slot + s.kachel_slots_max;
+  __shared__ double sh[16 * 2];
   if (durchgang == 1) {
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
   const int nb = gridDim.x;
-  const int per = (s.n + nb - 1) / nb;
-  const int lo = blockIdx.x * per;
-  const int hi = min(s.n, lo + per);
   unsigned epoch = 0;
+  auto zeilen = [&](int t, int* lo, int* hi) { *lo = s.kachel_zeilen[2 * t]; *hi = s.kachel_zeilen[2 * t + 1]; };
+  const T th2 = T(s.h2);
 
   // start: held along the normal get the target normal velocity (implizit.rs:375-384)
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const uint8_t fl = s.fflag[i];
-    if (fl == 2) {
-      s.dv[i] = zero3<T>();
-    } else if (fl == 1) {
-      const V3<double> n = s.fnorm64[i];
-      const double soll = s.vn_soll[i] - dot(s.v[i], n);
-      const V3<T> dvf = filtern(s, i, s.dv[i]);
-      s.dv[i] = dvf + s.fnorm[i] * T(soll);
+  for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+    int lo, hi;
+    zeilen(t, &lo, &hi);
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      const uint8_t fl = s.fflag[i];
+      if (fl == 2) {
+        s.dv[i] = zero3<T>();
+      } else if (fl == 1) {
+        const V3<double> n = s.fnorm64[i];
+        const double soll = s.vn_soll[i] - dot(s.v[i], n);
+        const V3<T> dvf = filter_fl(fl, s.fnorm[i], s.dv[i]);
+        s.dv[i] = dvf + s.fnorm[i] * T(soll);
+      }
     }
   }
   {
     double nix[1] = {0.0};
     if (!grid_allreduce<1>(nix, sh, slots, gs, epoch)) return;
   }
-  // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z)  (implizit.rs:385-391)
+  // r0 = S(b - A dv) and bb = sum S(b) P S(b) (implizit.rs:385-388)
   double part[2] = {0.0, 0.0};
-  const V3<T>* dvp = s.dv;
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return dvp[j]; });
-    const uint8_t fl = s.fflag[i];
-    const V3<T> bi = s.b[i];
-    const V3<T> ri = fl == 2 ? zero3<T>() : filtern(s, i, bi - ap);
-    const T* P = s.pinv + 9 * i;
-    if (fl != 2) {
-      const V3<T> fb = filtern(s, i, bi);
-      part[0] += (double)dot(fb, m3_mal(P, fb));
+  for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+    int lo, hi;
+    zeilen(t, &lo, &hi);
+    const uint32_t basis = s.off[GC_KAT * lo];
+    const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
+    for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = s.dv[s.kachel_v[v]];
+    __syncthreads();
+    kachel_beitraege(s, t, slot, ul);
+    __syncthreads();
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      const V3<T> ap = kachel_zeile(s, i, basis, slot) * th2 + s.dv[i] * s.mdiag[i];
+      const uint8_t fl = s.fflag[i];
+      const V3<T> n = s.fnorm[i];
+      const V3<T> bi = s.b[i];
+      const V3<T> ri = filter_fl(fl, n, bi - ap);
+      if (fl != 2) {
+        const V3<T> fb = filter_fl(fl, n, bi);
+        part[0] += (double)dot(fb, m3_mal(s.pinv + 9 * (size_t)i, fb));
+      }
+      const V3<T> u0 = filter_fl(fl, n, m3_mal(s.pinv + 9 * (size_t)i, ri));
+      part[1] += (double)dot(ri, u0);
+      s.cr[0][i] = ri;
     }
-    const V3<T> zi = m3_mal(P, ri);
-    const V3<T> cd = filtern(s, i, zi);
-    part[1] += (double)dot(ri, cd);
-    s.r[i] = ri;
-    s.z[i] = zi;
+    __syncthreads();
   }
   if (!grid_allreduce<2>(part, sh, slots, gs, epoch)) return;
   const double bb = part[0];
-  double delta = part[1];
-  const double ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
+  double gamma = part[1];
+  const double ziel = gmax(s.cg_tol2 * gmax(bb, gamma), s.boden);
   int it = 0;
-  double beta = 0.0;
-  int cur = 0;
-  while (delta > ziel && it < s.cg_max) {
-    // new direction p = S(z + p_old * beta), held completely: 0 (implizit.rs:420-422)
-    V3<T>* pn = s.p[cur];
-    const V3<T>* po = s.p[cur ^ 1];
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const V3<T> zz = it == 0 ? s.z[i] : s.z[i] + po[i] * T(beta);
-      pn[i] = s.fflag[i] == 2 ? zero3<T>() : filtern(s, i, zz);
+  if (gamma > ziel && s.cg_max > 0) {
+    // w0 = S A u0, delta0 = w0 . u0
+    double d0[1] = {0.0};
+    for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+      int lo, hi;
+      zeilen(t, &lo, &hi);
+      const uint32_t basis = s.off[GC_KAT * lo];
+      const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
+      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = u_start(s, s.kachel_v[v], s.cr[0]);
+      __syncthreads();
+      kachel_beitraege(s, t, slot, ul);
+      __syncthreads();
+      for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint8_t fl = s.fflag[i];
+        const V3<T> ui = u_start(s, (uint32_t)i, s.cr[0]);
+        const V3<T> wi = filter_fl(fl, s.fnorm[i], kachel_zeile(s, i, basis, slot) * th2 + ui * s.mdiag[i]);
+        s.cw[0][i] = wi;
+        s.cs[0][i] = zero3<T>();
+        s.cp[i] = zero3<T>();
+        s.cu[i] = ui;
+        d0[0] += (double)dot(wi, ui);
+      }
+      __syncthreads();
     }
-    {
-      double nix[1] = {0.0};
-      if (!grid_allreduce<1>(nix, sh, slots, gs, epoch)) return;
+    if (!grid_allreduce<1>(d0, sh, slots, gs, epoch)) return;
+    double ca = d0[0];  // p0 . A p0 with p0 = u0 (implizit.rs:405-408)
+    if (ca > 2.2250738585072014e-308 && isfinite(ca)) {
+      double alpha = gamma / ca, beta = 0.0;
+      int cur = 0;
+      while (true) {
+        // s = w + beta s, p = u + beta p, dv += alpha p, r -= alpha s, u = S P r, w = S A u
+        const T ta = T(alpha), tb = T(beta);
+        const V3<T>* R = s.cr[cur];
+        const V3<T>* W = s.cw[cur];
+        const V3<T>* S = s.cs[cur];
+        double gd[2] = {0.0, 0.0};
+        for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+          int lo, hi;
+          zeilen(t, &lo, &hi);
+          const uint32_t basis = s.off[GC_KAT * lo];
+          const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
+          for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x)
+            ul[v - v0] = u_naechstes(s, s.kachel_v[v], R, W, S, ta, tb, (V3<T>*)nullptr, (V3<T>*)nullptr);
+          __syncthreads();
+          kachel_beitraege(s, t, slot, ul);
+          __syncthreads();
+          for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+            V3<T> rn, sn;
+            const V3<T> un = u_naechstes(s, (uint32_t)i, R, W, S, ta, tb, &rn, &sn);
+            const V3<T> pn = s.cu[i] + s.cp[i] * tb;
+            s.dv[i] = s.dv[i] + pn * ta;
+            const V3<T> wn = filter_fl(s.fflag[i], s.fnorm[i], kachel_zeile(s, i, basis, slot) * th2 + un * s.mdiag[i]);
+            s.cr[cur ^ 1][i] = rn;
+            s.cw[cur ^ 1][i] = wn;
+            s.cs[cur ^ 1][i] = sn;
+            s.cp[i] = pn;
+            s.cu[i] = un;
+            gd[0] += (double)dot(rn, un);
+            gd[1] += (double)dot(wn, un);
+          }
+          __syncthreads();
+        }
+        if (!grid_allreduce<2>(gd, sh, slots, gs, epoch)) return;
+        ++it;
+        cur ^= 1;
+        const double gneu = gd[0];
+        beta = gneu / gamma;
+        gamma = gneu;
+        if (!(gamma > ziel) || it >= s.cg_max) break;
+        ca = gd[1] - beta * gneu / alpha;  // p . A p of the next direction
+        if (ca <= 2.2250738585072014e-308 || !isfinite(ca)) break;
+        alpha = gneu / ca;
+      }
     }
-    // ap = S(A p), ca = p . ap (implizit.rs:401-405)
-    double ca[1] = {0.0};
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return pn[j]; });
-      ap = s.fflag[i] == 2 ? zero3<T>() : filtern(s, i, ap);
-      s.ap[i] = ap;
-      ca[0] += (double)dot(pn[i], ap);
-    }
-    if (!grid_allreduce<1>(ca, sh, slots, gs, epoch)) return;
-    if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
-    const double alpha = delta / ca[0];
-    // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
-    double neu[1] = {0.0};
-    const T ta = T(alpha);
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      s.dv[i] = s.dv[i] + pn[i] * ta;
-      const V3<T> ri = s.r[i] - s.ap[i] * ta;
-      s.r[i] = ri;
-      const V3<T> zi = m3_mal(s.pinv + 9 * i, ri);
-      s.z[i] = zi;
-      neu[0] += (double)dot(ri, zi);
-    }
-    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch)) return;
-    beta = neu[0] / delta;
-    delta = neu[0];
-    cur ^= 1;
-    ++it;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     gs->iterationen = it;
-    gs->nicht_konv = delta > ziel ? 1 : 0;
-    gs->rest = sqrt(delta / gmax(bb, s.boden));
-    gs->delta = delta;
+    gs->nicht_konv = gamma > ziel ? 1 : 0;
+    gs->rest = sqrt(gamma / gmax(bb, s.boden));
+    gs->delta = gamma;
     gs->ziel = ziel;
     gs->bb = bb;
     gs->loesungen += 1;
     gs->cg_iterationen += it;
-    if (delta > ziel) gs->nicht_konvergiert += 1;
+    if (gamma > ziel) gs->nicht_konvergiert += 1;
     gs->rest_max = gmax(gs->rest_max, gs->rest);
   }
 }

@@ -179,6 +179,43 @@ GC_HD V3<T> mal_h_aus_k(const Membran<T>& m, const Hessen<T>& h, const V3<T> del
   return out;
 }
 
+// kontinuum.rs:616-664, all three outputs: the shared q and kg*q once, then
+// per vertex the same operations as mal_h_aus_k.
+template <typename T>
+GC_HD void mal_h_aus_alle(const Membran<T>& m, const Hessen<T>& h, const V3<T> delta[3], V3<T> out[3]) {
+  V3<T> f1 = h.f1, f2 = h.f2;
+  V3<T> c1 = c1_(m), c2 = c2_(m);
+  V3<T> q = zero3<T>();
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T d1 = dot(f1, delta[l]), d2 = dot(f2, delta[l]);
+    q.x += T(2) * comp(c1, l) * d1;
+    q.y += T(2) * comp(c2, l) * d2;
+    q.z += T(2) * (comp(c2, l) * d1 + comp(c1, l) * d2);
+  }
+  V3<T> w = kg_mal(m, q);
+  T w11 = h.w.x, w22 = h.w.y, w12 = h.w.z;
+  V3<T> t1 = zero3<T>(), t2 = zero3<T>();
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    t1.x += comp(c1, b) * delta[b].x; t2.x += comp(c2, b) * delta[b].x;
+    t1.y += comp(c1, b) * delta[b].y; t2.y += comp(c2, b) * delta[b].y;
+    t1.z += comp(c1, b) * delta[b].z; t2.z += comp(c2, b) * delta[b].z;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    T s1 = T(2) * (comp(c1, k) * w.x + comp(c2, k) * w.z);
+    T s2 = T(2) * (comp(c2, k) * w.y + comp(c1, k) * w.z);
+    V3<T> o = {s1 * f1.x + s2 * f2.x, s1 * f1.y + s2 * f2.y, s1 * f1.z + s2 * f2.z};
+    T pa = T(2) * (w11 * comp(c1, k) + w12 * comp(c2, k));
+    T qa = T(2) * (w12 * comp(c1, k) + w22 * comp(c2, k));
+    o.x += pa * t1.x + qa * t2.x;
+    o.y += pa * t1.y + qa * t2.y;
+    o.z += pa * t1.z + qa * t2.z;
+    out[k] = o;
+  }
+}
+
 // kontinuum.rs:675-708 for vertex k: J_k^T kg J_k + s_kk I, row-major 3x3.
 template <typename T>
 GC_HD void diag_block_k(const Membran<T>& m, const Hessen<T>& h, int k, T out[9]) {

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -57,7 +58,10 @@ struct gc_ctx {
   Slot* slots = nullptr;
   bool topo = false, hat_param = false, built = false;
   bool hat_vorher = false;
-  int pcg_blocks = 0, pcg_threads = 256, pcg_max_blocks = 0;
+  int pcg_blocks = 0, pcg_threads = 1024, pcg_max_blocks = 0;
+  size_t pcg_lds = 0;       // dynamic LDS of the PCG kernel (largest tile)
+  size_t lds_budget = 0;    // bytes of LDS a tile may fill
+  int n_kacheln = 0;
   // timing
   std::vector<hipEvent_t> ev;
   double ms_schritt = 0.0, ms_pcg = 0.0;
@@ -194,13 +198,127 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
       !s.x || !s.v || !s.mdiag || !s.f || !s.last || !s.b || !s.pinv || !s.fflag || !s.fnorm || !s.fnorm64 || !s.vn_soll ||
       !s.vk || !s.fn_kraft || !s.dv || !s.r || !s.z || !s.p[0] || !s.p[1] || !s.ap || !s.dv_vorher)
     return fehler(c, "hipMalloc failed (vectors)");
+  // Tiles: consecutive rows whose CSR entries plus the points their
+  // constraints touch fit the LDS budget. Per tile: the constraints touching
+  // its rows, each with the tile-local index of its points and the LDS slot of
+  // every (constraint, own point).
+  const size_t budget = c->lds_budget / sizeof(V3<T>);
+  std::vector<int32_t> kz;
+  std::vector<uint32_t> k_off{0}, k_el, v_off{0}, k_v;
+  std::vector<uint16_t> k_pos, k_lv;
+  size_t max_slots = 0, max_v = 0;
+  {
+    auto punkte_von = [&](uint32_t kat, uint32_t cc, uint32_t* q) -> int {
+      if (kat == 0) { q[0] = c->k_ij[2 * cc]; q[1] = c->k_ij[2 * cc + 1]; return 2; }
+      if (kat == 1) { for (int m = 0; m < 4; ++m) q[m] = c->b_p[4 * (size_t)cc + m]; return 4; }
+      for (int m = 0; m < 3; ++m) q[m] = c->m_p[3 * (size_t)cc + m];
+      return 3;
+    };
+    std::vector<std::pair<uint64_t, std::array<uint16_t, 4>>> liste;
+    std::vector<uint32_t> pkt;
+    int lo = 0;
+    size_t ziel_slots = budget * 3 / 4;
+    while (lo < n) {
+      int hi = lo;
+      while (hi < n && (size_t)(off[GC_KAT * (size_t)(hi + 1)] - off[GC_KAT * (size_t)lo]) <= ziel_slots) ++hi;
+      if (hi == lo) return fehler(c, "a single point has more constraints than fit in LDS");
+      const uint32_t basis = off[GC_KAT * (size_t)lo];
+      liste.clear();
+      for (int i = lo; i < hi; ++i)
+        for (int kat = 0; kat < 3; ++kat)
+          for (uint32_t e = off[GC_KAT * (size_t)i + kat]; e < off[GC_KAT * (size_t)i + kat + 1]; ++e) {
+            const uint64_t key = ((uint64_t)kat << 32) | (csr[e] >> 2);
+            std::array<uint16_t, 4> pos = {0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF};
+            pos[csr[e] & 3u] = (uint16_t)(e - basis);
+            liste.push_back({key, pos});
+          }
+      std::sort(liste.begin(), liste.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      // unique points of the listed constraints
+      pkt.clear();
+      for (const auto& l : liste) {
+        uint32_t q[4];
+        int np = punkte_von((uint32_t)(l.first >> 32), (uint32_t)(l.first & 0xFFFFFFFFu), q);
+        for (int m = 0; m < np; ++m) pkt.push_back(q[m]);
+      }
+      std::sort(pkt.begin(), pkt.end());
+      pkt.erase(std::unique(pkt.begin(), pkt.end()), pkt.end());
+      const size_t slots_hier = off[GC_KAT * (size_t)hi] - basis;
+      if (slots_hier + pkt.size() > budget || pkt.size() > 0xFFFEu) {
+        if (ziel_slots < 64) return fehler(c, "LDS budget too small for one tile");
+        ziel_slots = ziel_slots * 7 / 8;  // fewer rows, try again
+        continue;
+      }
+      max_slots = std::max(max_slots, slots_hier);
+      max_v = std::max(max_v, pkt.size());
+      for (size_t q = 0; q < liste.size();) {
+        std::array<uint16_t, 4> pos = liste[q].second;
+        size_t r = q + 1;
+        for (; r < liste.size() && liste[r].first == liste[q].first; ++r)
+          for (int m = 0; m < 4; ++m)
+            if (liste[r].second[m] != 0xFFFF) pos[m] = liste[r].second[m];
+        const uint32_t kat = (uint32_t)(liste[q].first >> 32), cc = (uint32_t)(liste[q].first & 0xFFFFFFFFu);
+        k_el.push_back((kat << 30) | cc);
+        uint32_t qp[4] = {0, 0, 0, 0};
+        int np = punkte_von(kat, cc, qp);
+        for (int m = 0; m < 4; ++m) {
+          k_pos.push_back(pos[m]);
+          uint16_t lvv = 0;
+          if (m < np) lvv = (uint16_t)(std::lower_bound(pkt.begin(), pkt.end(), qp[m]) - pkt.begin());
+          k_lv.push_back(lvv);
+        }
+        q = r;
+      }
+      for (uint32_t v : pkt) k_v.push_back(v);
+      kz.push_back(lo);
+      kz.push_back(hi);
+      k_off.push_back((uint32_t)k_el.size());
+      v_off.push_back((uint32_t)k_v.size());
+      lo = hi;
+      ziel_slots = budget * 3 / 4;
+    }
+  }
+  c->n_kacheln = (int)(kz.size() / 2);
+  int32_t* d_kz = dalloc<int32_t>(c, kz.size());
+  uint32_t* d_koff = dalloc<uint32_t>(c, k_off.size());
+  uint32_t* d_kel = dalloc<uint32_t>(c, k_el.size());
+  uint16_t* d_kpos = dalloc<uint16_t>(c, k_pos.size());
+  uint16_t* d_klv = dalloc<uint16_t>(c, k_lv.size());
+  uint32_t* d_voff = dalloc<uint32_t>(c, v_off.size());
+  uint32_t* d_kv = dalloc<uint32_t>(c, k_v.size());
+  if (!d_kz || !d_koff || !d_kel || !d_kpos || !d_klv || !d_voff || !d_kv) return fehler(c, "hipMalloc failed (tiles)");
+  if (hoch(c, d_kz, kz.data(), kz.size()) || hoch(c, d_koff, k_off.data(), k_off.size()) ||
+      hoch(c, d_kel, k_el.data(), k_el.size()) || hoch(c, d_kpos, k_pos.data(), k_pos.size()) ||
+      hoch(c, d_klv, k_lv.data(), k_lv.size()) || hoch(c, d_voff, v_off.data(), v_off.size()) ||
+      hoch(c, d_kv, k_v.data(), k_v.size()))
+    return 1;
+  s.n_kacheln = c->n_kacheln;
+  s.kachel_zeilen = d_kz;
+  s.kachel_el_off = d_koff;
+  s.kachel_el = d_kel;
+  s.kachel_pos = d_kpos;
+  s.kachel_lv = d_klv;
+  s.kachel_v_off = d_voff;
+  s.kachel_v = d_kv;
+  s.kachel_slots_max = (uint32_t)max_slots;
+  for (int q = 0; q < 2; ++q) {
+    s.cr[q] = dalloc<V3<T>>(c, n);
+    s.cw[q] = dalloc<V3<T>>(c, n);
+    s.cs[q] = dalloc<V3<T>>(c, n);
+    if (!s.cr[q] || !s.cw[q] || !s.cs[q]) return fehler(c, "hipMalloc failed (recurrences)");
+  }
+  s.cp = dalloc<V3<T>>(c, n);
+  s.cu = dalloc<V3<T>>(c, n);
+  if (!s.cp || !s.cu) return fehler(c, "hipMalloc failed (recurrences)");
+  const size_t groesste = max_slots + max_v;
   // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
+  c->pcg_lds = std::max<size_t>(16, groesste * sizeof(V3<T>));
+  (void)hipFuncSetAttribute((const void*)k_pcg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->pcg_lds);
   int occ = 0;
-  const size_t lds = (size_t)c->pcg_threads * 2 * sizeof(double);
-  HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, lds));
-  c->pcg_max_blocks = std::min(GC_MAX_BLOECKE, std::max(1, occ) * c->cus);
-  int want = c->opt.bloecke > 0 ? c->opt.bloecke : (n + c->pcg_threads - 1) / c->pcg_threads;
-  want = std::min(want, 128);
+  HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, c->pcg_lds));
+  if (occ < 1) return fehler(c, "PCG kernel does not fit on a CU with this LDS budget");
+  c->pcg_max_blocks = std::min(GC_MAX_BLOECKE, occ * c->cus);
+  int want = c->opt.bloecke > 0 ? c->opt.bloecke : 64;
+  want = std::min(want, c->n_kacheln);
   c->pcg_blocks = std::max(1, std::min(want, c->pcg_max_blocks));
   return 0;
 }
@@ -248,8 +366,7 @@ static int launch_bau(gc_ctx* c, double h) {
 template <typename T>
 static int launch_pcg(gc_ctx* c, int durchgang) {
   SysDev<T>& s = sys<T>(c);
-  const size_t lds = (size_t)c->pcg_threads * 2 * sizeof(double);
-  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), lds, c->stream, s, c->gs, c->slots, durchgang);
+  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, c->stream, s, c->gs, c->slots, durchgang);
   HIPCK(c, hipGetLastError());
   return 0;
 }
@@ -262,9 +379,9 @@ static int launch_schritt(gc_ctx* c, double h, hipEvent_t e_pcg0, hipEvent_t e_p
   hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s);
   if (e_pcg0) HIPCK(c, hipEventRecord(e_pcg0, st));
   if (launch_pcg<T>(c, 0)) return 1;
+  if (e_pcg1) HIPCK(c, hipEventRecord(e_pcg1, st));
   hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs, 0);
   if (launch_pcg<T>(c, 1)) return 1;
-  if (e_pcg1) HIPCK(c, hipEventRecord(e_pcg1, st));
   hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs, 1);
   hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs);
   HIPCK(c, hipGetLastError());
@@ -524,9 +641,14 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
   gc_ctx* c = new gc_ctx();
   c->opt = o;
   c->cus = p.multiProcessorCount;
-  c->pcg_threads = o.threads > 0 ? o.threads : 256;
-  if (c->pcg_threads > 1024 || (c->pcg_threads & (c->pcg_threads - 1))) {
-    melde("threads must be a power of two <= 1024");
+  c->lds_budget = (size_t)(o.lds_kb > 0 ? o.lds_kb : 128) * 1024;
+  if (c->lds_budget + 1024 > (size_t)p.sharedMemPerBlock) {
+    melde("lds_kb exceeds the LDS of a workgroup");
+    return nullptr;
+  }
+  c->pcg_threads = o.threads > 0 ? o.threads : 1024;
+  if (c->pcg_threads > 1024 || c->pcg_threads < 64 || (c->pcg_threads & (c->pcg_threads - 1))) {
+    melde("threads must be a power of two in [64, 1024]");
     delete c;
     return nullptr;
   }

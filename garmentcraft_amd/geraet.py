@@ -50,9 +50,9 @@ class GeraetSzene:
     FP64 reductions)."""
 
     def __init__(self, s: Szene, param: Optional[ImplizitParameter] = None, praezision: str = "fp64",
-                 bloecke: int = 0, threads: int = 0, geraet: int = 0):
+                 bloecke: int = 0, threads: int = 0, geraet: int = 0, lds_kb: int = 0):
         self.lib = L.lib()
-        opt = L.Optionen(L.GC_FP32 if praezision == "fp32" else L.GC_FP64, geraet, bloecke, threads)
+        opt = L.Optionen(L.GC_FP32 if praezision == "fp32" else L.GC_FP64, geraet, bloecke, threads, lds_kb)
         err = C.create_string_buffer(512)
         h = self.lib.gc_create(C.byref(opt), err, 512)
         if not h:

@@ -37,6 +37,7 @@ typedef struct gc_optionen {
   int32_t geraet;         /* HIP device ordinal */
   int32_t bloecke;        /* persistent PCG blocks; 0 = choose (autotuned table) */
   int32_t threads;        /* threads per block; 0 = choose */
+  int32_t lds_kb;         /* LDS a PCG tile may fill (KiB); 0 = choose */
 } gc_optionen;
 
 /* Topology of the system, as `System` carries it when the backend builds the
