@@ -86,13 +86,7 @@ struct SysDev {
   const uint32_t* kachel_v_off;   // point-list offsets (n_kacheln + 1)
   const uint32_t* kachel_v;       // points touched by the tile's constraints, global index
   uint32_t kachel_slots_max;      // LDS slot area (entries) of the largest tile
-  // Chronopoulos-Gear recurrences: published (double-buffered) r, w, s and the
-  // block-private p, u
-  V3<T>* cr[2];
-  V3<T>* cw[2];
-  V3<T>* cs[2];
-  V3<T>* cp;
-  V3<T>* cu;
+
   // bodies
   int32_t n_koerper;
   Koerper koerper[GC_MAX_KOERPER];
@@ -514,32 +508,23 @@ __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_
   return hp;
 }
 
-// The next preconditioned residual of point j, u = S P (r - alpha (w + beta s)),
-// from the published r, w, s of the previous iteration. The owner and every
-// tile that reads the point evaluate this same expression, so they agree to
-// the bit.
+// The search direction of point j, p = S(z + beta p_old) (implizit.rs:420-422);
+// the first direction is S(z) (implizit.rs:390). The owner and every tile that
+// reads the point evaluate this same expression, so they agree to the bit.
 template <typename T>
-__device__ __forceinline__ V3<T> u_naechstes(const SysDev<T>& s, uint32_t j, const V3<T>* R, const V3<T>* W,
-                                             const V3<T>* S, T ta, T tb, V3<T>* r_neu, V3<T>* s_neu) {
+__device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const V3<T>* po, T tb, bool erste) {
   const uint8_t fl = s.fflag[j];
-  const V3<T> sn = W[j] + S[j] * tb;
-  const V3<T> rn = R[j] - sn * ta;
-  const V3<T> z = m3_mal(s.pinv + 9 * (size_t)j, rn);
-  if (r_neu) *r_neu = rn;
-  if (s_neu) *s_neu = sn;
-  return filter_fl(fl, s.fnorm[j], z);
-}
-
-template <typename T>
-__device__ __forceinline__ V3<T> u_start(const SysDev<T>& s, uint32_t j, const V3<T>* R) {
-  return filter_fl(s.fflag[j], s.fnorm[j], m3_mal(s.pinv + 9 * (size_t)j, R[j]));
+  const V3<T> z = s.z[j];
+  const V3<T> q = erste ? z : z + po[j] * tb;
+  return filter_fl(fl, s.fnorm[j], q);
 }
 
 // ---------------------------------------------------------------------------
-// Persistent modified PCG (CpuLoeser::loese, implizit.rs:370-426) in the
-// Chronopoulos-Gear form: the same iterates, with both inner products of an
-// iteration in one grid all-reduce. One launch per solve, one block per CU; a
-// block owns tiles t = b, b+nb, ...
+// Persistent modified PCG (CpuLoeser::loese, implizit.rs:370-426), the
+// reference's recurrences. One launch per solve, one block per CU; a block
+// owns tiles t = b, b+nb, ... Two grid all-reduces per iteration (p.Ap and
+// r.z); the new direction needs no barrier of its own because every tile
+// rebuilds the directions of the points it reads (stage A).
 // durchgang 1 is the second solve after a release; it returns at once unless
 // a point was released.
 // ---------------------------------------------------------------------------
@@ -577,7 +562,7 @@ __global__ void __launch_bounds__(1024) k_pcg(SysDev<T> s, GridState* gs, Slot* 
     double nix[1] = {0.0};
     if (!grid_allreduce<1>(nix, sh, slots, gs, epoch)) return;
   }
-  // r0 = S(b - A dv) and bb = sum S(b) P S(b) (implizit.rs:385-388)
+  // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z) (implizit.rs:385-391)
   double part[2] = {0.0, 0.0};
   for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
     int lo, hi;
@@ -594,106 +579,85 @@ __global__ void __launch_bounds__(1024) k_pcg(SysDev<T> s, GridState* gs, Slot* 
       const V3<T> n = s.fnorm[i];
       const V3<T> bi = s.b[i];
       const V3<T> ri = filter_fl(fl, n, bi - ap);
+      const T* P = s.pinv + 9 * (size_t)i;
       if (fl != 2) {
         const V3<T> fb = filter_fl(fl, n, bi);
-        part[0] += (double)dot(fb, m3_mal(s.pinv + 9 * (size_t)i, fb));
+        part[0] += (double)dot(fb, m3_mal(P, fb));
       }
-      const V3<T> u0 = filter_fl(fl, n, m3_mal(s.pinv + 9 * (size_t)i, ri));
-      part[1] += (double)dot(ri, u0);
-      s.cr[0][i] = ri;
+      const V3<T> zi = m3_mal(P, ri);
+      part[1] += (double)dot(ri, filter_fl(fl, n, zi));
+      s.r[i] = ri;
+      s.z[i] = zi;
     }
     __syncthreads();
   }
   if (!grid_allreduce<2>(part, sh, slots, gs, epoch)) return;
   const double bb = part[0];
-  double gamma = part[1];
-  const double ziel = gmax(s.cg_tol2 * gmax(bb, gamma), s.boden);
+  double delta = part[1];
+  const double ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
   int it = 0;
-  if (gamma > ziel && s.cg_max > 0) {
-    // w0 = S A u0, delta0 = w0 . u0
-    double d0[1] = {0.0};
+  double beta = 0.0;
+  int cur = 0;
+  while (delta > ziel && it < s.cg_max) {
+    V3<T>* pn = s.p[cur];
+    const V3<T>* po = s.p[cur ^ 1];
+    const T tb = T(beta);
+    const bool erste = it == 0;
+    // ap = S(A p), ca = p . ap (implizit.rs:401-405)
+    double ca[1] = {0.0};
     for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
       int lo, hi;
       zeilen(t, &lo, &hi);
       const uint32_t basis = s.off[GC_KAT * lo];
       const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = u_start(s, s.kachel_v[v], s.cr[0]);
+      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung(s, s.kachel_v[v], po, tb, erste);
       __syncthreads();
       kachel_beitraege(s, t, slot, ul);
       __syncthreads();
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const uint8_t fl = s.fflag[i];
-        const V3<T> ui = u_start(s, (uint32_t)i, s.cr[0]);
-        const V3<T> wi = filter_fl(fl, s.fnorm[i], kachel_zeile(s, i, basis, slot) * th2 + ui * s.mdiag[i]);
-        s.cw[0][i] = wi;
-        s.cs[0][i] = zero3<T>();
-        s.cp[i] = zero3<T>();
-        s.cu[i] = ui;
-        d0[0] += (double)dot(wi, ui);
+        const V3<T> pi = richtung(s, (uint32_t)i, po, tb, erste);
+        const V3<T> ap = filter_fl(fl, s.fnorm[i], kachel_zeile(s, i, basis, slot) * th2 + pi * s.mdiag[i]);
+        pn[i] = pi;
+        s.ap[i] = ap;
+        ca[0] += (double)dot(pi, ap);
       }
       __syncthreads();
     }
-    if (!grid_allreduce<1>(d0, sh, slots, gs, epoch)) return;
-    double ca = d0[0];  // p0 . A p0 with p0 = u0 (implizit.rs:405-408)
-    if (ca > 2.2250738585072014e-308 && isfinite(ca)) {
-      double alpha = gamma / ca, beta = 0.0;
-      int cur = 0;
-      while (true) {
-        // s = w + beta s, p = u + beta p, dv += alpha p, r -= alpha s, u = S P r, w = S A u
-        const T ta = T(alpha), tb = T(beta);
-        const V3<T>* R = s.cr[cur];
-        const V3<T>* W = s.cw[cur];
-        const V3<T>* S = s.cs[cur];
-        double gd[2] = {0.0, 0.0};
-        for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
-          int lo, hi;
-          zeilen(t, &lo, &hi);
-          const uint32_t basis = s.off[GC_KAT * lo];
-          const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-          for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x)
-            ul[v - v0] = u_naechstes(s, s.kachel_v[v], R, W, S, ta, tb, (V3<T>*)nullptr, (V3<T>*)nullptr);
-          __syncthreads();
-          kachel_beitraege(s, t, slot, ul);
-          __syncthreads();
-          for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-            V3<T> rn, sn;
-            const V3<T> un = u_naechstes(s, (uint32_t)i, R, W, S, ta, tb, &rn, &sn);
-            const V3<T> pn = s.cu[i] + s.cp[i] * tb;
-            s.dv[i] = s.dv[i] + pn * ta;
-            const V3<T> wn = filter_fl(s.fflag[i], s.fnorm[i], kachel_zeile(s, i, basis, slot) * th2 + un * s.mdiag[i]);
-            s.cr[cur ^ 1][i] = rn;
-            s.cw[cur ^ 1][i] = wn;
-            s.cs[cur ^ 1][i] = sn;
-            s.cp[i] = pn;
-            s.cu[i] = un;
-            gd[0] += (double)dot(rn, un);
-            gd[1] += (double)dot(wn, un);
-          }
-          __syncthreads();
-        }
-        if (!grid_allreduce<2>(gd, sh, slots, gs, epoch)) return;
-        ++it;
-        cur ^= 1;
-        const double gneu = gd[0];
-        beta = gneu / gamma;
-        gamma = gneu;
-        if (!(gamma > ziel) || it >= s.cg_max) break;
-        ca = gd[1] - beta * gneu / alpha;  // p . A p of the next direction
-        if (ca <= 2.2250738585072014e-308 || !isfinite(ca)) break;
-        alpha = gneu / ca;
+    if (!grid_allreduce<1>(ca, sh, slots, gs, epoch)) return;
+    if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
+    const double alpha = delta / ca[0];
+    // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
+    double neu[1] = {0.0};
+    const T ta = T(alpha);
+    for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+      int lo, hi;
+      zeilen(t, &lo, &hi);
+      for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        s.dv[i] = s.dv[i] + pn[i] * ta;
+        const V3<T> ri = s.r[i] - s.ap[i] * ta;
+        s.r[i] = ri;
+        const V3<T> zi = m3_mal(s.pinv + 9 * (size_t)i, ri);
+        s.z[i] = zi;
+        neu[0] += (double)dot(ri, zi);
       }
     }
+    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch)) return;
+    beta = neu[0] / delta;
+    delta = neu[0];
+    cur ^= 1;
+    ++it;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     gs->iterationen = it;
-    gs->nicht_konv = gamma > ziel ? 1 : 0;
-    gs->rest = sqrt(gamma / gmax(bb, s.boden));
-    gs->delta = gamma;
+    gs->nicht_konv = delta > ziel ? 1 : 0;
+    gs->rest = sqrt(delta / gmax(bb, s.boden));
+    gs->delta = delta;
     gs->ziel = ziel;
     gs->bb = bb;
     gs->loesungen += 1;
     gs->cg_iterationen += it;
-    if (gamma > ziel) gs->nicht_konvergiert += 1;
+    if (delta > ziel) gs->nicht_konvergiert += 1;
     gs->rest_max = gmax(gs->rest_max, gs->rest);
   }
 }

@@ -300,15 +300,6 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   s.kachel_v_off = d_voff;
   s.kachel_v = d_kv;
   s.kachel_slots_max = (uint32_t)max_slots;
-  for (int q = 0; q < 2; ++q) {
-    s.cr[q] = dalloc<V3<T>>(c, n);
-    s.cw[q] = dalloc<V3<T>>(c, n);
-    s.cs[q] = dalloc<V3<T>>(c, n);
-    if (!s.cr[q] || !s.cw[q] || !s.cs[q]) return fehler(c, "hipMalloc failed (recurrences)");
-  }
-  s.cp = dalloc<V3<T>>(c, n);
-  s.cu = dalloc<V3<T>>(c, n);
-  if (!s.cp || !s.cu) return fehler(c, "hipMalloc failed (recurrences)");
   const size_t groesste = max_slots + max_v;
   // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
   c->pcg_lds = std::max<size_t>(16, groesste * sizeof(V3<T>));

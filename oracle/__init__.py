@@ -85,6 +85,8 @@ def lib():
         L.or_sim_setze_naht_ruhe.argtypes = [C.c_void_p, D]
         L.or_sim_setze_w.argtypes = [C.c_void_p, D]
         L.or_sim_setze_dv_vorher.argtypes = [C.c_void_p, D, C.c_int32]
+        L.or_sim_dv_vorher.argtypes = [C.c_void_p, D]
+        L.or_sim_dv_vorher.restype = C.c_int32
         L.or_sim_statistik.argtypes = [C.c_void_p, C.POINTER(Statistik_c)]
         L.or_sim_param.argtypes = [C.c_void_p, C.POINTER(Param_c)]
         L.or_sys_kanten.argtypes = [C.c_void_p, U32, D, D, D, D]
@@ -199,6 +201,14 @@ class OrakelSim:
         else:
             d = f64(dv)
             lib().or_sim_setze_dv_vorher(self.h, _p(d), d.shape[0])
+
+    def dv_vorher(self):
+        m = lib().or_sim_dv_vorher(self.h, None)
+        if m == 0:
+            return None
+        out = np.zeros((m, 3))
+        lib().or_sim_dv_vorher(self.h, _p(out))
+        return out
 
     def bild(self):
         lib().or_sim_bild(self.h)

@@ -1070,6 +1070,11 @@ void or_sim_setze_dv_vorher(or_sim* s, const double* dv, int32_t n) {
   s->dv_vorher.clear();
   for (int i = 0; i < n; ++i) s->dv_vorher.push_back(v3(dv + 3 * i));
 }
+int32_t or_sim_dv_vorher(const or_sim* s, double* out) {
+  if (out)
+    for (size_t i = 0; i < s->dv_vorher.size(); ++i) put(out + 3 * i, s->dv_vorher[i]);
+  return (int32_t)s->dv_vorher.size();
+}
 void or_sim_bild(or_sim* s) { s->bild(); }
 void or_sim_schritt(or_sim* s, double h) { s->schritt(h); }
 void or_sim_statistik(const or_sim* s, or_statistik* out) { *out = s->stat; }

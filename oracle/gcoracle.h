@@ -118,6 +118,7 @@ void    or_sim_setze_lage(or_sim* s, const double* x, const double* v);
 void    or_sim_setze_naht_ruhe(or_sim* s, const double* ruhe);
 void    or_sim_setze_w(or_sim* s, const double* w);
 void    or_sim_setze_dv_vorher(or_sim* s, const double* dv, int32_t n); /* n = 0 clears the warm start */
+int32_t or_sim_dv_vorher(const or_sim* s, double* out);            /* returns its length (0 = none) */
 void    or_sim_bild(or_sim* s);
 void    or_sim_schritt(or_sim* s, double h);
 void    or_sim_statistik(const or_sim* s, or_statistik* out);

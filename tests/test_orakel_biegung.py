@@ -1,0 +1,98 @@
+"""Dihedral bending (cpu.rs:279-307) and the directional bending stiffness
+(biegung.rs:92-173) of the oracle against the reference's own tests
+(cpu.rs:342-361, biegung.rs:186-309), and the product builder's restatements
+against the oracle, bit for bit."""
+import math
+
+import numpy as np
+
+import oracle as O
+from garmentcraft_amd import szene as S
+
+WEFT, WARP, BIAS = 2482.2424e-9, 2486.4924e-9, 2486.4924e-9
+
+
+def test_dieder_gradient_wie_differenzen():  # cpu.rs:342-361
+    q = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.4, 0.9, 0.3], [0.6, -0.8, 0.5]])
+    t, g = O.dieder(q.ravel())
+    eps = 1e-7
+    for k in range(4):
+        for a in range(3):
+            qp, qm = q.copy(), q.copy()
+            qp[k, a] += eps; qm[k, a] -= eps
+            d = (O.dieder(qp.ravel())[0] - O.dieder(qm.ravel())[0]) / (2 * eps)
+            assert abs(d - g[k, a]) < 1e-5
+    flach = np.array([[0, 0, 0], [1, 0, 0], [0.5, 1, 0], [0.5, -1, 0]], dtype=float)
+    assert abs(O.dieder(flach.ravel())[0]) < 1e-12
+    assert abs(t) > 0.1
+
+
+def test_dieder_entartet():
+    """implizit.rs:780-788: zero edge or zero area has no angle."""
+    q = np.array([[0, 0, 0], [0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=float)
+    assert O.dieder(q.ravel()) is None
+
+
+def test_winkel_fehler_kurzer_weg():  # cpu.rs:279-282
+    assert abs(O.winkel_fehler(math.pi - 1e-4, -math.pi + 1e-4) - (-2e-4)) < 1e-12
+    assert O.winkel_fehler(0.3, 0.1) == 0.3 - 0.1
+
+
+def test_isotrop_bleibt_bitgleich():  # biegung.rs:193-209
+    for d in (WEFT, 1.0e-12, 0.0, 6.0361e-8, 1.0e9):
+        for i in range(73):
+            th = i * (math.pi / 2) / 72.0
+            assert O.steifigkeit(d, d, d, th) == d
+
+
+def test_die_drei_richtungen_treffen():  # biegung.rs:212-227
+    assert O.steifigkeit(WEFT, WARP, BIAS, 0.0) == WARP
+    assert O.steifigkeit(WEFT, WARP, BIAS, math.pi / 2) == WEFT
+    assert abs(O.steifigkeit(WEFT, WARP, BIAS, math.pi / 4) - BIAS) <= 1e-15 * BIAS
+
+
+def test_weiche_schraege_minimum():  # biegung.rs:238-268
+    weft, warp, bias = 2.0, 3.0, 0.8
+    a = 2.0 * (weft + warp - 2.0 * bias)
+    b = 4.0 * bias - warp - 3.0 * weft
+    u = -b / (2.0 * a)
+    th = math.acos(math.sqrt(u))
+    assert abs(math.degrees(th) - 49.228260) < 1e-5
+    assert abs(O.steifigkeit(weft, warp, bias, th) - 519.0 / 680.0) <= 1e-13 * 519.0 / 680.0
+
+
+def test_acht_stoffe_nur_steifer():  # biegung.rs:275-290
+    vals = [O.steifigkeit(WEFT, WARP, BIAS, i * (math.pi / 2) / 1800.0) for i in range(1801)]
+    assert abs(min(vals) - WEFT) < 1e-18
+    assert abs((max(vals) - WEFT) / WEFT - 0.001926) < 2e-6
+
+
+def test_negative_steifigkeit_abgefangen():  # biegung.rs:293-309
+    weft, warp, bias = 1.0e-6, 10.0e-6, 1.0e-8
+    neg = 0
+    for i in range(3601):
+        d = O.steifigkeit(weft, warp, bias, i * (math.pi / 2) / 3600.0)
+        assert d > 0.0 and d >= 1.0e-3 * bias * (1 - 1e-12)
+        neg += d <= 1.0e-3 * bias * 1.000001
+    assert neg > 0
+
+
+def test_builder_restatements_bitgleich_mit_orakel():
+    """The product builder's Python restatements (membrane record, edge
+    direction factors, bending stiffness) equal the oracle's, bit for bit."""
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        ruhe = rng.uniform(-0.05, 0.05, size=(3, 2))
+        kanten = [float(np.hypot(*(ruhe[(k + 1) % 3] - ruhe[k]))) for k in range(3)]
+        dir2 = rng.normal(size=2)
+        mat = tuple(rng.uniform(10, 3000, size=3))
+        a = S.membran_bauen(ruhe, kanten, dir2, mat)
+        b = O.membran_bauen(ruhe.ravel(), kanten, dir2, mat)
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert np.array_equal(a, b)
+        e = rng.normal(size=2)
+        assert S.cos2_sin2(e, dir2) == O.cos2_sin2(e, dir2)
+        c2, s2 = S.cos2_sin2(e, dir2)
+        w3 = tuple(rng.uniform(1e-7, 1e-5, size=3))
+        assert S.steifigkeit_aus(c2, s2, *w3) == O.steifigkeit_aus(c2, s2, *w3)
