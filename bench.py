@@ -96,7 +96,8 @@ def main() -> int:
     value = args.steps / (wall_ms / 1e3)
     x_end, _ = g.lage()
     print(f"[bench] {args.steps} steps in {wall_ms:.1f} ms (device {ms_gesamt:.1f} ms, pcg {ms_pcg:.1f} ms), "
-          f"{st['cg_iterationen']} CG iterations", file=sys.stderr, flush=True)
+          f"{st['cg_iterationen']} CG iterations, {st['nicht_konvergiert']} unconverged solves, "
+          f"longest barrier wait {st['barriere_spins_max']} polls", file=sys.stderr, flush=True)
 
     b = pcg_bytes(s, args.praezision)
     iters = st["cg_iterationen"]
@@ -152,6 +153,7 @@ def main() -> int:
                    "punkte": s.n, "dreiecke": int(s.dreiecke.shape[0]), "biegungen": b["biegungen"],
                    "selbstkollision": False, "graph": bool(args.graph), "praezision": args.praezision,
                    "cg_iterationen_je_schritt": round(iters / args.steps, 1), "loesungen": loesungen,
+                   "nicht_konvergiert": st["nicht_konvergiert"],
                    "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

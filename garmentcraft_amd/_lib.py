@@ -54,6 +54,7 @@ class Statistik_c(C.Structure):
     _fields_ = [
         ("cg_iterationen", C.c_int64), ("loesungen", C.c_int64), ("nicht_konvergiert", C.c_int64),
         ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
+        ("barriere_spins_max", C.c_int64),
     ]
 
 

@@ -74,7 +74,6 @@ struct SysDev {
   V3<T>* p[2];
   V3<T>* ap;
   V3<double>* dv_vorher;
-  int32_t hat_vorher;
   // tiles of the PCG operator: row ranges whose CSR entries (and the points
   // their constraints touch) fit in LDS
   int32_t n_kacheln;
@@ -94,7 +93,7 @@ struct SysDev {
 
 // Shared device state of the grid-wide reductions and the step's counters.
 struct GridState {
-  unsigned int zaehler;      // barrier arrivals (monotonic within a launch)
+  unsigned int zaehler;      // barrier arrivals (monotonic within a launch, reset before each)
   unsigned int fehler;       // spin limit hit
   unsigned int frei[2];      // a point was released in pass k
   unsigned int n_fest;       // points held along a normal
@@ -102,6 +101,8 @@ struct GridState {
   // statistics of the last step (device side)
   long long cg_iterationen, loesungen, nicht_konvergiert, kontakte, haftend, freigegeben;
   double rest_max;
+  unsigned int spins_max;    // longest barrier wait (polls) since the last read
+  unsigned int pad1;
   // result of the last solve
   int iterationen;
   int nicht_konv;
@@ -115,8 +116,11 @@ struct alignas(64) Slot {
 };
 
 // ---------------------------------------------------------------------------
-// Grid-wide all-reduce: block partials into per-block slots, one arrival
-// counter, relaxed polling, then every block sums all slots in the same fixed
+// Grid-wide all-reduce: block partials go into per-block slots, thread 0
+// releases and arrives on one counter that only grows within a launch (a
+// kernel before each launch sets it to 0), relaxed polling until it reaches
+// blocks x barriers so far, thread 0 acquires,
+// then every block sums all slots in the same fixed
 // order (so every block holds the identical value — deterministic and equal to
 // the bit across blocks). Slots are double-buffered by epoch parity: a block
 // can only overwrite parity p after every block has published epoch e+1,
@@ -151,8 +155,11 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
   const unsigned nb = gridDim.x;
   Slot* buf = slots + (epoch & 1u) * GC_MAX_BLOECKE;
   __shared__ int s_fehler;
+  // block_sum ended with a workgroup barrier: every wave's stores happen-before
+  // thread 0's agent-scope release below, which makes them visible to the
+  // other blocks (release cumulativity of the HSA memory model).
   if (threadIdx.x == 0) {
-    for (int k = 0; k < NV; ++k) buf[blockIdx.x].v[k] = v[k];
+    for (int k = 0; k < NV; ++k) __hip_atomic_store(&buf[blockIdx.x].v[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(&gs->zaehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned ziel = nb * epoch;
@@ -163,6 +170,7 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
       if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { fehler = 1; break; }
     }
     if (fehler) __hip_atomic_store(&gs->fehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(&gs->spins_max, spins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_fehler = fehler;
   }
@@ -234,7 +242,8 @@ __device__ __forceinline__ V3<T> filtern(const SysDev<T>& s, uint32_t i, V3<T> v
 // Build kernels (implizit.rs:725-844; device form of gpu.rs aufbau_k/aufbau_b)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_bau_kanten(SysDev<T> s) {
+__global__ void k_bau_kanten(const SysDev<T>* __restrict__ sp) {
+  const SysDev<T>& s = *sp;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.nk; c += gridDim.x * blockDim.x) {
     const uint32_t i = s.k_ij[2 * c], j = s.k_ij[2 * c + 1];
     const double k = s.k_k[c];
@@ -268,7 +277,8 @@ __global__ void k_bau_kanten(SysDev<T> s) {
 // Bending geometry in FP64 in both modes (the angle and its gradients are a
 // per-step cost); only the gradients the operator streams are stored as T.
 template <typename T>
-__global__ void k_bau_biegung(SysDev<T> s) {
+__global__ void k_bau_biegung(const SysDev<T>* __restrict__ sp) {
+  const SysDev<T>& s = *sp;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.nb; c += gridDim.x * blockDim.x) {
     const uint32_t* bp = s.b_p + 4 * c;
     V3<double> q[4];
@@ -291,7 +301,8 @@ __global__ void k_bau_biegung(SysDev<T> s) {
 // 0.5*(F^T F - 1) cancels at small stretch, which in FP32 costs ~1e-4 relative
 // in the force. Only the tangent the operator streams is stored as T.
 template <typename T>
-__global__ void k_bau_membran(SysDev<T> s) {
+__global__ void k_bau_membran(const SysDev<T>* __restrict__ sp) {
+  const SysDev<T>& s = *sp;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < s.nm; c += gridDim.x * blockDim.x) {
     const uint32_t* mp = s.m_p + 3 * c;
     const V3<double> x0 = s.x[mp[0]];
@@ -315,7 +326,8 @@ __global__ void k_bau_membran(SysDev<T> s) {
 
 // Contacts of the substep (implizit.rs:872-883) and the held-point flags.
 template <typename T>
-__global__ void k_fest(SysDev<T> s, GridState* gs) {
+__global__ void k_fest(const SysDev<T>* __restrict__ sp, GridState* gs) {
+  const SysDev<T>& s = *sp;
   unsigned lokal = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     uint8_t fl = 0;
@@ -351,7 +363,8 @@ __global__ void k_fest(SysDev<T> s, GridState* gs) {
 // (implizit.rs:722-723, 745-751, 767-768, 793-795, 819-825, 887-894; the device
 // form of gpu.rs sammeln).
 template <typename T>
-__global__ void k_sammeln(SysDev<T> s) {
+__global__ void k_sammeln(const SysDev<T>* __restrict__ sp) {
+  const SysDev<T>& s = *sp;
   for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < s.n; ii += gridDim.x * blockDim.x) {
     const uint32_t i = (uint32_t)ii;
     const double m = s.masse[i];
@@ -421,11 +434,11 @@ __global__ void k_sammeln(SysDev<T> s) {
 }
 
 // Per-step reset of the shared counters (a fill kernel, so a captured step
-// records no memset node).
+// records no memset node). The barrier error flag is sticky: the host reads
+// and clears it after every call, so a failure in any substep is reported.
 __global__ void k_reset(GridState* gs) {
   if (threadIdx.x == 0) {
     gs->zaehler = 0;
-    gs->fehler = 0;
     gs->frei[0] = 0;
     gs->frei[1] = 0;
     gs->n_fest = 0;
@@ -434,10 +447,11 @@ __global__ void k_reset(GridState* gs) {
 
 // Warm start (implizit.rs:906): dv of the previous step, h*g on the first.
 template <typename T>
-__global__ void k_warm(SysDev<T> s) {
+__global__ void k_warm(const SysDev<T>* __restrict__ sp, int hat_vorher) {
+  const SysDev<T>& s = *sp;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     const V3<double> gh = V3<double>{s.g[0], s.g[1], s.g[2]} * s.h;
-    s.dv[i] = cast3<T>(s.hat_vorher ? s.dv_vorher[i] : gh);
+    s.dv[i] = cast3<T>(hat_vorher ? s.dv_vorher[i] : gh);
   }
 }
 
@@ -529,7 +543,8 @@ __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const 
 // a point was released.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(1024) k_pcg(SysDev<T> s, GridState* gs, Slot* slots, int durchgang) {
+__global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
+  const SysDev<T>& s = *sp;
   extern __shared__ __align__(16) unsigned char lds_roh[];
   V3<T>* slot = (V3<T>*)lds_roh;
   V3<T>* ul = slot + s.kachel_slots_max;
@@ -665,7 +680,8 @@ __global__ void __launch_bounds__(1024) k_pcg(SysDev<T> s, GridState* gs, Slot* 
 // Release pass (implizit.rs:916-942): normal force from the residual
 // A*dv - b; a held point that pulls is released.
 template <typename T>
-__global__ void k_freigabe(SysDev<T> s, GridState* gs, int durchgang) {
+__global__ void k_freigabe(const SysDev<T>* __restrict__ sp, GridState* gs, int durchgang) {
+  const SysDev<T>& s = *sp;
   if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   if (__hip_atomic_load(&gs->n_fest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) gs->zaehler = 0;
@@ -692,7 +708,8 @@ __global__ void k_freigabe(SysDev<T> s, GridState* gs, int durchgang) {
 
 // Friction, velocity and position update, push-out (implizit.rs:945-1013).
 template <typename T>
-__global__ void k_update(SysDev<T> s, GridState* gs) {
+__global__ void k_update(const SysDev<T>* __restrict__ sp, GridState* gs) {
+  const SysDev<T>& s = *sp;
   unsigned kont = 0, haft = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     const V3<double> dvi = cast3<double>(s.dv[i]);
@@ -733,7 +750,8 @@ __global__ void k_update(SysDev<T> s, GridState* gs) {
 
 // Utility kernels for the Loeser-level entry points.
 template <typename T>
-__global__ void k_mal(SysDev<T> s, const V3<T>* p, V3<T>* out, int nur_h) {
+__global__ void k_mal(const SysDev<T>* __restrict__ sp, const V3<T>* p, V3<T>* out, int nur_h) {
+  const SysDev<T>& s = *sp;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     auto pf = [&](uint32_t j) { return p[j]; };
     out[i] = nur_h ? mal_h_zeile(s, (uint32_t)i, pf) : mal_a_zeile(s, (uint32_t)i, pf);

@@ -54,6 +54,10 @@ struct gc_ctx {
   std::vector<void*> allocs;
   SysDev<double> sd{};
   SysDev<float> sf{};
+  // device copy of the active system description (kernels read it through a
+  // pointer) and the bytes of its last upload
+  void* sys_dev = nullptr;
+  std::vector<unsigned char> sys_alt;
   GridState* gs = nullptr;
   Slot* slots = nullptr;
   bool topo = false, hat_param = false, built = false;
@@ -67,7 +71,6 @@ struct gc_ctx {
   double ms_schritt = 0.0, ms_pcg = 0.0;
   // graph of one substep
   hipGraphExec_t graph = nullptr;
-  double graph_h = -1.0;
 };
 
 #define HIPCK(ctx, call)                                                                      \
@@ -116,6 +119,26 @@ template <typename T> static SysDev<T>& sys(gc_ctx* c);
 template <> SysDev<double>& sys<double>(gc_ctx* c) { return c->sd; }
 template <> SysDev<float>& sys<float>(gc_ctx* c) { return c->sf; }
 
+template <typename T>
+static const SysDev<T>* dsys(gc_ctx* c) { return (const SysDev<T>*)c->sys_dev; }
+
+// Uploads the host copy of the system description when it differs from the
+// last upload. Called by every entry point before it launches kernels, never
+// inside a graph capture: it waits for the stream first, so no kernel still
+// reads the old copy. A captured graph reads the description at replay time,
+// so parameter and body updates reach it without a new capture.
+template <typename T>
+static int sys_hoch(gc_ctx* c) {
+  const SysDev<T>& s = sys<T>(c);
+  const unsigned char* roh = (const unsigned char*)&s;
+  if (c->sys_alt.size() == sizeof(SysDev<T>) && std::memcmp(c->sys_alt.data(), roh, sizeof(SysDev<T>)) == 0) return 0;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  HIPCK(c, hipMemcpyAsync(c->sys_dev, roh, sizeof(SysDev<T>), hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->sys_alt.assign(roh, roh + sizeof(SysDev<T>));
+  return 0;
+}
+
 static unsigned gitter(long count) {
   long b = (count + BUILD_THREADS - 1) / BUILD_THREADS;
   return (unsigned)std::max(1L, std::min(b, 8192L));
@@ -127,7 +150,10 @@ static unsigned gitter(long count) {
 template <typename T>
 static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const std::vector<uint32_t>& csr) {
   SysDev<T>& s = sys<T>(c);
-  s = SysDev<T>{};
+  std::memset((void*)&s, 0, sizeof(SysDev<T>));  // padding too: uploads compare raw bytes
+  c->sys_alt.clear();
+  c->sys_dev = dalloc<SysDev<T>>(c, 1);
+  if (!c->sys_dev) return fehler(c, "hipMalloc failed (system description)");
   const int n = c->n, nk = c->nk, nb = (int)c->b_k.size(), nm = (int)(c->m_p.size() / 3);
   s.n = n; s.nk = nk; s.nb = nb; s.nm = nm;
   uint32_t* d_off = dalloc<uint32_t>(c, off.size());
@@ -325,7 +351,6 @@ static void param_setzen(gc_ctx* c) {
   s.kontakt_rand = c->param.kontakt_rand;
   s.aus_last = c->param.normalkraft_aus_last;
   s.boden = c->msum * 1e-18;
-  s.hat_vorher = c->hat_vorher ? 1 : 0;
   s.n_koerper = (int)c->koerper.size();
   for (size_t b = 0; b < c->koerper.size() && b < GC_MAX_KOERPER; ++b) s.koerper[b] = c->koerper[b].k;
 }
@@ -339,42 +364,42 @@ static void alle_param(gc_ctx* c) {
 // launch sequences
 // ---------------------------------------------------------------------------
 template <typename T>
-static int launch_bau(gc_ctx* c, double h) {
-  SysDev<T>& s = sys<T>(c);
-  s.h = h;
-  s.h2 = h * h;
+static int launch_bau(gc_ctx* c) {
+  const SysDev<T>& s = sys<T>(c);
+  const SysDev<T>* sp = dsys<T>(c);
   hipStream_t st = c->stream;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->gs);
-  if (s.nk) hipLaunchKernelGGL(k_bau_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, s);
-  if (s.nb) hipLaunchKernelGGL(k_bau_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, s);
-  if (s.nm) hipLaunchKernelGGL(k_bau_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, s);
-  hipLaunchKernelGGL(k_fest<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs);
-  hipLaunchKernelGGL(k_sammeln<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s);
+  if (s.nk) hipLaunchKernelGGL(k_bau_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sp);
+  if (s.nb) hipLaunchKernelGGL(k_bau_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, sp);
+  if (s.nm) hipLaunchKernelGGL(k_bau_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_fest<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs);
+  hipLaunchKernelGGL(k_sammeln<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp);
   HIPCK(c, hipGetLastError());
   return 0;
 }
 
 template <typename T>
 static int launch_pcg(gc_ctx* c, int durchgang) {
-  SysDev<T>& s = sys<T>(c);
-  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, c->stream, s, c->gs, c->slots, durchgang);
+  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, c->stream, dsys<T>(c), c->gs, c->slots,
+                     durchgang);
   HIPCK(c, hipGetLastError());
   return 0;
 }
 
 template <typename T>
-static int launch_schritt(gc_ctx* c, double h, hipEvent_t e_pcg0, hipEvent_t e_pcg1) {
-  SysDev<T>& s = sys<T>(c);
-  if (launch_bau<T>(c, h)) return 1;
+static int launch_schritt(gc_ctx* c, int hat_vorher, hipEvent_t e_pcg0, hipEvent_t e_pcg1) {
+  const SysDev<T>& s = sys<T>(c);
+  const SysDev<T>* sp = dsys<T>(c);
+  if (launch_bau<T>(c)) return 1;
   hipStream_t st = c->stream;
-  hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s);
+  hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, hat_vorher);
   if (e_pcg0) HIPCK(c, hipEventRecord(e_pcg0, st));
   if (launch_pcg<T>(c, 0)) return 1;
   if (e_pcg1) HIPCK(c, hipEventRecord(e_pcg1, st));
-  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs, 0);
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 0);
   if (launch_pcg<T>(c, 1)) return 1;
-  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs, 1);
-  hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, s, c->gs);
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 1);
+  hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs);
   HIPCK(c, hipGetLastError());
   return 0;
 }
@@ -383,7 +408,11 @@ static int pruefe_gitter(gc_ctx* c) {
   GridState g;
   HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
-  if (g.fehler) return fehler(c, "grid barrier wait exceeded its spin limit (blocks not co-resident?)");
+  if (g.fehler) {
+    unsigned null = 0;
+    HIPCK(c, hipMemcpy(&c->gs->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
+    return fehler(c, "grid barrier wait exceeded its spin limit (blocks not co-resident?)");
+  }
   return 0;
 }
 
@@ -397,10 +426,12 @@ static int schritte(gc_ctx* c, double h, int n) {
       c->ev.push_back(e);
     }
   }
+  sys<T>(c).h = h;
+  sys<T>(c).h2 = h * h;
+  if (sys_hoch<T>(c)) return 1;
   HIPCK(c, hipEventRecord(c->ev[0], c->stream));
   for (int k = 0; k < n; ++k) {
-    sys<T>(c).hat_vorher = c->hat_vorher ? 1 : 0;
-    if (launch_schritt<T>(c, h, c->ev[2 + 2 * k], c->ev[3 + 2 * k])) return 1;
+    if (launch_schritt<T>(c, c->hat_vorher ? 1 : 0, c->ev[2 + 2 * k], c->ev[3 + 2 * k])) return 1;
     c->hat_vorher = true;
   }
   HIPCK(c, hipEventRecord(c->ev[1], c->stream));
@@ -429,22 +460,19 @@ static int schritte_graph(gc_ctx* c, double h, int n) {
     done = 1;
     if (n == 1) return 0;
   }
-  if (!c->graph || c->graph_h != h) {
-    if (c->graph) {
-      HIPCK(c, hipGraphExecDestroy(c->graph));
-      c->graph = nullptr;
-    }
-    sys<T>(c).hat_vorher = 1;
+  sys<T>(c).h = h;
+  sys<T>(c).h2 = h * h;
+  if (sys_hoch<T>(c)) return 1;
+  if (!c->graph) {
     hipGraph_t g;
     HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_schritt<T>(c, h, nullptr, nullptr);
+    int rc = launch_schritt<T>(c, 1, nullptr, nullptr);
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (rc) return 1;
     if (e2 != hipSuccess) return fehler(c, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
     hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e3 != hipSuccess) return fehler(c, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
-    c->graph_h = h;
   }
   if (c->ev.size() < 2) {
     while (c->ev.size() < 2) {
@@ -504,7 +532,8 @@ static int mal(gc_ctx* c, const double* p, double* out, int nur_h) {
   if (!c->built) return fehler(c, "no system built (call gc_build first)");
   SysDev<T>& s = sys<T>(c);
   if (hoch_vec<T>(c, s.ap, p)) return 1;
-  hipLaunchKernelGGL(k_mal<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, s, (const V3<T>*)s.ap, s.r, nur_h);
+  if (sys_hoch<T>(c)) return 1;
+  hipLaunchKernelGGL(k_mal<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, dsys<T>(c), (const V3<T>*)s.ap, s.r, nur_h);
   HIPCK(c, hipGetLastError());
   return runter_vec<T>(c, out, s.r, c->n);
 }
@@ -514,12 +543,17 @@ static int loese(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
   if (!c->built) return fehler(c, "no system built (call gc_build first)");
   SysDev<T>& s = sys<T>(c);
   if (hoch_vec<T>(c, s.dv, dv)) return 1;
+  if (sys_hoch<T>(c)) return 1;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, c->gs);
   if (launch_pcg<T>(c, 0)) return 1;
   GridState g;
   HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
   if (runter_vec<T>(c, dv, s.dv, c->n)) return 1;
-  if (g.fehler) return fehler(c, "grid barrier wait exceeded its spin limit");
+  if (g.fehler) {
+    unsigned null = 0;
+    HIPCK(c, hipMemcpy(&c->gs->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
+    return fehler(c, "grid barrier wait exceeded its spin limit");
+  }
   if (lauf) {
     lauf->iterationen = g.iterationen;
     lauf->nicht_konvergiert = g.nicht_konv;
@@ -903,7 +937,16 @@ int gc_set_dv_vorher(gc_ctx* c, const double* dv, int32_t n) {
 int gc_build(gc_ctx* c, double h) {
   if (!c || !c->topo || !c->hat_param) return c ? fehler(c, "gc_build: topology and parameters must be set first") : 1;
   if (!(h > 0.0)) return fehler(c, "h must be > 0");
-  int rc = c->opt.praezision == GC_FP32 ? launch_bau<float>(c, h) : launch_bau<double>(c, h);
+  int rc;
+  if (c->opt.praezision == GC_FP32) {
+    c->sf.h = h;
+    c->sf.h2 = h * h;
+    rc = sys_hoch<float>(c) || launch_bau<float>(c);
+  } else {
+    c->sd.h = h;
+    c->sd.h2 = h * h;
+    rc = sys_hoch<double>(c) || launch_bau<double>(c);
+  }
   if (rc) return rc;
   HIPCK(c, hipStreamSynchronize(c->stream));
   c->built = true;
@@ -999,10 +1042,12 @@ int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
     out->haftend = g.haftend;
     out->freigegeben = g.freigegeben;
     out->rest_max = g.rest_max;
+    out->barriere_spins_max = g.spins_max;
   }
   if (reset) {
     g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
     g.rest_max = 0.0;
+    g.spins_max = 0;
     HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));
   }
   return 0;

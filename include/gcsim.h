@@ -98,6 +98,7 @@ typedef struct gc_statistik {
   int64_t haftend;
   int64_t freigegeben;
   double rest_max;
+  int64_t barriere_spins_max;  /* longest grid-barrier wait in polls (diagnostic) */
 } gc_statistik;
 
 /* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):

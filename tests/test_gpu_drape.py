@@ -65,17 +65,28 @@ def test_system_mit_kontakt_wie_orakel(kleiner_drape, praezision):
     g.close()
 
 
-@pytest.mark.parametrize("praezision,tol_x", [("fp64", 1e-9), ("fp32", 2e-6)])
-def test_drape_einzelschritte_wie_orakel(kleiner_drape, praezision, tol_x):
+@pytest.mark.parametrize("praezision,rtol,tol_x,tol_mittel", [("fp64", 1e-10, 1e-9, 2e-10), ("fp32", 1e-6, 2e-4, 2e-5)])
+def test_drape_einzelschritte_wie_orakel(kleiner_drape, praezision, rtol, tol_x, tol_mittel):
     """Each step from the oracle's own state (x, v, warm start): the whole step
     (build, contacts, solve, release, friction, update) is compared, and the
     comparison is not blurred by the divergence that the stick/slip and release
     decisions cause over a contact trajectory (tests/gpu.rs:185-200 compares
-    equilibria instead of trajectories for the same reason)."""
+    equilibria instead of trajectories for the same reason).
+
+    Two PCG runs that sum in different orders stop at the same relative
+    residual with solutions that differ by up to that tolerance, so FP64 is
+    compared at rtol 1e-10 (positions to 1e-9 m), FP32 at the benchmark's rtol
+    1e-6: an FP32 PCG loses orthogonality on the hardest steps (more
+    iterations than FP64 for the same recursive residual), so its worst step
+    is held to 2e-4 m and the mean of the per-step maxima to 2e-5 m. Every
+    discrete decision (held set after release, contacts, sticking points,
+    releases) must agree exactly in both precisions."""
     s = kleiner_drape
-    o = O.OrakelSim(s, O.param(**P))
-    g = GeraetSzene(s, ImplizitParameter(**P), praezision=praezision)
+    pp = dict(P, cg_toleranz=rtol)
+    o = O.OrakelSim(s, O.param(**pp))
+    g = GeraetSzene(s, ImplizitParameter(**pp), praezision=praezision)
     schlechteste = 0.0
+    maxima = []
     kontakt_schritte = 0
     for k in range(40):
         x, v = o.lage()
@@ -92,11 +103,16 @@ def test_drape_einzelschritte_wie_orakel(kleiner_drape, praezision, tol_x):
         xb, vb = g.lage()
         d = float(np.max(np.linalg.norm(xa - xb, axis=1)))
         schlechteste = max(schlechteste, d)
+        maxima.append(d)
+        assert np.array_equal(o.sys_fest()["flag"], g.fest()["flag"]), k
+        assert nach["haftend"] - vor["haftend"] == sg["haftend"], k
         kontakt_schritte += (nach["kontakte"] - vor["kontakte"]) > 0
         assert nach["kontakte"] - vor["kontakte"] == sg["kontakte"], k
         assert nach["freigegeben"] - vor["freigegeben"] == sg["freigegeben"], k
         assert d <= tol_x, (k, d)
-    print(praezision, "worst single-step position difference", schlechteste, "steps with contact", kontakt_schritte)
+    print(praezision, "worst single-step position difference", schlechteste, "mean", float(np.mean(maxima)),
+          "median", float(np.median(maxima)), "steps with contact", kontakt_schritte)
+    assert float(np.mean(maxima)) <= tol_mittel
     assert kontakt_schritte >= 10
     g.close()
 
