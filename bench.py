@@ -83,7 +83,6 @@ def main() -> int:
         step(dt, 1)
     g.statistik(reset=True)
     x_w, v_w = g.lage()
-    dv_w = None
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
     t_a = time.perf_counter()
@@ -94,7 +93,6 @@ def main() -> int:
     wall_ms = (t_b - t_a) * 1e3
     ms_per_step = wall_ms / args.steps
     value = args.steps / (wall_ms / 1e3)
-    x_end, _ = g.lage()
     print(f"[bench] {args.steps} steps in {wall_ms:.1f} ms (device {ms_gesamt:.1f} ms, pcg {ms_pcg:.1f} ms), "
           f"{st['cg_iterationen']} CG iterations, {st['nicht_konvergiert']} unconverged solves, "
           f"longest barrier wait {st['barriere_spins_max']} polls", file=sys.stderr, flush=True)
@@ -104,9 +102,9 @@ def main() -> int:
     loesungen = st["loesungen"]
     roofline = None
     if not args.graph and ms_pcg == ms_pcg and ms_pcg > 0:
-        # first PCG launch of every step, bracketed by events on its stream
-        erste = args.steps
-        algo = iters * b["iteration"] + erste * b["start"]
+        # both PCG launches of every step (first solve, solve after a
+        # release), bracketed by HIP events on the stream they run on
+        algo = iters * b["iteration"] + loesungen * b["start"]
         achieved = algo / (ms_pcg * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

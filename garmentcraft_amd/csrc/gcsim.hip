@@ -387,17 +387,20 @@ static int launch_pcg(gc_ctx* c, int durchgang) {
 }
 
 template <typename T>
-static int launch_schritt(gc_ctx* c, int hat_vorher, hipEvent_t e_pcg0, hipEvent_t e_pcg1) {
+// ev: four events (before/after each PCG launch) or nullptr
+static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
   const SysDev<T>& s = sys<T>(c);
   const SysDev<T>* sp = dsys<T>(c);
   if (launch_bau<T>(c)) return 1;
   hipStream_t st = c->stream;
   hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, hat_vorher);
-  if (e_pcg0) HIPCK(c, hipEventRecord(e_pcg0, st));
+  if (ev) HIPCK(c, hipEventRecord(ev[0], st));
   if (launch_pcg<T>(c, 0)) return 1;
-  if (e_pcg1) HIPCK(c, hipEventRecord(e_pcg1, st));
+  if (ev) HIPCK(c, hipEventRecord(ev[1], st));
   hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 0);
+  if (ev) HIPCK(c, hipEventRecord(ev[2], st));
   if (launch_pcg<T>(c, 1)) return 1;
+  if (ev) HIPCK(c, hipEventRecord(ev[3], st));
   hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 1);
   hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs);
   HIPCK(c, hipGetLastError());
@@ -419,8 +422,8 @@ static int pruefe_gitter(gc_ctx* c) {
 template <typename T>
 static int schritte(gc_ctx* c, double h, int n) {
   if (!c->topo || !c->hat_param) return fehler(c, "gc_step: topology and parameters must be set first");
-  if ((int)c->ev.size() < 2 * n + 2) {
-    while ((int)c->ev.size() < 2 * n + 2) {
+  if ((int)c->ev.size() < 4 * n + 2) {
+    while ((int)c->ev.size() < 4 * n + 2) {
       hipEvent_t e;
       HIPCK(c, hipEventCreate(&e));
       c->ev.push_back(e);
@@ -431,7 +434,7 @@ static int schritte(gc_ctx* c, double h, int n) {
   if (sys_hoch<T>(c)) return 1;
   HIPCK(c, hipEventRecord(c->ev[0], c->stream));
   for (int k = 0; k < n; ++k) {
-    if (launch_schritt<T>(c, c->hat_vorher ? 1 : 0, c->ev[2 + 2 * k], c->ev[3 + 2 * k])) return 1;
+    if (launch_schritt<T>(c, c->hat_vorher ? 1 : 0, &c->ev[2 + 4 * k])) return 1;
     c->hat_vorher = true;
   }
   HIPCK(c, hipEventRecord(c->ev[1], c->stream));
@@ -441,9 +444,10 @@ static int schritte(gc_ctx* c, double h, int n) {
   c->ms_schritt = ms;
   double pcg = 0.0;
   for (int k = 0; k < n; ++k) {
-    float m2 = 0.f;
-    HIPCK(c, hipEventElapsedTime(&m2, c->ev[2 + 2 * k], c->ev[3 + 2 * k]));
-    pcg += m2;
+    float m0 = 0.f, m1 = 0.f;
+    HIPCK(c, hipEventElapsedTime(&m0, c->ev[2 + 4 * k], c->ev[3 + 4 * k]));
+    HIPCK(c, hipEventElapsedTime(&m1, c->ev[4 + 4 * k], c->ev[5 + 4 * k]));
+    pcg += (double)m0 + (double)m1;
   }
   c->ms_pcg = pcg;
   c->built = false;
@@ -466,7 +470,7 @@ static int schritte_graph(gc_ctx* c, double h, int n) {
   if (!c->graph) {
     hipGraph_t g;
     HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_schritt<T>(c, 1, nullptr, nullptr);
+    int rc = launch_schritt<T>(c, 1, nullptr);
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (rc) return 1;
     if (e2 != hipSuccess) return fehler(c, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));

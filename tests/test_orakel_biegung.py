@@ -96,3 +96,17 @@ def test_builder_restatements_bitgleich_mit_orakel():
         c2, s2 = S.cos2_sin2(e, dir2)
         w3 = tuple(rng.uniform(1e-7, 1e-5, size=3))
         assert S.steifigkeit_aus(c2, s2, *w3) == O.steifigkeit_aus(c2, s2, *w3)
+
+
+def test_cos2_sin2_ist_der_winkel_der_normalen_zum_kettfaden():
+    """biegung.rs:156-172: squared cosine and sine of the angle between the
+    edge's in-plane normal and the warp, i.e. (sin^2, cos^2) of the edge's own
+    angle to the warp, measured here with atan2; degenerate vectors give None."""
+    rng = np.random.default_rng(8)
+    for _ in range(500):
+        e, w = rng.normal(size=2), rng.normal(size=2)
+        a = math.atan2(e[1], e[0]) - math.atan2(w[1], w[0])
+        c2, s2 = O.cos2_sin2(e, w)
+        assert abs(c2 - math.sin(a) ** 2) < 1e-14 and abs(s2 - math.cos(a) ** 2) < 1e-14
+    assert O.cos2_sin2([0.0, 0.0], [1.0, 0.0]) is None
+    assert O.cos2_sin2([1.0, 0.0], [1e-13, 0.0]) is None
