@@ -114,6 +114,11 @@ class Szene:
     zeit: float = 0.0
     membran_entartet: int = 0
     naht_biegung_ohne: int = 0
+    # self-collision inputs (selbstkollision.rs:660-680): stretch stiffness
+    # (N/m) of each point's and each triangle's piece, rest area per triangle
+    selbst_e_punkt: Optional[np.ndarray] = None
+    selbst_e_dreieck: Optional[np.ndarray] = None
+    selbst_a_dreieck: Optional[np.ndarray] = None
 
     @property
     def n(self) -> int:
@@ -310,6 +315,7 @@ def bauen(teile: Sequence[Teil], naehte: Sequence[Sequence[Teilnaht]],
     punkte_m_l = []
     biegesteif = []
     dreiecke_all = []
+    e_punkt_l, e_dreieck_l, a_dreieck_l = [], [], []
     membran_records = []      # (piece, local tri, global tri, uv)
     kanten_index_je_teil = []
     off = 0
@@ -351,8 +357,11 @@ def bauen(teile: Sequence[Teil], naehte: Sequence[Sequence[Teilnaht]],
                 # chart coordinates of the edge's end points in this triangle
                 e[2].setdefault(k_t, {i: q[k], j: q[(k + 1) % 3], o: q[(k + 2) % 3]})
             dreiecke_all.append([int(d[0]) + off, int(d[1]) + off, int(d[2]) + off])
+            e_dreieck_l.append(mat.dehnung_n_m)
+            a_dreieck_l.append(a * skala * skala)
             membran_records.append((ti, k_t, [int(d[0]), int(d[1]), int(d[2])], q))
         masse_l.append(np.array(masse_t))
+        e_punkt_l.append(np.full(n_t, mat.dehnung_n_m))
         rand = {}
         kanten_e = {}
         dir2 = t.fadenlauf
@@ -442,6 +451,7 @@ def bauen(teile: Sequence[Teil], naehte: Sequence[Sequence[Teilnaht]],
     kollisionsabstand = np.concatenate(koll)
     reibung = np.concatenate(reib)
     teil_von_punkt = np.concatenate(teilnr)
+    e_punkt = np.concatenate(e_punkt_l)
 
     # held extra points joined by seam springs (used by the hanging tube's ring)
     if extra_punkte is not None:
@@ -452,6 +462,7 @@ def bauen(teile: Sequence[Teil], naehte: Sequence[Sequence[Teilnaht]],
         kollisionsabstand = np.concatenate([kollisionsabstand, np.zeros(ex.shape[0])])
         reibung = np.concatenate([reibung, np.zeros(ex.shape[0])])
         teil_von_punkt = np.concatenate([teil_von_punkt, np.full(ex.shape[0], len(teile), dtype=np.uint32)])
+        e_punkt = np.concatenate([e_punkt, np.zeros(ex.shape[0])])
         for (c, e) in extra_punkte["naht"]:
             n_p.append((min(c, base + e), max(c, base + e)))
 
@@ -516,4 +527,6 @@ def bauen(teile: Sequence[Teil], naehte: Sequence[Sequence[Teilnaht]],
         teil_von_punkt=teil_von_punkt,
         koerper=list(koerper), param=param,
         membran_entartet=membran_entartet, naht_biegung_ohne=naht_biegung_ohne,
+        selbst_e_punkt=e_punkt, selbst_e_dreieck=np.asarray(e_dreieck_l, dtype=np.float64),
+        selbst_a_dreieck=np.asarray(a_dreieck_l, dtype=np.float64),
     )

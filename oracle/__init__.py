@@ -30,6 +30,7 @@ class Szene_c(C.Structure):
         ("n_biegung", C.c_int32), ("biegung_punkte", U32), ("biegung_ruhe", D), ("biegung_nachgiebigkeit", D),
         ("n_membran", C.c_int32), ("membran_punkte", U32), ("membran_daten", D),
         ("n_dreiecke", C.c_int32), ("dreiecke", U32), ("teil_von_punkt", U32),
+        ("selbst_e_punkt", D), ("selbst_e_dreieck", D), ("selbst_a_dreieck", D),
         ("bild_dt", C.c_double), ("schwerkraft", C.c_double * 3), ("daempfung", C.c_double),
     ]
 
@@ -48,11 +49,12 @@ class Statistik_c(C.Structure):
         ("cg_iterationen", C.c_int64), ("loesungen", C.c_int64), ("nicht_konvergiert", C.c_int64),
         ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
         ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
+        ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64),
     ]
 
 
 def bauen_falls_noetig() -> None:
-    src = [os.path.join(_HIER, f) for f in ("gcoracle.cpp", "gcoracle.h", "selbstkollision.cpp")]
+    src = [os.path.join(_HIER, f) for f in ("gcoracle.cpp", "gcoracle.h", "selbstkollision.h")]
     if not os.path.exists(_SO) or any(os.path.getmtime(s) > os.path.getmtime(_SO) for s in src):
         subprocess.run(["make", "-C", _HIER, "-s"], check=True)
 
@@ -154,6 +156,10 @@ class OrakelSim:
         c.n_dreiecke = s.dreiecke.shape[0]
         c.dreiecke = _p(arr(s.dreiecke, np.uint32), U32)
         c.teil_von_punkt = _p(arr(s.teil_von_punkt, np.uint32), U32)
+        for name in ("selbst_e_punkt", "selbst_e_dreieck", "selbst_a_dreieck"):
+            a = getattr(s, name, None)
+            if a is not None:
+                setattr(c, name, _p(arr(a, np.float64)))
         c.bild_dt = s.param.bild_dt
         for q in range(3):
             c.schwerkraft[q] = s.param.schwerkraft[q]
@@ -276,6 +282,84 @@ class OrakelSim:
 
     def sys_rest(self, dv):
         o = np.zeros((self.n, 3)); lib().or_sys_rest(self.h, _p(f64(dv)), _p(o)); return o
+
+    # self-collision (selbstkollision.rs) ------------------------------------
+    def selbst_naehe(self, brute: bool = False) -> dict:
+        """Proximity pairs of the current state, in the reference's order
+        (brute: every entry instead of the hash buckets)."""
+        L = lib()
+        m = L.or_selbst_naehe(self.h, int(brute), 0, None, None, None, None, None, None)
+        return _naehe_lesen(lambda *a: L.or_selbst_naehe(self.h, int(brute), *a), m)
+
+    def sys_federn(self) -> dict:
+        L = lib()
+        m = L.or_sys_federn(self.h, 0, None, None, None, None)
+        p = np.zeros((m, 5), np.uint32); g = np.zeros((m, 5)); k = np.zeros((m, 3)); b = np.zeros((m, 3, 3))
+        L.or_sys_federn(self.h, m, _p(p, U32), _p(g), _p(k), _p(b))
+        return {"punkte": p, "gewichte": g, "kraft": k, "block": b}
+
+    def v(self):
+        o = np.zeros((self.n, 3)); lib().or_sim_v(self.h, _p(o)); return o
+
+    def selbst_schnitte(self) -> dict:
+        L = lib()
+        m = L.or_selbst_schnitte(self.h, 0, None, None, None, None, None, None)
+        kd = np.zeros((m, 2), np.uint32); t = np.zeros(m); w = np.zeros((m, 3)); r = np.zeros((m, 3)); g = np.zeros(m)
+        wz = np.zeros(m, np.uint32)
+        L.or_selbst_schnitte(self.h, m, _p(kd, U32), _p(t), _p(w), _p(r), _p(g), _p(wz, U32))
+        return {"kante_dreieck": kd, "t": t, "w": w, "richtung": r, "gewicht": g, "wurzel": wz}
+
+    def selbst_stoesse(self, x0, x1) -> dict:
+        L = lib()
+        a, b = f64(x0), f64(x1)
+        m = L.or_selbst_stoesse(self.h, _p(a), _p(b), 0, None, None, None, None, None, None)
+        return _stoesse_lesen(lambda *q: L.or_selbst_stoesse(self.h, _p(a), _p(b), *q), m)
+
+
+def _naehe_lesen(fn, m) -> dict:
+    art = np.zeros(m, np.int32); p = np.zeros((m, 4), np.uint32); g = np.zeros((m, 4)); nr = np.zeros((m, 3))
+    ab = np.zeros(m); ru = np.zeros(m)
+    fn(m, _p(art, I32), _p(p, U32), _p(g), _p(nr), _p(ab), _p(ru))
+    return {"art": art, "punkte": p, "gewichte": g, "normale": nr, "abstand": ab, "ruhe": ru}
+
+
+def _stoesse_lesen(fn, m) -> dict:
+    art = np.zeros(m, np.int32); p = np.zeros((m, 4), np.uint32); lage = np.zeros((m, 2)); zeit = np.zeros(m)
+    seite = np.zeros(m); nr = np.zeros((m, 3))
+    fn(m, _p(art, I32), _p(p, U32), _p(lage), _p(zeit), _p(seite), _p(nr))
+    return {"art": art, "punkte": p, "lage": lage, "zeit": zeit, "seite": seite, "normale": nr}
+
+
+def netz_naehe(x, w, dreiecke, r_punkt, punkt_dreieck=True, kante_kante=True) -> dict:
+    """finde (selbstkollision.rs:387-398) on a bare mesh from triangles and radii."""
+    L = lib()
+    x, w, r = f64(x), f64(w), f64(r_punkt)
+    d = np.ascontiguousarray(dreiecke, dtype=np.uint32)
+    n, nt = x.shape[0], d.shape[0]
+    args = (n, _p(x), _p(w), nt, _p(d, U32), _p(r), int(punkt_dreieck), int(kante_kante))
+    m = L.or_netz_naehe(*args, 0, None, None, None, None, None, None)
+    return _naehe_lesen(lambda *a: L.or_netz_naehe(*args, *a), m)
+
+
+def netz_stoesse(x0, x1, w, masse, dreiecke=None, kanten=None, punkt_dreieck=True, kante_kante=True, iterationen=1):
+    """finde_stoesse then stossantwort (selbstkollision.rs:1196-1343) on a bare
+    mesh; returns (impacts, corrected x1, number of corrections)."""
+    L = lib()
+    x0 = f64(x0)
+    x1 = f64(x1).copy()
+    w, masse = f64(w), f64(masse)
+    d = np.ascontiguousarray(dreiecke if dreiecke is not None else np.zeros((0, 3)), dtype=np.uint32)
+    k = np.ascontiguousarray(kanten if kanten is not None else np.zeros((0, 2)), dtype=np.uint32)
+    n = x0.shape[0]
+    korr = C.c_int32()
+    probe = x1.copy()
+    m = L.or_netz_stoesse(n, _p(x0), _p(probe), _p(w), _p(masse), d.shape[0], _p(d, U32), k.shape[0], _p(k, U32),
+                          int(punkt_dreieck), int(kante_kante), 0, 0, None, None, None, None, None, C.byref(korr))
+    art = np.zeros(m, np.int32); p = np.zeros((m, 4), np.uint32); lage = np.zeros((m, 2)); zeit = np.zeros(m); seite = np.zeros(m)
+    L.or_netz_stoesse(n, _p(x0), _p(x1), _p(w), _p(masse), d.shape[0], _p(d, U32), k.shape[0], _p(k, U32),
+                      int(punkt_dreieck), int(kante_kante), int(iterationen), m, _p(art, I32), _p(p, U32), _p(lage),
+                      _p(zeit), _p(seite), C.byref(korr))
+    return {"art": art, "punkte": p, "lage": lage, "zeit": zeit, "seite": seite}, x1, korr.value
 
 
 # element-level helpers -------------------------------------------------------

@@ -8,10 +8,13 @@
 // (-ffp-contract=off), the numbers are those the reference CPU path computes.
 #include "gcoracle.h"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <set>
 #include <vector>
 
 namespace {
@@ -567,6 +570,11 @@ double capped_cylinder_sdf(const double c[2], double rad, double z0, double z1, 
 }
 
 // ---------------------------------------------------------------------------
+// Cloth self-collision (selbstkollision.rs)
+// ---------------------------------------------------------------------------
+#include "selbstkollision.h"
+
+// ---------------------------------------------------------------------------
 // The system of one substep (implizit.rs:78-280) and the solver (283-427)
 // ---------------------------------------------------------------------------
 struct Kante { uint32_t i, j; double k; V3 richtung; double quer; double ruhe; };  // implizit.rs:79-89
@@ -587,6 +595,9 @@ struct Szene {
   std::vector<MembranElement> membran_elemente;
   std::vector<std::array<uint32_t, 3>> dreiecke;
   std::vector<uint32_t> teil_von_punkt;
+  // self-collision inputs (selbstkollision.rs:623-693): stretch stiffness per
+  // point and per triangle, rest area per triangle
+  std::vector<double> selbst_e_punkt, selbst_e_dreieck, selbst_a_dreieck;
   std::vector<Koerper> koerper;
   double bild_dt = 1.0 / 60.0;
   V3 schwerkraft{0, 0, -9.81};
@@ -602,12 +613,13 @@ struct System {  // implizit.rs:123-172
   std::vector<Kante> kanten, naehte;
   std::vector<Biegung> biegungen;
   std::vector<Hessen> membran_h;
+  std::vector<Feder> federn;
   std::vector<V3> rhs, f, last;
   std::vector<M3> pinv;
 
   size_t n() const { return s->masse.size(); }
 
-  // implizit.rs:185-221 (springs of self-collision are not part of this restatement)
+  // implizit.rs:185-221
   void mal_h(const V3* p, V3* out) const {
     size_t n = this->n();
     for (size_t i = 0; i < n; ++i) out[i] = {0.0, 0.0, 0.0};
@@ -634,6 +646,12 @@ struct System {  // implizit.rs:123-172
       mal_h_aus(me[t].element, membran_h[t], d3, hu);
       for (int k = 0; k < 3; ++k) out[me[t].punkte[k]] = add(out[me[t].punkte[k]], hu[k]);
     }
+    for (const auto& fe : federn)
+      feder_paare(fe, [&](uint32_t q, uint32_t r, double om) {
+        V3 t = mul(m3_mal(fe.block, sub(p[q], p[r])), om);
+        out[q] = add(out[q], t);
+        out[r] = sub(out[r], t);
+      });
   }
   // implizit.rs:224-229
   void mal_a(const V3* p, V3* out) const {
@@ -668,6 +686,12 @@ struct System {  // implizit.rs:123-172
         for (int a = 0; a < 3; ++a)
           for (int b = 0; b < 3; ++b) dg[me[t].punkte[k]][a][b] += h2 * bl[k][a][b];
     }
+    for (const auto& fe : federn)
+      feder_paare(fe, [&](uint32_t q, uint32_t r, double om) {
+        for (uint32_t pp : {q, r})
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) dg[pp][a][b] += h2 * om * fe.block[a][b];
+      });
     return dg;
   }
 };
@@ -747,12 +771,111 @@ struct or_sim {
   std::vector<V3> dv_vorher;
   System sys;
   std::vector<Fest> fest;
+  // self-collision state (implizit.rs:580-584): mesh and material, built on
+  // first use; springs of the step; cuts of the step; steps in a row with a cut
+  bool hat_selbst = false;
+  Stoffnetz netz;
+  Stoffwerte werte;
+  std::vector<Feder> federn;
+  std::vector<Schnitt> schnitte;
+  std::vector<uint32_t> folge;
+
+  // selbstkollision.rs:623-693 (no folded allowance bands in these scenes:
+  // band_von stays empty)
+  void selbst_bauen() {
+    size_t n = s.x.size();
+    std::vector<double> r(n);
+    for (size_t i = 0; i < n; ++i) r[i] = s.kollisionsabstand[i] + param.selbst_zusatzdicke / 2.0;
+    netz = Stoffnetz::neu(s.dreiecke, r);
+    netz.naht_partner.assign(n, {});
+    for (const auto& nt : s.naht_punkte) {
+      netz.naht_partner[nt[0]].push_back(nt[1]);
+      netz.naht_partner[nt[1]].push_back(nt[0]);
+    }
+    for (size_t e = 0; e < s.abstand_punkte.size(); ++e) {
+      uint32_t i = s.abstand_punkte[e][0], j = s.abstand_punkte[e][1];
+      if (s.abstand_ruhe[e] != 0.0 || i == j) continue;
+      auto& a = netz.naht_partner[i];
+      if (std::find(a.begin(), a.end(), j) == a.end()) a.push_back(j);
+      auto& b = netz.naht_partner[j];
+      if (std::find(b.begin(), b.end(), i) == b.end()) b.push_back(i);
+    }
+    netz.teil_von = s.teil_von_punkt;
+    netz.band_von.assign(n, KEIN);
+    netz.dicke = s.kollisionsabstand;
+    size_t nt = s.dreiecke.size();
+    werte = Stoffwerte();
+    werte.e_punkt = s.selbst_e_punkt;
+    werte.e_punkt.resize(n, 0.0);
+    werte.e_dreieck = s.selbst_e_dreieck;
+    werte.e_dreieck.resize(nt, 0.0);
+    werte.a_dreieck = s.selbst_a_dreieck;
+    werte.a_dreieck.resize(nt, 0.0);
+    werte.a_punkt.assign(n, 0.0);
+    for (size_t t = 0; t < nt; ++t)
+      for (uint32_t i : s.dreiecke[t]) werte.a_punkt[i] += werte.a_dreieck[t] / 3.0;
+    for (const auto& kd : netz.kanten_dreiecke) {
+      double ka = 0.0, ke = 0.0;
+      int cnt = 0;
+      for (uint32_t t : kd)
+        if (t != KEIN) { ka += werte.a_dreieck[t]; ke += werte.e_dreieck[t]; ++cnt; }
+      double k = (double)std::max(cnt, 1);
+      werte.a_kante.push_back(ka / k);
+      werte.e_kante.push_back(ke / k);
+    }
+    werte.mu_punkt = s.reibung;
+    werte.naht_partner.assign(n, {});
+    for (size_t c = 0; c < s.naht_punkte.size(); ++c) {
+      uint32_t i = s.naht_punkte[c][0], j = s.naht_punkte[c][1];
+      werte.naht_partner[i].push_back({j, (uint32_t)c});
+      werte.naht_partner[j].push_back({i, (uint32_t)c});
+    }
+    hat_selbst = true;
+  }
+
+  // implizit.rs:668-707: proximity springs (damping the approach in v),
+  // cuts, regions and push springs
+  void selbst_federn() {
+    federn.clear();
+    schnitte.clear();
+    if (!param.selbstkollision) return;
+    if (!hat_selbst || netz.r_punkt.size() != s.x.size()) selbst_bauen();
+    std::vector<Naehe> naehe = finde(netz, s.x, s.w, true, true);
+    for (const auto& p : naehe) {
+      if (p.art == PUNKT_DREIECK) stat.naehe_punkt_dreieck += 1;
+      else stat.naehe_kante_kante += 1;
+    }
+    federn = ::federn(netz, werte, naehe, param.selbst_steifigkeit, s.masse, s.w, s.naht_ruhe, s.v);
+    std::vector<Schnitt> sn = finde_schnitte(netz, s.x, param.selbst_schnitte_an_naehten != 0);
+    std::vector<size_t> wurzeln = gebiete(netz, sn);
+    stat.durchdringungen += (int64_t)sn.size();
+    for (size_t i = 0; i < wurzeln.size(); ++i) stat.durchdringungsgebiete += wurzeln[i] == i ? 1 : 0;
+    double rand = param.kontakt_rand;
+    auto normale = [&](size_t i, V3* out) {
+      for (const auto& k : s.koerper) {
+        Kontakt kt;
+        if (k.kontakt(s.x[i], s.kollisionsabstand[i] + rand, &kt)) { *out = kt.normale; return true; }
+      }
+      return false;
+    };
+    auto eingeklemmt = [&](const std::array<uint32_t, 2>& kp, const std::array<uint32_t, 3>& dp) {
+      V3 summe{0.0, 0.0, 0.0};
+      for (uint32_t i : dp) { V3 nn; if (normale(i, &nn)) summe = add(summe, nn); }
+      if (!(norm(summe) > 0.0)) return false;
+      for (uint32_t i : kp) { V3 nk; if (normale(i, &nk) && dot(nk, summe) < 0.0) return true; }
+      return false;
+    };
+    std::vector<Feder> schub = schubfedern(netz, werte, sn, wurzeln, param.selbst_steifigkeit, folge, eingeklemmt);
+    federn.insert(federn.end(), schub.begin(), schub.end());
+    schnitte = std::move(sn);
+  }
 
   // implizit.rs:663-896 (glue zones and self-collision springs are not restated here)
   void system_bauen(double h) {
     size_t n = s.x.size();
     V3 g = s.schwerkraft;
     double c = s.daempfung;
+    selbst_federn();
     std::vector<V3> f(n), last(n);
     for (size_t i = 0; i < n; ++i) f[i] = sub(mul(g, s.masse[i]), mul(s.v[i], c * s.masse[i]));
     for (size_t i = 0; i < n; ++i) last[i] = mul(g, s.masse[i]);
@@ -806,6 +929,11 @@ struct or_sim {
       bg.ruhe = ruhe;
       biegungen.push_back(bg);
     }
+    for (const auto& fe : federn)
+      for (int q = 0; q < 5; ++q) {
+        uint32_t i = fe.punkte[q];
+        f[i] = add(f[i], mul(fe.kraft, fe.gewichte[q]));
+      }
     for (const auto& m : s.membran_elemente) {
       V3 p3[3] = {s.x[m.punkte[0]], s.x[m.punkte[1]], s.x[m.punkte[2]]};
       V3 fe[3];
@@ -825,6 +953,7 @@ struct or_sim {
     sys.h = h; sys.c = c; sys.cg_max = param.cg_max; sys.cg_toleranz = param.cg_toleranz; sys.s = &s;
     sys.kanten = std::move(kanten); sys.naehte = std::move(naehte); sys.biegungen = std::move(biegungen);
     sys.membran_h = std::move(mh);
+    sys.federn = federn;
     fest.assign(n, Fest());
     for (size_t i = 0; i < n; ++i) {
       if (s.w[i] <= 0.0) continue;
@@ -854,6 +983,8 @@ struct or_sim {
   void schritt(double h) {
     size_t n = s.x.size();
     V3 g = s.schwerkraft;
+    std::vector<V3> x_anfang;
+    if (param.selbstkollision) x_anfang = s.x;
     system_bauen(h);
     std::vector<V3> dv;
     if (dv_vorher.size() == n) dv = dv_vorher;
@@ -906,6 +1037,16 @@ struct or_sim {
           double vn = dot(sub(s.v[i], kt.geschwindigkeit), kt.normale);
           if (vn < 0.0) s.v[i] = sub(s.v[i], mul(kt.normale, vn));
         }
+      }
+    }
+    // implizit.rs:1030-1037: impacts on the path x_anfang -> x, held points x 1e9
+    if (param.selbstkollision && hat_selbst) {
+      std::vector<Stoss> st = finde_stoesse(netz, x_anfang, s.x, s.w, true, true);
+      st = ohne_durchdrungene(netz, st, schnitte);
+      if (!st.empty()) {
+        std::vector<double> m_eff(n);
+        for (size_t i = 0; i < n; ++i) m_eff[i] = fest[i].da ? s.masse[i] * 1e9 : s.masse[i];
+        stat.selbst_stoesse += (int64_t)stossantwort(st, s.x, m_eff, s.w, (size_t)param.selbst_stoss_iterationen);
       }
     }
   }
@@ -1036,6 +1177,9 @@ or_sim* or_sim_neu(const or_szene* d, const or_param* p) {
   }
   for (int t = 0; t < d->n_dreiecke; ++t) s.dreiecke.push_back({d->dreiecke[3 * t], d->dreiecke[3 * t + 1], d->dreiecke[3 * t + 2]});
   if (d->teil_von_punkt) s.teil_von_punkt.assign(d->teil_von_punkt, d->teil_von_punkt + n);
+  if (d->selbst_e_punkt) s.selbst_e_punkt.assign(d->selbst_e_punkt, d->selbst_e_punkt + n);
+  if (d->selbst_e_dreieck) s.selbst_e_dreieck.assign(d->selbst_e_dreieck, d->selbst_e_dreieck + d->n_dreiecke);
+  if (d->selbst_a_dreieck) s.selbst_a_dreieck.assign(d->selbst_a_dreieck, d->selbst_a_dreieck + d->n_dreiecke);
   s.bild_dt = d->bild_dt;
   s.schwerkraft = v3(d->schwerkraft);
   s.daempfung = d->daempfung;
@@ -1169,6 +1313,120 @@ void or_sys_rest(const or_sim* s, const double* dv, double* out) {
   or_sys_mal_a(s, dv, out);
   for (size_t i = 0; i < s->sys.rhs.size(); ++i)
     for (int a = 0; a < 3; ++a) out[3 * i + a] -= s->sys.rhs[i][a];
+}
+
+static void naehe_aus(const std::vector<Naehe>& l, int32_t max, int32_t* art, uint32_t* punkte4, double* gewichte4,
+                      double* normale3, double* abstand, double* ruhe) {
+  for (size_t q = 0; q < l.size() && (int32_t)q < max; ++q) {
+    const Naehe& p = l[q];
+    if (art) art[q] = (int32_t)p.art;
+    for (int k = 0; k < 4; ++k) {
+      if (punkte4) punkte4[4 * q + k] = p.punkte[k];
+      if (gewichte4) gewichte4[4 * q + k] = p.gewichte[k];
+    }
+    if (normale3) put(normale3 + 3 * q, p.normale);
+    if (abstand) abstand[q] = p.abstand;
+    if (ruhe) ruhe[q] = p.ruhe;
+  }
+}
+static void stoesse_aus(const std::vector<Stoss>& l, int32_t max, int32_t* art, uint32_t* punkte4, double* lage2,
+                        double* zeit, double* seite, double* normale3) {
+  for (size_t q = 0; q < l.size() && (int32_t)q < max; ++q) {
+    const Stoss& p = l[q];
+    if (art) art[q] = (int32_t)p.art;
+    for (int k = 0; k < 4; ++k)
+      if (punkte4) punkte4[4 * q + k] = p.punkte[k];
+    if (lage2) { lage2[2 * q] = p.lage[0]; lage2[2 * q + 1] = p.lage[1]; }
+    if (zeit) zeit[q] = p.zeit;
+    if (seite) seite[q] = p.seite;
+    if (normale3) put(normale3 + 3 * q, p.normale);
+  }
+}
+int32_t or_selbst_naehe(or_sim* s, int32_t brute, int32_t max, int32_t* art, uint32_t* punkte4, double* gewichte4, double* normale3,
+                        double* abstand, double* ruhe) {
+  if (!s->hat_selbst || s->netz.r_punkt.size() != s->s.x.size()) s->selbst_bauen();
+  std::vector<Naehe> l = finde(s->netz, s->s.x, s->s.w, true, true, brute != 0);
+  naehe_aus(l, max, art, punkte4, gewichte4, normale3, abstand, ruhe);
+  return (int32_t)l.size();
+}
+int32_t or_sys_federn(const or_sim* s, int32_t max, uint32_t* punkte5, double* gewichte5, double* kraft3, double* block9) {
+  const auto& l = s->sys.federn;
+  for (size_t q = 0; q < l.size() && (int32_t)q < max; ++q) {
+    for (int k = 0; k < 5; ++k) {
+      if (punkte5) punkte5[5 * q + k] = l[q].punkte[k];
+      if (gewichte5) gewichte5[5 * q + k] = l[q].gewichte[k];
+    }
+    if (kraft3) put(kraft3 + 3 * q, l[q].kraft);
+    if (block9)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) block9[9 * q + 3 * a + b] = l[q].block[a][b];
+  }
+  return (int32_t)l.size();
+}
+void or_sim_v(const or_sim* s, double* v) {
+  for (size_t i = 0; i < s->s.v.size(); ++i) put(v + 3 * i, s->s.v[i]);
+}
+int32_t or_selbst_schnitte(or_sim* s, int32_t max, uint32_t* kante_dreieck, double* t, double* w3, double* richtung3,
+                           double* gewicht, uint32_t* wurzel_out) {
+  if (!s->hat_selbst || s->netz.r_punkt.size() != s->s.x.size()) s->selbst_bauen();
+  std::vector<Schnitt> l = finde_schnitte(s->netz, s->s.x, s->param.selbst_schnitte_an_naehten != 0);
+  std::vector<size_t> wz = gebiete(s->netz, l);
+  for (size_t q = 0; q < l.size() && (int32_t)q < max; ++q) {
+    if (kante_dreieck) { kante_dreieck[2 * q] = l[q].kante; kante_dreieck[2 * q + 1] = l[q].dreieck; }
+    if (t) t[q] = l[q].t;
+    if (w3) for (int k = 0; k < 3; ++k) w3[3 * q + k] = l[q].w[k];
+    if (richtung3) put(richtung3 + 3 * q, l[q].richtung);
+    if (gewicht) gewicht[q] = l[q].gewicht;
+    if (wurzel_out) wurzel_out[q] = (uint32_t)wz[q];
+  }
+  return (int32_t)l.size();
+}
+int32_t or_selbst_stoesse(or_sim* s, const double* x0, const double* x1, int32_t max, int32_t* art, uint32_t* punkte4,
+                          double* lage2, double* zeit, double* seite, double* normale3) {
+  if (!s->hat_selbst || s->netz.r_punkt.size() != s->s.x.size()) s->selbst_bauen();
+  size_t n = s->s.x.size();
+  std::vector<V3> a(n), b(n);
+  for (size_t i = 0; i < n; ++i) { a[i] = v3(x0 + 3 * i); b[i] = v3(x1 + 3 * i); }
+  std::vector<Stoss> l = finde_stoesse(s->netz, a, b, s->s.w, true, true);
+  stoesse_aus(l, max, art, punkte4, lage2, zeit, seite, normale3);
+  return (int32_t)l.size();
+}
+static Stoffnetz bares_netz(int32_t n, int32_t nt, const uint32_t* dreiecke, const double* r_punkt, int32_t nk, const uint32_t* kanten) {
+  std::vector<std::array<uint32_t, 3>> dr;
+  for (int t = 0; t < nt; ++t) dr.push_back({dreiecke[3 * t], dreiecke[3 * t + 1], dreiecke[3 * t + 2]});
+  std::vector<double> r(r_punkt ? r_punkt : nullptr, r_punkt ? r_punkt + n : nullptr);
+  r.resize(n, 0.0);
+  if (nt > 0 || nk == 0) return Stoffnetz::neu(dr, r);
+  Stoffnetz s;
+  for (int e = 0; e < nk; ++e) s.kanten.push_back({kanten[2 * e], kanten[2 * e + 1]});
+  s.r_punkt = r;
+  return s;
+}
+int32_t or_netz_naehe(int32_t n, const double* x, const double* w, int32_t nt, const uint32_t* dreiecke, const double* r_punkt,
+                      int32_t punkt_dreieck, int32_t kante_kante, int32_t max, int32_t* art, uint32_t* punkte4,
+                      double* gewichte4, double* normale3, double* abstand, double* ruhe) {
+  Stoffnetz netz = bares_netz(n, nt, dreiecke, r_punkt, 0, nullptr);
+  std::vector<V3> xv(n);
+  for (int i = 0; i < n; ++i) xv[i] = v3(x + 3 * i);
+  std::vector<double> wv(w, w + n);
+  std::vector<Naehe> l = finde(netz, xv, wv, punkt_dreieck != 0, kante_kante != 0);
+  naehe_aus(l, max, art, punkte4, gewichte4, normale3, abstand, ruhe);
+  return (int32_t)l.size();
+}
+int32_t or_netz_stoesse(int32_t n, const double* x0, double* x1, const double* w, const double* masse, int32_t nt,
+                        const uint32_t* dreiecke, int32_t nk, const uint32_t* kanten, int32_t punkt_dreieck,
+                        int32_t kante_kante, int32_t iterationen, int32_t max, int32_t* art, uint32_t* punkte4,
+                        double* lage2, double* zeit, double* seite, int32_t* korrekturen) {
+  Stoffnetz netz = bares_netz(n, nt, dreiecke, nullptr, nk, kanten);
+  std::vector<V3> a(n), b(n);
+  for (int i = 0; i < n; ++i) { a[i] = v3(x0 + 3 * i); b[i] = v3(x1 + 3 * i); }
+  std::vector<double> wv(w, w + n), mv(masse, masse + n);
+  std::vector<Stoss> l = finde_stoesse(netz, a, b, wv, punkt_dreieck != 0, kante_kante != 0);
+  stoesse_aus(l, max, art, punkte4, lage2, zeit, seite, nullptr);
+  size_t z = stossantwort(l, b, mv, wv, (size_t)iterationen);
+  if (korrekturen) *korrekturen = (int32_t)z;
+  for (int i = 0; i < n; ++i) put(x1 + 3 * i, b[i]);
+  return (int32_t)l.size();
 }
 
 }  // extern "C"

@@ -79,6 +79,9 @@ typedef struct or_szene {
   int32_t n_dreiecke;              /* cloth triangles (self-collision); may be 0 */
   const uint32_t* dreiecke;
   const uint32_t* teil_von_punkt;  /* piece index per point (self-collision); may be NULL */
+  const double* selbst_e_punkt;    /* n: stretch stiffness (N/m) of the point's piece; may be NULL */
+  const double* selbst_e_dreieck;  /* n_dreiecke: stretch stiffness of the triangle's piece; may be NULL */
+  const double* selbst_a_dreieck;  /* n_dreiecke: rest area (m^2); may be NULL */
   double bild_dt;
   double schwerkraft[3];
   double daempfung;
@@ -102,6 +105,7 @@ typedef struct or_statistik {
   int64_t cg_iterationen, loesungen, nicht_konvergiert, kontakte, haftend, freigegeben;
   double rest_max;
   int64_t naehe_punkt_dreieck, naehe_kante_kante, selbst_stoesse;
+  int64_t durchdringungen, durchdringungsgebiete;
 } or_statistik;
 
 typedef struct or_sim or_sim;
@@ -141,6 +145,37 @@ void    or_sys_mal_a(const or_sim* s, const double* p, double* out);
 /* CpuLoeser::loese on the built system and contact set; dv in = warm start. */
 void    or_sys_loese(const or_sim* s, double* dv, int32_t* iterationen, int32_t* nicht_konvergiert, double* rest);
 void    or_sys_rest(const or_sim* s, const double* dv, double* out);
+
+/* ---- self-collision (selbstkollision.rs) ---- */
+/* Proximity pairs of the current state (finde, :387-398), in the reference's
+ * order; no side effects. Writes up to `max` entries, returns the count.
+ * art: 0 point-triangle, 1 edge-edge. brute != 0: every entry instead of the
+ * hash's buckets (the check of the hash). */
+int32_t or_selbst_naehe(or_sim* s, int32_t brute, int32_t max, int32_t* art, uint32_t* punkte4, double* gewichte4,
+                        double* normale3, double* abstand, double* ruhe);
+/* Springs of the last built system (proximity springs, then push springs). */
+int32_t or_sys_federn(const or_sim* s, int32_t max, uint32_t* punkte5, double* gewichte5, double* kraft3, double* block9);
+/* Velocities after the last build (the proximity damping changes them). */
+void    or_sim_v(const or_sim* s, double* v);
+/* Cuts of the current state with their region roots (finde_schnitte, gebiete). */
+int32_t or_selbst_schnitte(or_sim* s, int32_t max, uint32_t* kante_dreieck, double* t, double* w3, double* richtung3,
+                           double* gewicht, uint32_t* wurzel);
+/* Impacts on the path x0 -> x1 of this scene's mesh (finde_stoesse, :1196-1282). */
+int32_t or_selbst_stoesse(or_sim* s, const double* x0, const double* x1, int32_t max, int32_t* art, uint32_t* punkte4,
+                          double* lage2, double* zeit, double* seite, double* normale3);
+
+/* The same searches on a bare mesh (Stoffnetz::neu from triangles and radii,
+ * no seam partners); for the reference's own unit tests. With nt = 0 the
+ * mesh has only the nk given edges. */
+int32_t or_netz_naehe(int32_t n, const double* x, const double* w, int32_t nt, const uint32_t* dreiecke, const double* r_punkt,
+                      int32_t punkt_dreieck, int32_t kante_kante, int32_t max, int32_t* art, uint32_t* punkte4,
+                      double* gewichte4, double* normale3, double* abstand, double* ruhe);
+/* finde_stoesse on x0 -> x1, then stossantwort on x1 (in place) with masse.
+ * Returns the number of impacts found; *korrekturen = the response's count. */
+int32_t or_netz_stoesse(int32_t n, const double* x0, double* x1, const double* w, const double* masse, int32_t nt,
+                        const uint32_t* dreiecke, int32_t nk, const uint32_t* kanten, int32_t punkt_dreieck,
+                        int32_t kante_kante, int32_t iterationen, int32_t max, int32_t* art, uint32_t* punkte4,
+                        double* lage2, double* zeit, double* seite, int32_t* korrekturen);
 
 #ifdef __cplusplus
 }
