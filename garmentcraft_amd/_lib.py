@@ -35,6 +35,8 @@ class Topologie(C.Structure):
         ("n_naehte", C.c_int32), ("naht_punkte", U32), ("naht_ruhe", D),
         ("n_biegungen", C.c_int32), ("biegung_punkte", U32), ("biegung_k", D), ("biegung_ruhe", D),
         ("n_membran", C.c_int32), ("membran_punkte", U32), ("membran_daten", D),
+        ("n_dreiecke", C.c_int32), ("dreiecke", U32), ("selbst_e_punkt", D), ("selbst_e_dreieck", D),
+        ("selbst_a_dreieck", D),
     ]
 
 
@@ -43,6 +45,8 @@ class Param(C.Structure):
         ("schwerkraft", C.c_double * 3), ("daempfung", C.c_double), ("cg_max", C.c_int32),
         ("cg_toleranz", C.c_double), ("naht_k", C.c_double), ("kontakt_rand", C.c_double),
         ("normalkraft_aus_last", C.c_int32),
+        ("selbstkollision", C.c_int32), ("selbst_zusatzdicke", C.c_double), ("selbst_steifigkeit", C.c_double),
+        ("selbst_stoss_iterationen", C.c_int32), ("selbst_schnitte_an_naehten", C.c_int32),
     ]
 
 
@@ -55,6 +59,8 @@ class Statistik_c(C.Structure):
         ("cg_iterationen", C.c_int64), ("loesungen", C.c_int64), ("nicht_konvergiert", C.c_int64),
         ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
         ("barriere_spins_max", C.c_int64),
+        ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
+        ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64),
     ]
 
 
@@ -89,6 +95,11 @@ SIGNATUREN = [
     ("gc_step_graph", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
     ("gc_read_statistik", C.c_int, [C.c_void_p, C.POINTER(Statistik_c), C.c_int32]),
     ("gc_zeiten", C.c_int, [C.c_void_p, D, D]),
+    ("gc_read_naehe", C.c_int, [C.c_void_p, C.c_int32, I32, U32, D, D, D, D]),
+    ("gc_read_federn", C.c_int, [C.c_void_p, C.c_int32, U32, D, D, D]),
+    ("gc_read_schnitte", C.c_int, [C.c_void_p, C.c_int32, U32, D, D, D, D, U32]),
+    ("gc_read_stoesse", C.c_int, [C.c_void_p, C.c_int32, I32, U32, D, D, D, D, I32]),
+    ("gc_get_v", C.c_int, [C.c_void_p, D]),
 ]
 
 _lib = None

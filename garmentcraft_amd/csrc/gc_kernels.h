@@ -86,6 +86,20 @@ struct SysDev {
   const uint32_t* kachel_v;       // points touched by the tile's constraints, global index
   uint32_t kachel_slots_max;      // LDS slot area (entries) of the largest tile
 
+  // springs of cloth self-collision, rebuilt every step (counts on the
+  // device; empty lists when it is off): per point its spring pairs
+  // (spring << 5 | q * 5 + r, in spring then pair order) and its force slots
+  // (spring << 3 | q)
+  int32_t selbst;
+  const uint32_t* fe_off;
+  const uint32_t* fe_ent;
+  const uint32_t* fk_off;
+  const uint32_t* fk_ent;
+  const uint32_t* fe_punkte;    // 5 per spring
+  const double* fe_gew;         // 5 per spring
+  const V3<double>* fe_kraft;
+  const T* fe_block;            // 9 per spring, bulk type
+
   // bodies
   int32_t n_koerper;
   Koerper koerper[GC_MAX_KOERPER];
@@ -102,7 +116,8 @@ struct GridState {
   long long cg_iterationen, loesungen, nicht_konvergiert, kontakte, haftend, freigegeben;
   double rest_max;
   unsigned int spins_max;    // longest barrier wait (polls) since the last read
-  unsigned int pad1;
+  unsigned int selbst_ueberlauf;  // a self-collision list exceeded its capacity
+  long long naehe_pd, naehe_kk, selbst_stoesse, durchdringungen, gebiete;
   // result of the last solve
   int iterationen;
   int nicht_konv;
@@ -218,7 +233,26 @@ __device__ __forceinline__ V3<T> mal_h_zeile(const SysDev<T>& s, uint32_t i, PF 
     V3<T> d3[3] = {p(mp[0]), p(mp[1]), p(mp[2])};
     out = out + mal_h_aus_k(s.m_el[c], s.m_h[c], d3, (int)k);
   }
-  return out;
+  return feder_zeile(s, i, out, p);
+}
+
+// Springs of row i, added to acc in the order of System::mal_h
+// (implizit.rs:214-220): per spring its pairs (first side q, second side r)
+// with the block K times (p_q - p_r) times -g_q g_r.
+template <typename T, typename PF>
+__device__ __forceinline__ V3<T> feder_zeile(const SysDev<T>& s, uint32_t i, V3<T> acc, PF p) {
+  if (!s.selbst) return acc;
+  for (uint32_t e = s.fe_off[i]; e < s.fe_off[i + 1]; ++e) {
+    const uint32_t code = s.fe_ent[e], f = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
+    const uint32_t pq = s.fe_punkte[5 * f + q], prr = s.fe_punkte[5 * f + r];
+    const T om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
+    const T* B = s.fe_block + 9 * f;
+    const V3<T> d = p(pq) - p(prr);
+    const V3<T> t = V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
+                          B[6] * d.x + B[7] * d.y + B[8] * d.z} * om;
+    acc = pq == i ? acc + t : acc - t;
+  }
+  return acc;
 }
 
 // A*p at row i (implizit.rs:224-229).
@@ -387,6 +421,11 @@ __global__ void k_sammeln(const SysDev<T>* __restrict__ sp) {
       const uint32_t ent = s.csr[e], c = ent >> 2, mm = ent & 3u;
       f = f - cast3<double>(s.b_grad[4 * c + mm]) * s.b_kf[c];
     }
+    if (s.selbst)
+      for (uint32_t e = s.fk_off[i]; e < s.fk_off[i + 1]; ++e) {
+        const uint32_t code = s.fk_ent[e], fe = code >> 3, q = code & 7u;
+        f = f + s.fe_kraft[fe] * s.fe_gew[5 * fe + q];
+      }
     for (uint32_t e = o[2]; e < o[3]; ++e) {
       const uint32_t ent = s.csr[e], c = ent >> 2, k = ent & 3u;
       f = f + s.m_f[3 * c + k];
@@ -427,6 +466,12 @@ __global__ void k_sammeln(const SysDev<T>* __restrict__ sp) {
       diag_block_k(s.m_el[c], s.m_h[c], (int)k, bl);
       for (int q = 0; q < 9; ++q) d[q] += h2 * bl[q];
     }
+    if (s.selbst)
+      for (uint32_t e = s.fe_off[i]; e < s.fe_off[i + 1]; ++e) {  // implizit.rs:267-277
+        const uint32_t code = s.fe_ent[e], fe = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
+        const T hom = T(s.h2 * (-s.fe_gew[5 * fe + q] * s.fe_gew[5 * fe + r]));
+        for (int a = 0; a < 9; ++a) d[a] += hom * s.fe_block[9 * fe + a];
+      }
     T inv[9];
     m3_inv(d, inv);
     for (int q = 0; q < 9; ++q) s.pinv[9 * i + q] = inv[q];
@@ -589,7 +634,8 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
     kachel_beitraege(s, t, slot, ul);
     __syncthreads();
     for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const V3<T> ap = kachel_zeile(s, i, basis, slot) * th2 + s.dv[i] * s.mdiag[i];
+      const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return s.dv[j]; });
+      const V3<T> ap = hp * th2 + s.dv[i] * s.mdiag[i];
       const uint8_t fl = s.fflag[i];
       const V3<T> n = s.fnorm[i];
       const V3<T> bi = s.b[i];
@@ -632,7 +678,9 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const uint8_t fl = s.fflag[i];
         const V3<T> pi = richtung(s, (uint32_t)i, po, tb, erste);
-        const V3<T> ap = filter_fl(fl, s.fnorm[i], kachel_zeile(s, i, basis, slot) * th2 + pi * s.mdiag[i]);
+        const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot),
+                                     [&](uint32_t j) { return richtung(s, j, po, tb, erste); });
+        const V3<T> ap = filter_fl(fl, s.fnorm[i], hp * th2 + pi * s.mdiag[i]);
         pn[i] = pi;
         s.ap[i] = ap;
         ca[0] += (double)dot(pi, ap);

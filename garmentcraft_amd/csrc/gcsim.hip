@@ -6,11 +6,12 @@
 //   reset, build edges / bending / membranes, contacts, per-point gather,
 //   warm start, PCG, release, PCG (only after a release), release, update.
 #include "../../include/gcsim.h"
-#include "gc_kernels.h"
+#include "gc_selbst.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -69,6 +70,25 @@ struct gc_ctx {
   // timing
   std::vector<hipEvent_t> ev;
   double ms_schritt = 0.0, ms_pcg = 0.0;
+  // cloth self-collision: inputs, the mesh tables built from them on first
+  // use (for the extra thickness they were built with), device buffers
+  std::vector<uint32_t> sk_dreiecke_in;
+  std::vector<double> sk_e_punkt, sk_e_dreieck, sk_a_dreieck;
+  std::vector<uint32_t> sk_abstand_in;        // all distance constraints (weld partners)
+  std::vector<double> sk_abstand_ruhe_in;
+  std::vector<uint32_t> sk_naht_in;
+  std::vector<double> sk_naht_ruhe;
+  bool sk_bereit = false;
+  double sk_dicke = -1.0;
+  SelbstDev sk{};
+  SelbstDev* sk_dev = nullptr;
+  uint32_t sk_eimer_p = 0, sk_eimer_k = 0, sk_eimer_d = 0, sk_eimer_max = 0;
+  double* sk_zellen = nullptr;               // device: proximity, cut, impact cell
+  uint32_t* sk_fe_fuell = nullptr;
+  uint32_t* sk_fk_fuell = nullptr;
+  uint32_t* sk_st_fuell = nullptr;
+  double* sk_naht_ruhe_dev = nullptr;
+  void* sk_block_t = nullptr;                // spring blocks in the bulk type
   // graph of one substep
   hipGraphExec_t graph = nullptr;
 };
@@ -87,12 +107,16 @@ static int fehler(gc_ctx* c, const std::string& m) {
   return 1;
 }
 
+// Device buffer, zeroed in order with the uploads that follow on the
+// context's stream (a plain hipMemset runs on the null stream, which a
+// non-blocking stream does not wait for, and could clear a buffer after its
+// upload).
 template <typename X>
 static X* dalloc(gc_ctx* c, size_t count) {
   void* p = nullptr;
   size_t bytes = std::max<size_t>(1, count) * sizeof(X);
   if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-  if (hipMemset(p, 0, bytes) != hipSuccess) return nullptr;
+  if (hipMemsetAsync(p, 0, bytes, c->stream) != hipSuccess) return nullptr;
   c->allocs.push_back(p);
   return (X*)p;
 }
@@ -361,6 +385,336 @@ static void alle_param(gc_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
+// cloth self-collision: mesh tables and buffers (selbstkollision.rs:157-190,
+// 623-693), built on the host once per garment and extra thickness
+// ---------------------------------------------------------------------------
+static bool selbst_an(const gc_ctx* c) { return c->param.selbstkollision != 0 && !c->sk_dreiecke_in.empty(); }
+
+template <typename X>
+static X* hoch_neu(gc_ctx* c, const std::vector<X>& v) {
+  X* d = dalloc<X>(c, v.size());
+  if (!d) return nullptr;
+  if (!v.empty() && hipMemcpyAsync(d, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, c->stream) != hipSuccess) return nullptr;
+  return d;
+}
+
+template <typename T>
+static int selbst_vorbereiten(gc_ctx* c) {
+  SysDev<T>& sy = sys<T>(c);
+  if (!selbst_an(c)) {
+    sy.selbst = 0;
+    return 0;
+  }
+  const double dicke = c->param.selbst_zusatzdicke;
+  if (c->sk_bereit && c->sk_dicke == dicke) {
+    sy.selbst = 1;
+    return 0;
+  }
+  if (c->sk_bereit) return fehler(c, "selbst_zusatzdicke changed after the first step (create a new context)");
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  const int n = c->n;
+  const int nt = (int)(c->sk_dreiecke_in.size() / 3);
+  // Stoffnetz::neu (:157-190): edges in first-encounter order
+  std::map<std::pair<uint32_t, uint32_t>, uint32_t> index;
+  std::vector<uint32_t> kanten, kd, dk(3 * (size_t)nt);
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t a = c->sk_dreiecke_in[3 * t + k], b = c->sk_dreiecke_in[3 * t + (k + 1) % 3];
+      const std::pair<uint32_t, uint32_t> key{std::min(a, b), std::max(a, b)};
+      auto it = index.find(key);
+      uint32_t e;
+      if (it == index.end()) {
+        e = (uint32_t)(kanten.size() / 2);
+        kanten.push_back(key.first);
+        kanten.push_back(key.second);
+        kd.push_back(SK_KEIN);
+        kd.push_back(SK_KEIN);
+        index[key] = e;
+      } else {
+        e = it->second;
+      }
+      if (kd[2 * e] == SK_KEIN) kd[2 * e] = (uint32_t)t;
+      else if (kd[2 * e + 1] == SK_KEIN) kd[2 * e + 1] = (uint32_t)t;
+      dk[3 * (size_t)t + k] = e;
+    }
+  const int ne = (int)(kanten.size() / 2);
+  std::vector<double> rp(n), rk(ne), rd(nt);
+  double r_max = 0.0;
+  for (int i = 0; i < n; ++i) {
+    rp[i] = c->koll[i] + dicke / 2.0;  // aus_szene (:625)
+    r_max = std::fmax(r_max, rp[i]);
+  }
+  for (int e = 0; e < ne; ++e) rk[e] = std::fmax(rp[kanten[2 * e]], rp[kanten[2 * e + 1]]);
+  for (int t = 0; t < nt; ++t) {
+    double m = 0.0;
+    for (int k = 0; k < 3; ++k) m = std::fmax(m, rp[c->sk_dreiecke_in[3 * t + k]]);
+    rd[t] = m;
+  }
+  // seam and weld partners (:629-650)
+  std::vector<std::vector<uint32_t>> partner(n);
+  const size_t ns = c->sk_naht_in.size() / 2;
+  for (size_t q = 0; q < ns; ++q) {
+    const uint32_t i = c->sk_naht_in[2 * q], j = c->sk_naht_in[2 * q + 1];
+    partner[i].push_back(j);
+    partner[j].push_back(i);
+  }
+  for (size_t e = 0; e < c->sk_abstand_ruhe_in.size(); ++e) {
+    const uint32_t i = c->sk_abstand_in[2 * e], j = c->sk_abstand_in[2 * e + 1];
+    if (c->sk_abstand_ruhe_in[e] != 0.0 || i == j) continue;
+    if (std::find(partner[i].begin(), partner[i].end(), j) == partner[i].end()) partner[i].push_back(j);
+    if (std::find(partner[j].begin(), partner[j].end(), i) == partner[j].end()) partner[j].push_back(i);
+  }
+  std::vector<uint32_t> p_off(n + 1, 0), p_list;
+  for (int i = 0; i < n; ++i) {
+    p_off[i] = (uint32_t)p_list.size();
+    p_list.insert(p_list.end(), partner[i].begin(), partner[i].end());
+  }
+  p_off[n] = (uint32_t)p_list.size();
+  // Stoffwerte (:655-692)
+  std::vector<double> e_p = c->sk_e_punkt, e_d = c->sk_e_dreieck, a_d = c->sk_a_dreieck;
+  e_p.resize(n, 0.0);
+  e_d.resize(nt, 0.0);
+  a_d.resize(nt, 0.0);
+  std::vector<double> a_p(n, 0.0), e_k(ne), a_k(ne);
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < 3; ++k) a_p[c->sk_dreiecke_in[3 * t + k]] += a_d[t] / 3.0;
+  for (int e = 0; e < ne; ++e) {
+    double sa = 0.0, se = 0.0;
+    int cnt = 0;
+    for (int q = 0; q < 2; ++q)
+      if (kd[2 * e + q] != SK_KEIN) {
+        sa += a_d[kd[2 * e + q]];
+        se += e_d[kd[2 * e + q]];
+        ++cnt;
+      }
+    const double k = (double)std::max(cnt, 1);
+    a_k[e] = sa / k;
+    e_k[e] = se / k;
+  }
+  std::vector<std::vector<uint32_t>> nw(n);
+  for (size_t q = 0; q < ns; ++q) {
+    const uint32_t i = c->sk_naht_in[2 * q], j = c->sk_naht_in[2 * q + 1];
+    nw[i].push_back(j);
+    nw[i].push_back((uint32_t)q);
+    nw[j].push_back(i);
+    nw[j].push_back((uint32_t)q);
+  }
+  std::vector<uint32_t> nw_off(n + 1, 0), nw_list;
+  for (int i = 0; i < n; ++i) {
+    nw_off[i] = (uint32_t)(nw_list.size() / 2);
+    nw_list.insert(nw_list.end(), nw[i].begin(), nw[i].end());
+  }
+  nw_off[n] = (uint32_t)(nw_list.size() / 2);
+
+  SelbstDev& k = c->sk;
+  k = SelbstDev{};
+  k.n = n;
+  k.nt = nt;
+  k.ne = ne;
+  k.r_max = r_max;
+  k.steifigkeit = c->param.selbst_steifigkeit;
+  k.an_naehten = c->param.selbst_schnitte_an_naehten;
+  k.stoss_iterationen = c->param.selbst_stoss_iterationen;
+  const size_t mx = (size_t)std::max(n, std::max(ne, nt));
+  c->sk_eimer_p = (uint32_t)std::max<size_t>(2 * (size_t)n, 64);
+  c->sk_eimer_k = (uint32_t)std::max<size_t>(2 * (size_t)ne, 64);
+  c->sk_eimer_d = (uint32_t)std::max<size_t>(2 * (size_t)nt, 64);
+  c->sk_eimer_max = std::max(c->sk_eimer_p, std::max(c->sk_eimer_k, c->sk_eimer_d));
+  k.kapazitaet_eintraege = (uint32_t)(32 * mx);
+  k.kapazitaet_naehe = (uint32_t)(4 * ((size_t)n + ne));
+  k.kapazitaet_schnitte = (uint32_t)(2 * (size_t)ne + 1024);
+  k.kapazitaet_tausch = 6 * k.kapazitaet_schnitte;
+  k.kapazitaet_federn = k.kapazitaet_naehe + k.kapazitaet_schnitte;
+  k.kapazitaet_fe_ent = 12 * k.kapazitaet_federn;
+  k.kapazitaet_fk_ent = 5 * k.kapazitaet_federn;
+  k.kapazitaet_st_ent = 4 * (uint32_t)(nt + ne);
+  bool ok = true;
+  auto chk = [&](const void* q) { ok = ok && q != nullptr; };
+  k.dreiecke = hoch_neu(c, c->sk_dreiecke_in); chk(k.dreiecke);
+  k.kanten = hoch_neu(c, kanten); chk(k.kanten);
+  k.kanten_dreiecke = hoch_neu(c, kd); chk(k.kanten_dreiecke);
+  k.dreieck_kanten = hoch_neu(c, dk); chk(k.dreieck_kanten);
+  k.r_punkt = hoch_neu(c, rp); chk(k.r_punkt);
+  k.r_kante = hoch_neu(c, rk); chk(k.r_kante);
+  k.r_dreieck = hoch_neu(c, rd); chk(k.r_dreieck);
+  k.partner_off = hoch_neu(c, p_off); chk(k.partner_off);
+  k.partner = hoch_neu(c, p_list); chk(k.partner);
+  k.nahtw_off = hoch_neu(c, nw_off); chk(k.nahtw_off);
+  k.nahtw = hoch_neu(c, nw_list); chk(k.nahtw);
+  k.e_punkt = hoch_neu(c, e_p); chk(k.e_punkt);
+  k.a_punkt = hoch_neu(c, a_p); chk(k.a_punkt);
+  k.e_kante = hoch_neu(c, e_k); chk(k.e_kante);
+  k.a_kante = hoch_neu(c, a_k); chk(k.a_kante);
+  k.e_dreieck = hoch_neu(c, e_d); chk(k.e_dreieck);
+  k.a_dreieck = hoch_neu(c, a_d); chk(k.a_dreieck);
+  k.mu = sy.reib;
+  c->sk_naht_ruhe_dev = hoch_neu(c, c->sk_naht_ruhe); chk(c->sk_naht_ruhe_dev);
+  k.naht_ruhe = c->sk_naht_ruhe_dev;
+  k.x = sy.x;
+  k.v = sy.v;
+  k.masse = sy.masse;
+  k.w = sy.w;
+  k.fflag = sy.fflag;
+  k.x_anfang = dalloc<V3<double>>(c, n); chk(k.x_anfang);
+  const std::vector<uint32_t> eins(nt, 1u);  // folge starts at 1 (:1024-1026)
+  k.folge = hoch_neu(c, eins); chk(k.folge);
+  k.n_dreieck = dalloc<V3<double>>(c, nt); chk(k.n_dreieck);
+  k.kantenebene = dalloc<V3<double>>(c, ne); chk(k.kantenebene);
+  k.quader = dalloc<Quader>(c, mx); chk(k.quader);
+  k.eimer_anfang = dalloc<uint32_t>(c, c->sk_eimer_max + 1); chk(k.eimer_anfang);
+  k.eimer_fuell = dalloc<uint32_t>(c, c->sk_eimer_max + 1); chk(k.eimer_fuell);
+  k.eintraege = dalloc<uint32_t>(c, k.kapazitaet_eintraege); chk(k.eintraege);
+  k.zahl = dalloc<uint32_t>(c, (size_t)n + ne + nt + 1); chk(k.zahl);
+  k.naehe = dalloc<Naehe>(c, k.kapazitaet_naehe); chk(k.naehe);
+  k.schnitte = dalloc<Schnitt>(c, k.kapazitaet_schnitte); chk(k.schnitte);
+  k.schnitt_wurzel = dalloc<uint32_t>(c, k.kapazitaet_schnitte); chk(k.schnitt_wurzel);
+  k.uf = dalloc<uint32_t>(c, 2 * (size_t)k.kapazitaet_schnitte); chk(k.uf);
+  k.summe = dalloc<V3<double>>(c, k.kapazitaet_schnitte); chk(k.summe);
+  k.tausch = dalloc<uint32_t>(c, 2 * (size_t)k.kapazitaet_tausch); chk(k.tausch);
+  k.stoesse = dalloc<Stoss>(c, (size_t)nt + ne); chk(k.stoesse);
+  k.fe_punkte = dalloc<uint32_t>(c, 5 * (size_t)k.kapazitaet_federn); chk(k.fe_punkte);
+  k.fe_gew = dalloc<double>(c, 5 * (size_t)k.kapazitaet_federn); chk(k.fe_gew);
+  k.fe_kraft = dalloc<V3<double>>(c, k.kapazitaet_federn); chk(k.fe_kraft);
+  k.fe_block = dalloc<double>(c, 9 * (size_t)k.kapazitaet_federn); chk(k.fe_block);
+  k.fe_off = dalloc<uint32_t>(c, (size_t)n + 1); chk(k.fe_off);
+  k.fe_ent = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(k.fe_ent);
+  k.fk_off = dalloc<uint32_t>(c, (size_t)n + 1); chk(k.fk_off);
+  k.fk_ent = dalloc<uint32_t>(c, k.kapazitaet_fk_ent); chk(k.fk_ent);
+  k.st_off = dalloc<uint32_t>(c, (size_t)n + 1); chk(k.st_off);
+  k.st_ent = dalloc<uint32_t>(c, k.kapazitaet_st_ent); chk(k.st_ent);
+  k.st_dx = dalloc<V3<double>>(c, 5 * ((size_t)nt + ne)); chk(k.st_dx);
+  k.zaehler = dalloc<uint32_t>(c, SK_Z_ANZAHL); chk(k.zaehler);
+  c->sk_zellen = dalloc<double>(c, 3); chk(c->sk_zellen);
+  c->sk_fe_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_fe_fuell);
+  c->sk_fk_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_fk_fuell);
+  c->sk_st_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_st_fuell);
+  c->sk_block_t = dalloc<T>(c, 9 * (size_t)k.kapazitaet_federn); chk(c->sk_block_t);
+  c->sk_dev = dalloc<SelbstDev>(c, 1); chk(c->sk_dev);
+  if (!ok) return fehler(c, "hipMalloc/hipMemcpy failed (self-collision buffers)");
+  HIPCK(c, hipMemcpyAsync(c->sk_dev, &k, sizeof(SelbstDev), hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  sy.selbst = 1;
+  sy.fe_off = k.fe_off;
+  sy.fe_ent = k.fe_ent;
+  sy.fk_off = k.fk_off;
+  sy.fk_ent = k.fk_ent;
+  sy.fe_punkte = k.fe_punkte;
+  sy.fe_gew = k.fe_gew;
+  sy.fe_kraft = k.fe_kraft;
+  sy.fe_block = (const T*)c->sk_block_t;
+  c->sk_bereit = true;
+  c->sk_dicke = dicke;
+  return 0;
+}
+
+static unsigned sk_gitter(long count) {
+  long b = (count + BUILD_THREADS - 1) / BUILD_THREADS;
+  return (unsigned)std::max(1L, std::min(b, 4096L));
+}
+
+// One spatial hash over c->sk.quader[0..anzahl) with the cell at *zelle_p.
+static void sk_hash(gc_ctx* c, int anzahl, const double* zelle_p, uint32_t eimer) {
+  hipStream_t st = c->stream;
+  SelbstDev* sp = c->sk_dev;
+  hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(eimer + 1)), dim3(BUILD_THREADS), 0, st, c->sk.eimer_fuell, (int)eimer + 1);
+  hipLaunchKernelGGL(k_sk_eimer_zaehlen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, c->sk.eimer_fuell, (int)eimer, c->sk.eimer_anfang,
+                     c->sk.zaehler + SK_Z_EINTRAEGE);
+  hipLaunchKernelGGL(k_sk_eimer_fuellen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
+  hipLaunchKernelGGL(k_sk_eimer_sortieren, dim3(sk_gitter(eimer)), dim3(BUILD_THREADS), 0, st, sp, eimer);
+}
+
+// Before the build (implizit.rs:668-707): pairs, springs and damping, cuts,
+// regions, push springs, the per-point spring lists.
+template <typename T>
+static int launch_selbst_bau(gc_ctx* c) {
+  hipStream_t st = c->stream;
+  SelbstDev* sp = c->sk_dev;
+  const SelbstDev& k = c->sk;
+  const SysDev<T>* syp = dsys<T>(c);
+  hipLaunchKernelGGL(k_sk_null, dim3(1), dim3(64), 0, st, k.zaehler, (int)SK_Z_ANZAHL);
+  hipLaunchKernelGGL(k_sk_zelle_naehe, dim3(1), dim3(64), 0, st, sp, c->sk_zellen);
+  hipLaunchKernelGGL(k_sk_normalen_dreieck, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_normalen_kante, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp);
+  // point-triangle (:400-467)
+  hipLaunchKernelGGL(k_sk_quader_punkte, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
+  sk_hash(c, k.n, c->sk_zellen, c->sk_eimer_p);
+  hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 0);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.zahl, k.nt, (uint32_t*)nullptr, k.zaehler + SK_Z_NAEHE_PD);
+  hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 1);
+  // edge-edge (:506-592)
+  hipLaunchKernelGGL(k_sk_quader_kanten, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp);
+  sk_hash(c, k.ne, c->sk_zellen, c->sk_eimer_k);
+  hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 0);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.zahl, k.ne, (uint32_t*)nullptr, k.zaehler + SK_Z_NAEHE_KK);
+  hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 1);
+  hipLaunchKernelGGL(k_sk_naehe_summe, dim3(1), dim3(64), 0, st, sp);
+  // springs and the damping of the approach (:713-787)
+  hipLaunchKernelGGL(k_sk_federn, dim3(sk_gitter(k.kapazitaet_naehe)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_daempfung, dim3(1), dim3(64), 0, st, sp);
+  // cuts, regions, push springs (:858-1059)
+  hipLaunchKernelGGL(k_sk_quader_dreiecke, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 1);
+  sk_hash(c, k.nt, c->sk_zellen + 1, c->sk_eimer_d);
+  hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
+                     c->sk_eimer_d, 0);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.zahl, k.ne, (uint32_t*)nullptr, k.zaehler + SK_Z_SCHNITTE);
+  hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
+                     c->sk_eimer_d, 1);
+  hipLaunchKernelGGL(k_sk_schnitte_summe, dim3(1), dim3(64), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_gebiete, dim3(1), dim3(64), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_schub<T>, dim3(1), dim3(64), 0, st, sp, syp);
+  hipLaunchKernelGGL(k_sk_folge, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp);
+  // per-point spring lists
+  hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.fe_off, k.n + 1);
+  hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.fk_off, k.n + 1);
+  hipLaunchKernelGGL(k_sk_feder_zaehlen, dim3(sk_gitter(k.kapazitaet_federn)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.fe_off, k.n, c->sk_fe_fuell, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.fk_off, k.n, c->sk_fk_fuell, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_sk_feder_pruefen, dim3(1), dim3(64), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_feder_fuellen, dim3(sk_gitter(k.kapazitaet_federn)), dim3(BUILD_THREADS), 0, st, sp, c->sk_fe_fuell,
+                     c->sk_fk_fuell);
+  hipLaunchKernelGGL(k_sk_feder_sortieren, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_feder_leeren, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_feder_bloecke<T>, dim3(sk_gitter(9L * k.kapazitaet_federn)), dim3(BUILD_THREADS), 0, st, sp,
+                     (T*)c->sk_block_t);
+  hipLaunchKernelGGL(k_sk_statistik, dim3(1), dim3(64), 0, st, sp, c->gs, 0);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+// After the update (implizit.rs:1030-1037): impacts on the path of the step.
+template <typename T>
+static int launch_selbst_stoss(gc_ctx* c) {
+  hipStream_t st = c->stream;
+  SelbstDev* sp = c->sk_dev;
+  const SelbstDev& k = c->sk;
+  const SysDev<T>* syp = dsys<T>(c);
+  hipLaunchKernelGGL(k_sk_null, dim3(1), dim3(64), 0, st, k.zaehler + SK_Z_STOSS_ZAHL, 1);
+  hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 2);
+  hipLaunchKernelGGL(k_sk_quader_weg_punkte, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
+  sk_hash(c, k.n, c->sk_zellen + 2, c->sk_eimer_p);
+  hipLaunchKernelGGL(k_sk_stoss_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
+                     c->sk_eimer_p);
+  hipLaunchKernelGGL(k_sk_quader_weg_kanten, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp);
+  sk_hash(c, k.ne, c->sk_zellen + 2, c->sk_eimer_k);
+  hipLaunchKernelGGL(k_sk_stoss_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
+                     c->sk_eimer_k);
+  hipLaunchKernelGGL(k_sk_stoss_filter, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.st_off, k.n + 1);
+  hipLaunchKernelGGL(k_sk_stoss_zaehlen, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.st_off, k.n, c->sk_st_fuell, (uint32_t*)nullptr);
+  hipLaunchKernelGGL(k_sk_stoss_fuellen, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_st_fuell);
+  for (int it = 0; it < k.stoss_iterationen; ++it) {
+    hipLaunchKernelGGL(k_sk_stoss_beitrag<T>, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp, syp);
+    hipLaunchKernelGGL(k_sk_stoss_anwenden<T>, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp, syp);
+  }
+  hipLaunchKernelGGL(k_sk_statistik, dim3(1), dim3(64), 0, st, sp, c->gs, 1);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // launch sequences
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -369,6 +723,7 @@ static int launch_bau(gc_ctx* c) {
   const SysDev<T>* sp = dsys<T>(c);
   hipStream_t st = c->stream;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->gs);
+  if (s.selbst && launch_selbst_bau<T>(c)) return 1;
   if (s.nk) hipLaunchKernelGGL(k_bau_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sp);
   if (s.nb) hipLaunchKernelGGL(k_bau_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, sp);
   if (s.nm) hipLaunchKernelGGL(k_bau_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, sp);
@@ -391,8 +746,9 @@ template <typename T>
 static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
   const SysDev<T>& s = sys<T>(c);
   const SysDev<T>* sp = dsys<T>(c);
-  if (launch_bau<T>(c)) return 1;
   hipStream_t st = c->stream;
+  if (s.selbst) hipLaunchKernelGGL(k_sk_kopie_x, dim3(sk_gitter(s.n)), dim3(BUILD_THREADS), 0, st, c->sk_dev);
+  if (launch_bau<T>(c)) return 1;
   hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, hat_vorher);
   if (ev) HIPCK(c, hipEventRecord(ev[0], st));
   if (launch_pcg<T>(c, 0)) return 1;
@@ -403,6 +759,7 @@ static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
   if (ev) HIPCK(c, hipEventRecord(ev[3], st));
   hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 1);
   hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs);
+  if (s.selbst && launch_selbst_stoss<T>(c)) return 1;
   HIPCK(c, hipGetLastError());
   return 0;
 }
@@ -415,6 +772,11 @@ static int pruefe_gitter(gc_ctx* c) {
     unsigned null = 0;
     HIPCK(c, hipMemcpy(&c->gs->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
     return fehler(c, "grid barrier wait exceeded its spin limit (blocks not co-resident?)");
+  }
+  if (g.selbst_ueberlauf) {
+    unsigned null = 0;
+    HIPCK(c, hipMemcpy(&c->gs->selbst_ueberlauf, &null, sizeof(unsigned), hipMemcpyHostToDevice));
+    return fehler(c, "a self-collision list exceeded its capacity (state of the step is not the reference's)");
   }
   return 0;
 }
@@ -431,6 +793,7 @@ static int schritte(gc_ctx* c, double h, int n) {
   }
   sys<T>(c).h = h;
   sys<T>(c).h2 = h * h;
+  if (selbst_vorbereiten<T>(c)) return 1;
   if (sys_hoch<T>(c)) return 1;
   HIPCK(c, hipEventRecord(c->ev[0], c->stream));
   for (int k = 0; k < n; ++k) {
@@ -466,6 +829,7 @@ static int schritte_graph(gc_ctx* c, double h, int n) {
   }
   sys<T>(c).h = h;
   sys<T>(c).h2 = h * h;
+  if (selbst_vorbereiten<T>(c)) return 1;
   if (sys_hoch<T>(c)) return 1;
   if (!c->graph) {
     hipGraph_t g;
@@ -763,6 +1127,17 @@ int gc_set_topology(gc_ctx* c, const gc_topologie* t) {
   c->b_ruhe.assign(t->biegung_ruhe, t->biegung_ruhe + t->n_biegungen);
   c->m_p.assign(t->membran_punkte, t->membran_punkte + 3 * (size_t)t->n_membran);
   c->m_d.assign(t->membran_daten, t->membran_daten + 12 * (size_t)t->n_membran);
+  if (t->n_dreiecke > 0 && t->dreiecke) {
+    if (!pruefe(t->dreiecke, 3 * (size_t)t->n_dreiecke)) return fehler(c, "point index out of range in triangles");
+    c->sk_dreiecke_in.assign(t->dreiecke, t->dreiecke + 3 * (size_t)t->n_dreiecke);
+    if (t->selbst_e_punkt) c->sk_e_punkt.assign(t->selbst_e_punkt, t->selbst_e_punkt + n);
+    if (t->selbst_e_dreieck) c->sk_e_dreieck.assign(t->selbst_e_dreieck, t->selbst_e_dreieck + t->n_dreiecke);
+    if (t->selbst_a_dreieck) c->sk_a_dreieck.assign(t->selbst_a_dreieck, t->selbst_a_dreieck + t->n_dreiecke);
+  }
+  c->sk_abstand_in.assign(t->kanten_punkte, t->kanten_punkte + 2 * (size_t)t->n_kanten);
+  c->sk_abstand_ruhe_in.assign(t->kanten_ruhe, t->kanten_ruhe + t->n_kanten);
+  c->sk_naht_in.assign(t->naht_punkte, t->naht_punkte + 2 * (size_t)t->n_naehte);
+  c->sk_naht_ruhe.assign(t->naht_ruhe, t->naht_ruhe + t->n_naehte);
   // CSR by point and category, each category in constraint order
   std::vector<uint32_t> cnt((size_t)GC_KAT * n, 0);
   for (int k = 0; k < c->nk; ++k) {
@@ -820,6 +1195,8 @@ int gc_set_naht_ruhe(gc_ctx* c, const double* ruhe) {
     if (c->k_naht[k]) c->k_ruhe[k] = ruhe[-(c->k_quelle[k] + 1)];
   double* dr = const_cast<double*>(c->opt.praezision == GC_FP32 ? c->sf.k_ruhe : c->sd.k_ruhe);
   if (hoch(c, dr, c->k_ruhe.data(), c->k_ruhe.size())) return 1;
+  c->sk_naht_ruhe.assign(ruhe, ruhe + c->n_naehte);
+  if (c->sk_naht_ruhe_dev && hoch(c, c->sk_naht_ruhe_dev, c->sk_naht_ruhe.data(), c->sk_naht_ruhe.size())) return 1;
   HIPCK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -945,16 +1322,16 @@ int gc_build(gc_ctx* c, double h) {
   if (c->opt.praezision == GC_FP32) {
     c->sf.h = h;
     c->sf.h2 = h * h;
-    rc = sys_hoch<float>(c) || launch_bau<float>(c);
+    rc = selbst_vorbereiten<float>(c) || sys_hoch<float>(c) || launch_bau<float>(c);
   } else {
     c->sd.h = h;
     c->sd.h2 = h * h;
-    rc = sys_hoch<double>(c) || launch_bau<double>(c);
+    rc = selbst_vorbereiten<double>(c) || sys_hoch<double>(c) || launch_bau<double>(c);
   }
   if (rc) return rc;
   HIPCK(c, hipStreamSynchronize(c->stream));
   c->built = true;
-  return 0;
+  return pruefe_gitter(c);
 }
 
 int gc_set_fest(gc_ctx* c, const int32_t* flag, const double* normale, const double* vn_soll, const double* vk) {
@@ -1047,13 +1424,129 @@ int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
     out->freigegeben = g.freigegeben;
     out->rest_max = g.rest_max;
     out->barriere_spins_max = g.spins_max;
+    out->naehe_punkt_dreieck = g.naehe_pd;
+    out->naehe_kante_kante = g.naehe_kk;
+    out->selbst_stoesse = g.selbst_stoesse;
+    out->durchdringungen = g.durchdringungen;
+    out->durchdringungsgebiete = g.gebiete;
   }
   if (reset) {
     g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
+    g.naehe_pd = g.naehe_kk = g.selbst_stoesse = g.durchdringungen = g.gebiete = 0;
     g.rest_max = 0.0;
     g.spins_max = 0;
     HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));
   }
+  return 0;
+}
+
+static int sk_zaehler_lesen(gc_ctx* c, uint32_t* z) {
+  HIPCK(c, hipMemcpyAsync(z, c->sk.zaehler, SK_Z_ANZAHL * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_read_naehe(gc_ctx* c, int32_t max, int32_t* art, uint32_t* punkte4, double* gewichte4, double* normale3, double* abstand,
+                  double* ruhe) {
+  if (!c) return -1;
+  if (!c->sk_bereit) return 0;
+  uint32_t z[SK_Z_ANZAHL];
+  if (sk_zaehler_lesen(c, z)) return -1;
+  const uint32_t m = z[SK_Z_NAEHE];
+  std::vector<Naehe> l(std::min<uint32_t>(m, (uint32_t)std::max(max, 0)));
+  if (!l.empty()) {
+    if (hipMemcpy(l.data(), c->sk.naehe, l.size() * sizeof(Naehe), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  }
+  for (size_t q = 0; q < l.size(); ++q) {
+    if (art) art[q] = (int32_t)l[q].art;
+    for (int k = 0; k < 4; ++k) {
+      if (punkte4) punkte4[4 * q + k] = l[q].punkte[k];
+      if (gewichte4) gewichte4[4 * q + k] = l[q].gewichte[k];
+    }
+    if (normale3) { normale3[3 * q] = l[q].normale.x; normale3[3 * q + 1] = l[q].normale.y; normale3[3 * q + 2] = l[q].normale.z; }
+    if (abstand) abstand[q] = l[q].abstand;
+    if (ruhe) ruhe[q] = l[q].ruhe;
+  }
+  return (int)m;
+}
+
+int gc_read_federn(gc_ctx* c, int32_t max, uint32_t* punkte5, double* gewichte5, double* kraft3, double* block9) {
+  if (!c) return -1;
+  if (!c->sk_bereit) return 0;
+  uint32_t z[SK_Z_ANZAHL];
+  if (sk_zaehler_lesen(c, z)) return -1;
+  const uint32_t m = std::min(z[SK_Z_FEDERN], c->sk.kapazitaet_federn);
+  const size_t k = std::min<uint32_t>(m, (uint32_t)std::max(max, 0));
+  if (k) {
+    if (punkte5 && hipMemcpy(punkte5, c->sk.fe_punkte, 5 * k * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (gewichte5 && hipMemcpy(gewichte5, c->sk.fe_gew, 5 * k * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (kraft3 && hipMemcpy(kraft3, c->sk.fe_kraft, k * sizeof(V3<double>), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (block9 && hipMemcpy(block9, c->sk.fe_block, 9 * k * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  }
+  return (int)m;
+}
+
+int gc_read_schnitte(gc_ctx* c, int32_t max, uint32_t* kante_dreieck, double* t, double* w3, double* richtung3, double* gewicht,
+                     uint32_t* wurzel) {
+  if (!c) return -1;
+  if (!c->sk_bereit) return 0;
+  uint32_t z[SK_Z_ANZAHL];
+  if (sk_zaehler_lesen(c, z)) return -1;
+  const uint32_t m = z[SK_Z_SCHNITTE];
+  const size_t k = std::min<uint32_t>(m, (uint32_t)std::max(max, 0));
+  std::vector<Schnitt> l(k);
+  std::vector<uint32_t> wz(k);
+  if (k) {
+    if (hipMemcpy(l.data(), c->sk.schnitte, k * sizeof(Schnitt), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (hipMemcpy(wz.data(), c->sk.schnitt_wurzel, k * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  }
+  for (size_t q = 0; q < k; ++q) {
+    if (kante_dreieck) { kante_dreieck[2 * q] = l[q].kante; kante_dreieck[2 * q + 1] = l[q].dreieck; }
+    if (t) t[q] = l[q].t;
+    if (w3) for (int j = 0; j < 3; ++j) w3[3 * q + j] = l[q].w[j];
+    if (richtung3) { richtung3[3 * q] = l[q].richtung.x; richtung3[3 * q + 1] = l[q].richtung.y; richtung3[3 * q + 2] = l[q].richtung.z; }
+    if (gewicht) gewicht[q] = l[q].gewicht;
+    if (wurzel) wurzel[q] = wz[q];
+  }
+  return (int)m;
+}
+
+int gc_read_stoesse(gc_ctx* c, int32_t max, int32_t* art, uint32_t* punkte4, double* lage2, double* zeit, double* seite,
+                    double* normale3, int32_t* gilt) {
+  if (!c) return -1;
+  if (!c->sk_bereit) return 0;
+  const size_t slots = (size_t)c->sk.nt + c->sk.ne;
+  std::vector<uint32_t> da(slots);
+  std::vector<Stoss> l(slots);
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (slots) {
+    if (hipMemcpy(da.data(), c->sk.zahl, slots * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (hipMemcpy(l.data(), c->sk.stoesse, slots * sizeof(Stoss), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  }
+  int m = 0;
+  for (size_t q = 0; q < slots; ++q) {
+    if (!da[q]) continue;
+    if (m < max) {
+      const Stoss& st = l[q];
+      if (art) art[m] = (int32_t)st.art;
+      for (int k = 0; k < 4; ++k)
+        if (punkte4) punkte4[4 * m + k] = st.punkte[k];
+      if (lage2) { lage2[2 * m] = st.lage[0]; lage2[2 * m + 1] = st.lage[1]; }
+      if (zeit) zeit[m] = st.zeit;
+      if (seite) seite[m] = st.seite;
+      if (normale3) { normale3[3 * m] = st.normale.x; normale3[3 * m + 1] = st.normale.y; normale3[3 * m + 2] = st.normale.z; }
+      if (gilt) gilt[m] = (st.gilt & 1u) ? 1 : 0;
+    }
+    ++m;
+  }
+  return m;
+}
+
+int gc_get_v(gc_ctx* c, double* v) {
+  if (!c || !c->topo || !v) return 1;
+  V3<double>* dv = c->opt.praezision == GC_FP32 ? c->sf.v : c->sd.v;
+  HIPCK(c, hipMemcpyAsync(v, dv, c->n * sizeof(V3<double>), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 

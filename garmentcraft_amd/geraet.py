@@ -27,6 +27,11 @@ class ImplizitParameter:
     naht_k: float = 500.0
     kontakt_rand: float = 1e-4
     normalkraft_aus_last: bool = False
+    selbstkollision: bool = False
+    selbst_zusatzdicke: float = 2.5e-3
+    selbst_steifigkeit: float = 1e-3
+    selbst_stoss_iterationen: int = 1
+    selbst_schnitte_an_naehten: bool = False
 
 
 @dataclass
@@ -114,6 +119,12 @@ class GeraetSzene:
         t.n_membran = s.membran_punkte.shape[0]
         t.membran_punkte = _p(arr(s.membran_punkte, np.uint32), L.U32)
         t.membran_daten = _p(arr(s.membran_daten, np.float64))
+        t.n_dreiecke = s.dreiecke.shape[0]
+        t.dreiecke = _p(arr(s.dreiecke, np.uint32), L.U32)
+        for name in ("selbst_e_punkt", "selbst_e_dreieck", "selbst_a_dreieck"):
+            a = getattr(s, name, None)
+            if a is not None:
+                setattr(t, name, _p(arr(a, np.float64)))
         self._ck(self.lib.gc_set_topology(self.h, C.byref(t)), "gc_set_topology")
 
     def set_param(self, p: ImplizitParameter) -> None:
@@ -127,6 +138,11 @@ class GeraetSzene:
         c.naht_k = p.naht_k
         c.kontakt_rand = p.kontakt_rand
         c.normalkraft_aus_last = 1 if p.normalkraft_aus_last else 0
+        c.selbstkollision = 1 if p.selbstkollision else 0
+        c.selbst_zusatzdicke = p.selbst_zusatzdicke
+        c.selbst_steifigkeit = p.selbst_steifigkeit
+        c.selbst_stoss_iterationen = p.selbst_stoss_iterationen
+        c.selbst_schnitte_an_naehten = 1 if p.selbst_schnitte_an_naehten else 0
         self._ck(self.lib.gc_set_param(self.h, C.byref(c)), "gc_set_param")
 
     def add_koerper(self, kb) -> int:
@@ -240,3 +256,41 @@ class GeraetSzene:
         r = np.zeros((n_kanten_geraet, 3)); q = np.zeros(n_kanten_geraet); g = np.zeros((n_biegungen, 4, 3))
         self._ck(self.lib.gc_read_records(self.h, _p(r), _p(q), _p(g)), "gc_read_records")
         return {"richtung": r, "quer": q, "grad": g}
+
+    # -- self-collision lists of the last build / step ---------------------
+    def naehe(self) -> dict:
+        m = self.lib.gc_read_naehe(self.h, 0, None, None, None, None, None, None)
+        self._ck(0 if m >= 0 else 1, "gc_read_naehe")
+        art = np.zeros(m, np.int32); p = np.zeros((m, 4), np.uint32); g = np.zeros((m, 4)); nr = np.zeros((m, 3))
+        ab = np.zeros(m); ru = np.zeros(m)
+        self.lib.gc_read_naehe(self.h, m, _p(art, L.I32), _p(p, L.U32), _p(g), _p(nr), _p(ab), _p(ru))
+        return {"art": art, "punkte": p, "gewichte": g, "normale": nr, "abstand": ab, "ruhe": ru}
+
+    def federn(self) -> dict:
+        m = self.lib.gc_read_federn(self.h, 0, None, None, None, None)
+        self._ck(0 if m >= 0 else 1, "gc_read_federn")
+        p = np.zeros((m, 5), np.uint32); g = np.zeros((m, 5)); k = np.zeros((m, 3)); b = np.zeros((m, 3, 3))
+        self.lib.gc_read_federn(self.h, m, _p(p, L.U32), _p(g), _p(k), _p(b))
+        return {"punkte": p, "gewichte": g, "kraft": k, "block": b}
+
+    def schnitte(self) -> dict:
+        m = self.lib.gc_read_schnitte(self.h, 0, None, None, None, None, None, None)
+        self._ck(0 if m >= 0 else 1, "gc_read_schnitte")
+        kd = np.zeros((m, 2), np.uint32); t = np.zeros(m); w = np.zeros((m, 3)); r = np.zeros((m, 3)); g = np.zeros(m)
+        wz = np.zeros(m, np.uint32)
+        self.lib.gc_read_schnitte(self.h, m, _p(kd, L.U32), _p(t), _p(w), _p(r), _p(g), _p(wz, L.U32))
+        return {"kante_dreieck": kd, "t": t, "w": w, "richtung": r, "gewicht": g, "wurzel": wz}
+
+    def stoesse(self) -> dict:
+        m = self.lib.gc_read_stoesse(self.h, 0, None, None, None, None, None, None, None)
+        self._ck(0 if m >= 0 else 1, "gc_read_stoesse")
+        art = np.zeros(m, np.int32); p = np.zeros((m, 4), np.uint32); lage = np.zeros((m, 2)); zeit = np.zeros(m)
+        seite = np.zeros(m); nr = np.zeros((m, 3)); gilt = np.zeros(m, np.int32)
+        self.lib.gc_read_stoesse(self.h, m, _p(art, L.I32), _p(p, L.U32), _p(lage), _p(zeit), _p(seite), _p(nr),
+                                 _p(gilt, L.I32))
+        return {"art": art, "punkte": p, "lage": lage, "zeit": zeit, "seite": seite, "normale": nr, "gilt": gilt}
+
+    def v(self):
+        o = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_get_v(self.h, _p(o)), "gc_get_v")
+        return o

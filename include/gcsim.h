@@ -68,6 +68,15 @@ typedef struct gc_topologie {
   int32_t n_membran;
   const uint32_t* membran_punkte;    /* 3 per element */
   const double* membran_daten;       /* 12 per element */
+  /* Cloth self-collision (selbstkollision.rs:623-693): the cloth triangles
+   * (Szene::dreiecke), the stretch stiffness (N/m) of each point's and each
+   * triangle's piece, and each triangle's rest area. May be 0/NULL when
+   * self-collision stays off. */
+  int32_t n_dreiecke;
+  const uint32_t* dreiecke;          /* 3 per triangle */
+  const double* selbst_e_punkt;      /* n */
+  const double* selbst_e_dreieck;    /* n_dreiecke */
+  const double* selbst_a_dreieck;    /* n_dreiecke */
 } gc_topologie;
 
 /* ImplizitParameter (implizit.rs:456-531) plus the scene's Simparameter
@@ -80,6 +89,14 @@ typedef struct gc_param {
   double naht_k;
   double kontakt_rand;
   int32_t normalkraft_aus_last;
+  /* ImplizitParameter's self-collision fields (implizit.rs:482-494, defaults
+   * 522-526): on/off, extra collision thickness (m), spring stiffness factor,
+   * impact response iterations, search cuts across seams too. */
+  int32_t selbstkollision;
+  double selbst_zusatzdicke;
+  double selbst_steifigkeit;
+  int32_t selbst_stoss_iterationen;
+  int32_t selbst_schnitte_an_naehten;
 } gc_param;
 
 /* CgLauf (implizit.rs:293-299). */
@@ -99,6 +116,11 @@ typedef struct gc_statistik {
   int64_t freigegeben;
   double rest_max;
   int64_t barriere_spins_max;  /* longest grid-barrier wait in polls (diagnostic) */
+  int64_t naehe_punkt_dreieck;
+  int64_t naehe_kante_kante;
+  int64_t selbst_stoesse;
+  int64_t durchdringungen;
+  int64_t durchdringungsgebiete;
 } gc_statistik;
 
 /* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):
@@ -164,6 +186,26 @@ int gc_read_system(gc_ctx* ctx, double* rhs, double* f, double* last, double* pi
 /* Per distance constraint (edges with k != 0 in input order, then seams):
  * direction (3), cross factor; per bending constraint: 4 gradients (12). */
 int gc_read_records(gc_ctx* ctx, double* richtung, double* quer, double* biege_grad);
+
+/* --- self-collision lists of the last build / step -------------------------- */
+
+/* Proximity pairs (selbstkollision::finde, :387-398) of the last build, in the
+ * reference's order. art: 0 point-triangle, 1 edge-edge. Writes up to max,
+ * returns the count. */
+int gc_read_naehe(gc_ctx* ctx, int32_t max, int32_t* art, uint32_t* punkte4, double* gewichte4, double* normale3,
+                  double* abstand, double* ruhe);
+/* Springs of the last build (`System::federn`, implizit.rs:154): proximity
+ * springs then push springs; block row-major. */
+int gc_read_federn(gc_ctx* ctx, int32_t max, uint32_t* punkte5, double* gewichte5, double* kraft3, double* block9);
+/* Cuts of the last build with their region roots (finde_schnitte, gebiete). */
+int gc_read_schnitte(gc_ctx* ctx, int32_t max, uint32_t* kante_dreieck, double* t, double* w3, double* richtung3,
+                     double* gewicht, uint32_t* wurzel);
+/* Impacts found on the path of the last step (finde_stoesse, :1196-1282), in
+ * the reference's order; gilt = 1 when ohne_durchdrungene kept it. */
+int gc_read_stoesse(gc_ctx* ctx, int32_t max, int32_t* art, uint32_t* punkte4, double* lage2, double* zeit,
+                    double* seite, double* normale3, int32_t* gilt);
+/* Velocities (after a build they carry the proximity damping). */
+int gc_get_v(gc_ctx* ctx, double* v);
 
 /* --- the whole step on the device ----------------------------------------- */
 
