@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark: implicit backward-Euler steps/s of the 40k-vertex woven drape at
-PCG rtol 1e-6 on one MI355X, FP32 bulk with FP64 reductions.
+PCG rtol 1e-6 on one MI355X, FP32 bulk with FP64 reductions, cloth
+self-collision on.
 
-One step = one ImplizitCg::schritt of dt = 1/120 s (build, PCG with the
-contact filter, release pass, friction, update) with every input resident in
-HBM. Prints one JSON line with the throughput, the PCG kernel's roofline and
+One step = one ImplizitCg::schritt of dt = 1/120 s (self-collision pairs,
+springs, cuts; build; PCG with the contact filter; release pass; friction;
+update; impacts) with every input resident in HBM. Prints one JSON line with the throughput, the PCG kernel's roofline and
 the FP64 CPU restatement timed on a bounded sample of the same workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--graph] [--praezision fp32|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--graph] [--praezision fp32|fp64] [--ohne-selbstkollision]
 """
 from __future__ import annotations
 
@@ -59,6 +60,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bloecke", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--ohne-selbstkollision", action="store_true", help="self-collision off (the plain drape)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -74,7 +76,8 @@ def main() -> int:
     s, bericht = szenen.drape(sdf=sdf)
     print(f"[bench] scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     dt = s.param.bild_dt
-    param = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0)
+    selbst = not args.ohne_selbstkollision
+    param = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0, selbstkollision=selbst)
     g = GeraetSzene(s, param, praezision=args.praezision, bloecke=args.bloecke, threads=args.threads)
     print(f"[bench] device: {g.name()}", file=sys.stderr, flush=True)
 
@@ -95,7 +98,9 @@ def main() -> int:
     value = args.steps / (wall_ms / 1e3)
     print(f"[bench] {args.steps} steps in {wall_ms:.1f} ms (device {ms_gesamt:.1f} ms, pcg {ms_pcg:.1f} ms), "
           f"{st['cg_iterationen']} CG iterations, {st['nicht_konvergiert']} unconverged solves, "
-          f"longest barrier wait {st['barriere_spins_max']} polls", file=sys.stderr, flush=True)
+          f"longest barrier wait {st['barriere_spins_max']} polls; self-collision: "
+          f"{st['naehe_punkt_dreieck']} point-triangle and {st['naehe_kante_kante']} edge-edge pairs, "
+          f"{st['durchdringungen']} cuts, {st['selbst_stoesse']} impacts", file=sys.stderr, flush=True)
 
     b = pcg_bytes(s, args.praezision)
     iters = st["cg_iterationen"]
@@ -114,7 +119,8 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         import oracle as O
-        o = O.OrakelSim(s, O.param(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0))
+        o = O.OrakelSim(s, O.param(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0,
+                                   selbstkollision=1 if selbst else 0))
         o.setze_lage(x_w, v_w)
         # warm start of the oracle: the device's dv of the last warm-up step is
         # not exported; start from h*g like a fresh solver, which costs the CPU
@@ -134,7 +140,7 @@ def main() -> int:
                "cpu": platform.processor() or platform.machine()}
 
     out = {
-        "metric": "implicit steps/s @40k verts, PCG rtol 1e-6 (plain drape)",
+        "metric": "implicit steps/s @40k verts, PCG rtol 1e-6",
         "value": round(value, 3),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -147,11 +153,14 @@ def main() -> int:
         "dtype": "f32+f64" if args.praezision == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": "drape: 28 woven panels, 24 seams, ellipsoid-cylinder grid SDF 256^3 at 3 mm, "
-                               "friction 0.3, dt 1/120, rtol 1e-6",
+                               "friction 0.3, " + ("self-collision on, " if selbst else "self-collision off, ") +
+                               "dt 1/120, rtol 1e-6",
                    "punkte": s.n, "dreiecke": int(s.dreiecke.shape[0]), "biegungen": b["biegungen"],
-                   "selbstkollision": False, "graph": bool(args.graph), "praezision": args.praezision,
+                   "selbstkollision": selbst, "graph": bool(args.graph), "praezision": args.praezision,
                    "cg_iterationen_je_schritt": round(iters / args.steps, 1), "loesungen": loesungen,
                    "nicht_konvergiert": st["nicht_konvergiert"],
+                   "naehe_je_schritt": round((st["naehe_punkt_dreieck"] + st["naehe_kante_kante"]) / args.steps, 1),
+                   "stoesse": st["selbst_stoesse"], "durchdringungen": st["durchdringungen"],
                    "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -47,6 +47,7 @@ class Param(C.Structure):
         ("normalkraft_aus_last", C.c_int32),
         ("selbstkollision", C.c_int32), ("selbst_zusatzdicke", C.c_double), ("selbst_steifigkeit", C.c_double),
         ("selbst_stoss_iterationen", C.c_int32), ("selbst_schnitte_an_naehten", C.c_int32),
+        ("durchstoss_pruefen", C.c_int32),
     ]
 
 
@@ -60,7 +61,8 @@ class Statistik_c(C.Structure):
         ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
         ("barriere_spins_max", C.c_int64),
         ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
-        ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64),
+        ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64), ("durchstoesse", C.c_int64),
+        ("cg_rundungsgrenze", C.c_int64),
     ]
 
 

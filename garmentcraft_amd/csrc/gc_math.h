@@ -389,3 +389,69 @@ GC_HD bool kontakt(const Koerper& k, V3<double> x, double abstand, V3<double>* n
   *tiefe = t;
   return true;
 }
+
+// Entry of the path a -> b into the body: the counterpart of
+// Koerpernetz::durchstoss (netzkoerper.rs:541-596, used at implizit.rs:984-1003)
+// for the grid SDF that stands in for the mesh body. First sign change
+// outside -> inside of the sampled distance at half-spacing samples along the
+// path, refined by 60 bisections; the normal is the normalised gradient there.
+// A path that starts inside does not enter; a cylinder has none (koerper.rs:101).
+GC_HD bool durchstoss(const Koerper& k, V3<double> a, V3<double> b, V3<double>* p, V3<double>* n, V3<double>* vk) {
+  if (k.art == GC_ZYLINDER) return false;
+  const V3<double> d = b - a;
+  if (dot(d, d) < 1e-24) return false;
+  auto abstand = [&](V3<double> x, double* out) {
+    const double q[3] = {x.x - k.punkt[0], x.y - k.punkt[1], x.z - k.punkt[2]};
+    double g[3];
+    return sdf_sample(k, q, out, g);
+  };
+  double d0;
+  if (abstand(a, &d0) && !(d0 >= 0.0)) return false;
+  // Only the part of the path inside the grid can enter (outside it nothing
+  // is inside the body): clip to the grid box, so the samples are bounded by
+  // the grid's diagonal whatever the path's length.
+  double ta = 0.0, tb = 1.0;
+  for (int q = 0; q < 3; ++q) {
+    const double aq = q == 0 ? a.x : (q == 1 ? a.y : a.z), dq = q == 0 ? d.x : (q == 1 ? d.y : d.z);
+    const double lo = k.origin[q] + k.punkt[q], hi = lo + (double)(k.dims[q] - 1) * k.spacing;
+    if (dq == 0.0) {
+      if (!(aq >= lo && aq <= hi)) return false;
+      continue;
+    }
+    const double u0 = (lo - aq) / dq, u1 = (hi - aq) / dq;
+    ta = fmax(ta, fmin(u0, u1));
+    tb = fmin(tb, fmax(u0, u1));
+  }
+  if (!(ta <= tb)) return false;
+  const double l = norm(d) * (tb - ta);
+  if (!isfinite(l)) return false;
+  int schritte = (int)ceil(l / (0.5 * k.spacing));
+  if (schritte < 1) schritte = 1;
+  double t0 = ta;
+  for (int s = 1; s <= schritte; ++s) {
+    const double t1 = ta + (tb - ta) * ((double)s / (double)schritte);
+    double d1;
+    if (!abstand(a + d * t1, &d1) || d1 >= 0.0) {
+      t0 = t1;
+      continue;
+    }
+    double lo = t0, hi = t1;
+    for (int it = 0; it < 60; ++it) {
+      const double m = 0.5 * (lo + hi);
+      double dm;
+      if (!abstand(a + d * m, &dm) || dm >= 0.0) lo = m;
+      else hi = m;
+    }
+    *p = a + d * hi;
+    const double q[3] = {p->x - k.punkt[0], p->y - k.punkt[1], p->z - k.punkt[2]};
+    double dd, g[3];
+    sdf_sample(k, q, &dd, g);
+    const V3<double> gv = {g[0], g[1], g[2]};
+    const double lg = norm(gv);
+    if (!(lg > 1e-12)) return false;
+    *n = gv * (1.0 / lg);
+    *vk = V3<double>{k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+    return true;
+  }
+  return false;
+}

@@ -107,7 +107,8 @@ struct SelbstDev {
   Quader* quader;                   // max(n, ne, nt)
   uint32_t* eimer_anfang;           // eimer_max + 1
   uint32_t* eimer_fuell;            // eimer_max
-  uint32_t* eintraege;              // kapazitaet_eintraege
+  uint32_t* eintraege;              // kapazitaet_eintraege, bucket contents in entry order
+  uint32_t* eintraege_roh;          // the same after the atomic fill, unordered
   uint32_t kapazitaet_eintraege;
   uint32_t* zahl;                   // max(n, ne, nt) + 1: per-query counts, scanned
   // lists
@@ -367,28 +368,36 @@ __global__ void k_sk_eimer_fuellen(const SelbstDev* __restrict__ sp, int anzahl,
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < anzahl; e += gridDim.x * blockDim.x)
     sk_je_eimer(s, s.quader[e], zelle, eimer, [&](uint32_t h) {
       const uint32_t pos = atomicAdd(&s.eimer_fuell[h], 1u);
-      s.eintraege[pos] = (uint32_t)e;
+      s.eintraege_roh[pos] = (uint32_t)e;
     });
 }
 
-// Entries of a bucket in index order, as the reference's sequential fill leaves them.
+// Entries of a bucket in index order, as the reference's sequential fill
+// leaves them: every filled slot finds its bucket (binary search over the
+// bucket starts) and its rank among the bucket's entries (entries are
+// distinct within a bucket). Parallel over slots, so a bucket that grows
+// large (a coarse cell) costs its size per slot, not its size squared in one
+// thread.
 __global__ void k_sk_eimer_sortieren(const SelbstDev* __restrict__ sp, uint32_t eimer) {
   const SelbstDev& s = *sp;
-  if (s.zaehler[SK_Z_EINTRAEGE] > s.kapazitaet_eintraege) {
+  const uint32_t gesamt = s.zaehler[SK_Z_EINTRAEGE];
+  if (gesamt > s.kapazitaet_eintraege) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&s.zaehler[SK_Z_UEBERLAUF], 1u);
     return;
   }
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < eimer; h += gridDim.x * blockDim.x) {
-    const uint32_t a = s.eimer_anfang[h], b = s.eimer_anfang[h + 1];
-    for (uint32_t i = a + 1; i < b; ++i) {
-      const uint32_t v = s.eintraege[i];
-      uint32_t j = i;
-      while (j > a && s.eintraege[j - 1] > v) {
-        s.eintraege[j] = s.eintraege[j - 1];
-        --j;
-      }
-      s.eintraege[j] = v;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < gesamt; p += gridDim.x * blockDim.x) {
+    // last bucket h with anfang[h] <= p (empty buckets share their start)
+    uint32_t lo = 0, hi = eimer;  // anfang[lo] <= p < anfang[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s.eimer_anfang[mid] <= p) lo = mid;
+      else hi = mid;
     }
+    const uint32_t a = s.eimer_anfang[lo], b = s.eimer_anfang[lo + 1];
+    const uint32_t v = s.eintraege_roh[p];
+    uint32_t rang = 0;
+    for (uint32_t q = a; q < b; ++q) rang += s.eintraege_roh[q] < v ? 1u : 0u;
+    s.eintraege[a + rang] = v;
   }
 }
 

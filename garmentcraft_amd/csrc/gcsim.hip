@@ -374,6 +374,7 @@ static void param_setzen(gc_ctx* c) {
   s.naht_k = c->param.naht_k;
   s.kontakt_rand = c->param.kontakt_rand;
   s.aus_last = c->param.normalkraft_aus_last;
+  s.durchstoss_pruefen = c->param.durchstoss_pruefen;
   s.boden = c->msum * 1e-18;
   s.n_koerper = (int)c->koerper.size();
   for (size_t b = 0; b < c->koerper.size() && b < GC_MAX_KOERPER; ++b) s.koerper[b] = c->koerper[b].k;
@@ -564,6 +565,7 @@ static int selbst_vorbereiten(gc_ctx* c) {
   k.eimer_anfang = dalloc<uint32_t>(c, c->sk_eimer_max + 1); chk(k.eimer_anfang);
   k.eimer_fuell = dalloc<uint32_t>(c, c->sk_eimer_max + 1); chk(k.eimer_fuell);
   k.eintraege = dalloc<uint32_t>(c, k.kapazitaet_eintraege); chk(k.eintraege);
+  k.eintraege_roh = dalloc<uint32_t>(c, k.kapazitaet_eintraege); chk(k.eintraege_roh);
   k.zahl = dalloc<uint32_t>(c, (size_t)n + ne + nt + 1); chk(k.zahl);
   k.naehe = dalloc<Naehe>(c, k.kapazitaet_naehe); chk(k.naehe);
   k.schnitte = dalloc<Schnitt>(c, k.kapazitaet_schnitte); chk(k.schnitte);
@@ -621,7 +623,7 @@ static void sk_hash(gc_ctx* c, int anzahl, const double* zelle_p, uint32_t eimer
   hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, c->sk.eimer_fuell, (int)eimer, c->sk.eimer_anfang,
                      c->sk.zaehler + SK_Z_EINTRAEGE);
   hipLaunchKernelGGL(k_sk_eimer_fuellen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
-  hipLaunchKernelGGL(k_sk_eimer_sortieren, dim3(sk_gitter(eimer)), dim3(BUILD_THREADS), 0, st, sp, eimer);
+  hipLaunchKernelGGL(k_sk_eimer_sortieren, dim3(sk_gitter(c->sk.kapazitaet_eintraege)), dim3(BUILD_THREADS), 0, st, sp, eimer);
 }
 
 // Before the build (implizit.rs:668-707): pairs, springs and damping, cuts,
@@ -1429,10 +1431,13 @@ int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
     out->selbst_stoesse = g.selbst_stoesse;
     out->durchdringungen = g.durchdringungen;
     out->durchdringungsgebiete = g.gebiete;
+    out->durchstoesse = g.durchstoesse;
+    out->cg_rundungsgrenze = g.rundungsgrenze;
   }
   if (reset) {
     g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
-    g.naehe_pd = g.naehe_kk = g.selbst_stoesse = g.durchdringungen = g.gebiete = 0;
+    g.naehe_pd = g.naehe_kk = g.selbst_stoesse = g.durchdringungen = g.gebiete = g.durchstoesse = 0;
+    g.rundungsgrenze = 0;
     g.rest_max = 0.0;
     g.spins_max = 0;
     HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));

@@ -32,6 +32,7 @@ class ImplizitParameter:
     selbst_steifigkeit: float = 1e-3
     selbst_stoss_iterationen: int = 1
     selbst_schnitte_an_naehten: bool = False
+    durchstoss_pruefen: bool = True
 
 
 @dataclass
@@ -143,6 +144,7 @@ class GeraetSzene:
         c.selbst_steifigkeit = p.selbst_steifigkeit
         c.selbst_stoss_iterationen = p.selbst_stoss_iterationen
         c.selbst_schnitte_an_naehten = 1 if p.selbst_schnitte_an_naehten else 0
+        c.durchstoss_pruefen = 1 if p.durchstoss_pruefen else 0
         self._ck(self.lib.gc_set_param(self.h, C.byref(c)), "gc_set_param")
 
     def add_koerper(self, kb) -> int:

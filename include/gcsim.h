@@ -97,6 +97,10 @@ typedef struct gc_param {
   double selbst_steifigkeit;
   int32_t selbst_stoss_iterationen;
   int32_t selbst_schnitte_an_naehten;
+  /* implizit.rs:476 (default on): a path through a body's skin is set onto
+   * its entry (Koerpernetz::durchstoss, netzkoerper.rs:541-596; for the grid
+   * SDF, the first outside -> inside crossing of the sampled distance). */
+  int32_t durchstoss_pruefen;
 } gc_param;
 
 /* CgLauf (implizit.rs:293-299). */
@@ -121,6 +125,8 @@ typedef struct gc_statistik {
   int64_t selbst_stoesse;
   int64_t durchdringungen;
   int64_t durchdringungsgebiete;
+  int64_t durchstoesse;
+  int64_t cg_rundungsgrenze;   /* FP32 solves ended at their rounding floor (no FP64 counterpart) */
 } gc_statistik;
 
 /* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):

@@ -426,6 +426,70 @@ struct Koerper {
     *out = {mul(gv, 1.0 / lg), tiefe, geschwindigkeit};
     return true;
   }
+  // Entry of the path a -> b into the body, the counterpart of
+  // Koerpernetz::durchstoss (netzkoerper.rs:541-596, used at implizit.rs:984-1003)
+  // for the grid SDF that stands in for the mesh body: the first sign change
+  // outside -> inside of the sampled distance, at half-spacing samples along
+  // the path, refined by 60 bisections; the normal is the normalised gradient
+  // there. A path that starts inside does not enter (exits do not count); a
+  // cylinder has none (koerper.rs:101).
+  bool durchstoss(V3 a, V3 b, V3* p, V3* n, V3* vk) const {
+    if (art == 0) return false;
+    V3 d = sub(b, a);
+    if (dot(d, d) < 1e-24) return false;
+    auto abstand = [&](V3 x, double* out) {
+      V3 q = sub(x, versatz);
+      double g[3];
+      return sdf_sample(werte.data(), dims, origin, spacing, q.data(), out, g);
+    };
+    double d0;
+    if (abstand(a, &d0) && !(d0 >= 0.0)) return false;
+    // clip the path to the grid box: outside the grid nothing is inside the
+    // body, and the samples stay bounded by the grid's diagonal
+    double ta = 0.0, tb = 1.0;
+    for (int q = 0; q < 3; ++q) {
+      const double lo = origin[q] + versatz[q], hi = lo + (double)(dims[q] - 1) * spacing;
+      if (d[q] == 0.0) {
+        if (!(a[q] >= lo && a[q] <= hi)) return false;
+        continue;
+      }
+      const double u0 = (lo - a[q]) / d[q], u1 = (hi - a[q]) / d[q];
+      ta = std::fmax(ta, std::fmin(u0, u1));
+      tb = std::fmin(tb, std::fmax(u0, u1));
+    }
+    if (!(ta <= tb)) return false;
+    const double l = norm(d) * (tb - ta);
+    if (!std::isfinite(l)) return false;
+    int schritte = (int)std::ceil(l / (0.5 * spacing));
+    if (schritte < 1) schritte = 1;
+    double t0 = ta;
+    for (int k = 1; k <= schritte; ++k) {
+      const double t1 = ta + (tb - ta) * ((double)k / (double)schritte);
+      double d1;
+      if (!abstand(add(a, mul(d, t1)), &d1) || d1 >= 0.0) {
+        t0 = t1;
+        continue;
+      }
+      double lo = t0, hi = t1;  // inside at hi, not inside at lo
+      for (int it = 0; it < 60; ++it) {
+        const double m = 0.5 * (lo + hi);
+        double dm;
+        if (!abstand(add(a, mul(d, m)), &dm) || dm >= 0.0) lo = m;
+        else hi = m;
+      }
+      *p = add(a, mul(d, hi));
+      V3 q = sub(*p, versatz);
+      double dd, g[3];
+      sdf_sample(werte.data(), dims, origin, spacing, q.data(), &dd, g);
+      V3 gv{g[0], g[1], g[2]};
+      const double lg = norm(gv);
+      if (!(lg > 1e-12)) return false;
+      *n = mul(gv, 1.0 / lg);
+      *vk = geschwindigkeit;
+      return true;
+    }
+    return false;
+  }
   // koerper.rs:122-133
   Koerper verschoben(double t) const {
     Koerper k = *this;
@@ -1030,6 +1094,26 @@ struct or_sim {
       s.v[i] = v;
       V3 alt = s.x[i];
       s.x[i] = add(alt, mul(v, h));
+      // implizit.rs:984-1003: a path through the skin is set onto the entry;
+      // the impact takes the normal velocity relative to the body, Coulomb up
+      // to mu |vn| from the tangential one
+      if (param.durchstoss_pruefen)
+        for (const auto& k : s.koerper) {
+          V3 p, nk, vk;
+          if (k.durchstoss(alt, s.x[i], &p, &nk, &vk)) {
+            stat.durchstoesse += 1;
+            s.x[i] = add(p, mul(nk, s.kollisionsabstand[i]));
+            V3 rel = sub(s.v[i], vk);
+            double vn = dot(rel, nk);
+            if (vn < 0.0) {
+              V3 t = sub(rel, mul(nk, vn));
+              double lt = norm(t);
+              double grenze = s.reibung[i] * -vn;
+              t = lt <= grenze ? V3{0.0, 0.0, 0.0} : mul(t, 1.0 - grenze / lt);
+              s.v[i] = add(vk, t);
+            }
+          }
+        }
       for (const auto& k : s.koerper) {
         Kontakt kt;
         if (k.kontakt(s.x[i], s.kollisionsabstand[i], &kt)) {
@@ -1141,7 +1225,7 @@ int or_sdf_sample(const float* werte, const int dims[3], const double origin[3],
 void or_param_default(or_param* p) {  // implizit.rs:505-531
   p->schritte_je_bild = 5; p->cg_max = 50; p->cg_toleranz = 1e-5; p->naht_k = 500.0; p->kontakt_rand = 1e-4;
   p->normalkraft_aus_last = 0; p->selbstkollision = 0; p->selbst_zusatzdicke = 2.5e-3; p->selbst_steifigkeit = 1e-3;
-  p->selbst_stoss_iterationen = 1; p->selbst_schnitte_an_naehten = 0;
+  p->selbst_stoss_iterationen = 1; p->selbst_schnitte_an_naehten = 0; p->durchstoss_pruefen = 1;
 }
 
 or_sim* or_sim_neu(const or_szene* d, const or_param* p) {

@@ -99,6 +99,7 @@ typedef struct or_param {
   double selbst_steifigkeit;
   int32_t selbst_stoss_iterationen;
   int32_t selbst_schnitte_an_naehten;
+  int32_t durchstoss_pruefen;     /* implizit.rs:476, default on */
 } or_param;
 
 typedef struct or_statistik {
@@ -106,6 +107,7 @@ typedef struct or_statistik {
   double rest_max;
   int64_t naehe_punkt_dreieck, naehe_kante_kante, selbst_stoesse;
   int64_t durchdringungen, durchdringungsgebiete;
+  int64_t durchstoesse;
 } or_statistik;
 
 typedef struct or_sim or_sim;

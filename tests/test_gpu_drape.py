@@ -59,7 +59,10 @@ def test_system_mit_kontakt_wie_orakel(kleiner_drape, praezision):
     dvb, lb = g.loese(dv0)
     print(praezision, "oracle", la, "device", lb)
     assert not lb.nicht_konvergiert
-    assert abs(la["iterationen"] - lb.iterationen) <= max(3, la["iterationen"] // 20)
+    # FP64 follows the oracle's iterates; FP32 reaches rtol 1e-6 close to its
+    # rounding floor, where the last decade can take a few dozen more iterations
+    spiel = la["iterationen"] // 20 if praezision == "fp64" else la["iterationen"] // 8
+    assert abs(la["iterationen"] - lb.iterationen) <= max(3, spiel)
     # both solutions meet the same relative tolerance; their difference is bounded by it
     assert rel(dva, dvb) <= (1e-5 if praezision == "fp64" else 1e-4)
     g.close()
@@ -128,3 +131,37 @@ def test_graph_wie_eager(kleiner_drape):
     xb, vb = b.lage()
     assert np.array_equal(xa, xb) and np.array_equal(va, vb)
     a.close(); b.close()
+
+
+def test_durchstoss_wie_orakel(kleiner_drape):
+    """The cloth thrown at the body at 3 m/s: paths through the skin are set
+    onto the entry (implizit.rs:984-1003); single steps from the oracle's
+    state, FP64, with equal entry counts."""
+    s = kleiner_drape
+    pp = dict(P, cg_toleranz=1e-10)
+    o = O.OrakelSim(s, O.param(**pp))
+    for _ in range(8):
+        o.schritt(H)
+    x, v = o.lage()
+    v = v.copy()
+    v[:, 2] -= 3.0
+    o.setze_lage(x, v)
+    g = GeraetSzene(s, ImplizitParameter(**pp), praezision="fp64")
+    summe = 0
+    for k in range(6):
+        x, v = o.lage()
+        g.set_state(x, v)
+        g.set_dv_vorher(o.dv_vorher())
+        vor = o.statistik()
+        g.statistik(reset=True)
+        o.schritt(H)
+        g.schritte(H, 1)
+        nach = o.statistik()
+        assert nach["durchstoesse"] - vor["durchstoesse"] == g.statistik()["durchstoesse"], k
+        summe += nach["durchstoesse"] - vor["durchstoesse"]
+        xa, _ = o.lage()
+        xb, _ = g.lage()
+        assert float(np.max(np.linalg.norm(xa - xb, axis=1))) <= 1e-9, k
+    print("entries", summe)
+    assert summe >= 5  # the 2 x 2 drape gives 10 entries over these six steps
+    g.close()

@@ -274,3 +274,26 @@ def test_pcg_wie_dichte_loesung():
     dv, lauf = o.sys_loese(dv0)
     assert not lauf["nicht_konvergiert"]
     assert np.max(np.abs(dv.ravel() - dichte)) <= 1e-8 * np.max(np.abs(dichte))
+
+
+@pytest.mark.parametrize("mu", [0.3, 0.05])
+def test_durchstoss_setzt_auf_den_eintritt(mu):
+    """implizit.rs:984-1003 with the grid body: a point 10 mm above a plane,
+    thrown down at 3 m/s with 0.3 m/s sideways, crosses the skin within the
+    step. It is set onto the entry point plus its collision distance; the
+    normal velocity goes, Coulomb on the impact takes up to mu |vn| of the
+    tangential one (here all of it, or 5 % of 3.08 m/s)."""
+    h, abstand = 1.0 / 120.0, 2e-3
+    z0, vx, vz = 0.01, 0.3, -3.0
+    s = ein_punkt((0.01, 0.02, z0), (vx, 0.0, vz), mu, [ebene()])
+    o = O.OrakelSim(s, O.param(schritte_je_bild=1, cg_max=100, cg_toleranz=1e-12))
+    o.schritt(h)
+    x, v = o.lage()
+    vz1 = vz - 9.81 * h                      # free fall in the solve (no contact at the build)
+    t = z0 / (-vz1 * h)                      # entry along the path
+    assert abs(x[0, 0] - (0.01 + vx * h * t)) < 1e-12
+    assert abs(x[0, 2] - abstand) < 1e-12
+    grenze = mu * -vz1
+    soll_vx = 0.0 if vx <= grenze else vx * (1.0 - grenze / vx)
+    assert abs(v[0, 0] - soll_vx) < 1e-12 and abs(v[0, 2]) < 1e-15
+    assert o.statistik()["durchstoesse"] == 1
