@@ -100,6 +100,8 @@ struct SysDev {
   const double* fe_gew;         // 5 per spring
   const V3<double>* fe_kraft;
   const T* fe_block;            // 9 per spring, bulk type
+  V3<T>* fe_t;                  // per entry of fe_ent: its term K (p_q - p_r) (-g_q g_r)
+  uint32_t* fe_tq;              // per entry: the point on side q of its pair
 
   // bodies
   int32_t n_koerper;
@@ -253,6 +255,37 @@ __device__ __forceinline__ V3<T> feder_zeile(const SysDev<T>& s, uint32_t i, V3<
     const V3<T> t = V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
                           B[6] * d.x + B[7] * d.y + B[8] * d.z} * om;
     acc = pq == i ? acc + t : acc - t;
+  }
+  return acc;
+}
+
+// The spring terms of rows lo..hi-1 (the entries fe_off[lo]..fe_off[hi]),
+// computed by the whole block, one entry per thread: each entry gathers two
+// directions and a block of its own, so a row with many springs no longer
+// walks them in sequence while the grid waits on it. Same expression as
+// feder_zeile; the block's __syncthreads() publishes them to feder_summe.
+template <typename T, typename PF>
+__device__ __forceinline__ void feder_terme(const SysDev<T>& s, int lo, int hi, PF p) {
+  const uint32_t e0 = s.fe_off[lo], e1 = s.fe_off[hi];
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const uint32_t code = s.fe_ent[e], f = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
+    const uint32_t pq = s.fe_punkte[5 * f + q], prr = s.fe_punkte[5 * f + r];
+    const T om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
+    const T* B = s.fe_block + 9 * f;
+    const V3<T> d = p(pq) - p(prr);
+    s.fe_t[e] = V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
+                      B[6] * d.x + B[7] * d.y + B[8] * d.z} * om;
+    s.fe_tq[e] = pq;
+  }
+}
+
+// Row i's springs from the terms of feder_terme, added in the order of
+// feder_zeile.
+template <typename T>
+__device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc) {
+  for (uint32_t e = s.fe_off[i]; e < s.fe_off[i + 1]; ++e) {
+    const V3<T> t = s.fe_t[e];
+    acc = s.fe_tq[e] == i ? acc + t : acc - t;
   }
   return acc;
 }
@@ -685,12 +718,13 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
       for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung(s, s.kachel_v[v], po, tb, erste);
       __syncthreads();
       kachel_beitraege(s, t, slot, ul);
+      if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung(s, j, po, tb, erste); });
       __syncthreads();
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const uint8_t fl = s.fflag[i];
         const V3<T> pi = richtung(s, (uint32_t)i, po, tb, erste);
-        const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot),
-                                     [&](uint32_t j) { return richtung(s, j, po, tb, erste); });
+        const V3<T> hk = kachel_zeile(s, i, basis, slot);
+        const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk) : hk;
         const V3<T> ap = filter_fl(fl, s.fnorm[i], hp * th2 + pi * s.mdiag[i]);
         pn[i] = pi;
         s.ap[i] = ap;

@@ -89,6 +89,8 @@ struct gc_ctx {
   uint32_t* sk_st_fuell = nullptr;
   double* sk_naht_ruhe_dev = nullptr;
   void* sk_block_t = nullptr;                // spring blocks in the bulk type
+  void* sk_fe_t = nullptr;                   // per spring entry: its term in the PCG (bulk type)
+  uint32_t* sk_fe_tq = nullptr;              // per spring entry: its point on side q
   // graph of one substep
   hipGraphExec_t graph = nullptr;
 };
@@ -591,6 +593,8 @@ static int selbst_vorbereiten(gc_ctx* c) {
   c->sk_fk_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_fk_fuell);
   c->sk_st_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_st_fuell);
   c->sk_block_t = dalloc<T>(c, 9 * (size_t)k.kapazitaet_federn); chk(c->sk_block_t);
+  c->sk_fe_t = dalloc<V3<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_t);
+  c->sk_fe_tq = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_tq);
   c->sk_dev = dalloc<SelbstDev>(c, 1); chk(c->sk_dev);
   if (!ok) return fehler(c, "hipMalloc/hipMemcpy failed (self-collision buffers)");
   HIPCK(c, hipMemcpyAsync(c->sk_dev, &k, sizeof(SelbstDev), hipMemcpyHostToDevice, c->stream));
@@ -604,6 +608,8 @@ static int selbst_vorbereiten(gc_ctx* c) {
   sy.fe_gew = k.fe_gew;
   sy.fe_kraft = k.fe_kraft;
   sy.fe_block = (const T*)c->sk_block_t;
+  sy.fe_t = (V3<T>*)c->sk_fe_t;
+  sy.fe_tq = c->sk_fe_tq;
   c->sk_bereit = true;
   c->sk_dicke = dicke;
   return 0;

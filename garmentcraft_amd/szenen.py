@@ -369,6 +369,11 @@ class DrapeKonfig:
     hoehe_m: float = 0.34
     material: Optional[Material] = None
     bild_dt: float = 1.0 / 120.0
+    # cloth shift in the plane: the 7 x 4 grid centred on the body would put
+    # the neck under the unsewn border between rows 1 and 2 (a 6 mm gap that
+    # the neck pushes through and the cloth catches on); half a cell in -y
+    # puts it under the middle of a panel
+    versatz_mm: Tuple[float, float] = (0.0, -176.0)
 
 
 def drape_material() -> Material:
@@ -386,8 +391,8 @@ def drape(cfg: Optional[DrapeKonfig] = None, sdf: Optional[GitterSdf] = None, mi
     mat = cfg.material or drape_material()
     rng = np.random.default_rng(cfg.seed)
     nc, nr, cw = cfg.spalten, cfg.zeilen, cfg.zelle_mm
-    x0 = -0.5 * nc * cw
-    y0 = -0.5 * nr * cw
+    x0 = -0.5 * nc * cw + cfg.versatz_mm[0]
+    y0 = -0.5 * nr * cw + cfg.versatz_mm[1]
     ecken = np.zeros((nc + 1, nr + 1, 2))
     for i in range(nc + 1):
         for j in range(nr + 1):

@@ -21,7 +21,8 @@ P = dict(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6)
 @pytest.fixture(scope="module")
 def kleiner_drape():
     sdf = szenen.koerper_sdf(aufloesung=96, spacing=0.008)
-    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0)
+    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0,
+                              versatz_mm=(0.0, 0.0))
     s, _ = szenen.drape(cfg, sdf=sdf)
     return s
 

@@ -155,7 +155,8 @@ def test_schritte_mit_selbstkollision_wie_orakel(praezision, tol_x):
 def test_drape_mit_selbstkollision_graph_wie_eager():
     """The self-collision kernels replay in a captured graph to the bit."""
     sdf = szenen.koerper_sdf(aufloesung=96, spacing=0.008)
-    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0)
+    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0,
+                              versatz_mm=(0.0, 0.0))
     s, _ = szenen.drape(cfg, sdf=sdf)
     a = geraet(s, "fp32", cg_toleranz=1e-6)
     b = geraet(s, "fp32", cg_toleranz=1e-6)
@@ -173,7 +174,8 @@ def test_drape_baut_wie_orakel():
     """The drape after 40 oracle steps with self-collision: the FP64 device build
     gives the oracle's proximity list, springs and right-hand side."""
     sdf = szenen.koerper_sdf(aufloesung=96, spacing=0.008)
-    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0)
+    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0,
+                              versatz_mm=(0.0, 0.0))
     s, _ = szenen.drape(cfg, sdf=sdf)
     lauf = O.OrakelSim(s, O.param(**dict(P, cg_toleranz=1e-6)))
     for _ in range(40):
