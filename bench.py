@@ -61,12 +61,17 @@ def main() -> int:
     ap.add_argument("--bloecke", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--ohne-selbstkollision", action="store_true", help="self-collision off (the plain drape)")
+    ap.add_argument("--workload", default="drape40k", choices=["drape40k", "stapel64"],
+                    help="drape40k: the 40k-point drape; stapel64: 64 garments of 2k-60k points in one batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: one garment's step lives on one device, and "
                                    "sharding across GPUs is not built", "n_gpus": args.gpus}))
         return 2
+
+    if args.workload == "stapel64":
+        return stapel_lauf(args)
 
     from garmentcraft_amd import szenen
     from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
@@ -167,6 +172,101 @@ def main() -> int:
                    "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def stapel_lauf(args) -> int:
+    """64 garments of 2k-60k points (log-uniform), latex or woven, stiffness
+    over one decade, friction 0.1-0.5, all on the drape's body: one batched
+    substep per step (one persistent PCG launch per solve pass for all of them,
+    replayed from a captured graph, launch shape from the autotuner) against
+    the same 64 systems stepped one after another with plain gc_step calls."""
+    from garmentcraft_amd import szenen
+    from garmentcraft_amd.geraet import GeraetStapel, GeraetSzene, ImplizitParameter
+
+    steps = args.steps if args.steps != 300 else 10
+    warmup = args.warmup if args.warmup != 10 else 3
+    t0 = time.time()
+    sdf = szenen.koerper_sdf()
+    satz = szenen.stapel(sdf=sdf, prozesse=min(16, os.cpu_count() or 1))
+    punkte = [s.n for s, _ in satz]
+    print(f"[bench] {len(satz)} scenes, {sum(punkte)} points ({min(punkte)}-{max(punkte)}) built in "
+          f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    dt = 1.0 / 120.0
+    selbst = not args.ohne_selbstkollision
+    param = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0, selbstkollision=selbst)
+
+    def kontexte():
+        gs = []
+        for s, _ in satz:
+            gs.append(GeraetSzene(s, param, praezision=args.praezision, koerper_quelle=gs[0] if gs else None))
+        return gs
+
+    A = kontexte()
+    st = GeraetStapel(A)
+    st.schritte(dt, 1)
+    wahl = st.autotune(dt)
+    nb, threads = st.bloecke()
+    print(f"[bench] autotune: {wahl}", file=sys.stderr, flush=True)
+    if warmup > 1:
+        st.schritte_graph(dt, warmup - 1)
+    for g in A:
+        g.statistik(reset=True)
+    t_a = time.perf_counter()
+    st.schritte_graph(dt, steps)
+    t_b = time.perf_counter()
+    wall_ms = (t_b - t_a) * 1e3
+    value = steps / (wall_ms / 1e3)
+    stat = [g.statistik() for g in A]
+    iters = sum(q["cg_iterationen"] for q in stat)
+    print(f"[bench] batch: {steps} graph steps in {wall_ms:.1f} ms, {iters} CG iterations in all, "
+          f"max per system {max(q['cg_iterationen'] for q in stat)}", file=sys.stderr, flush=True)
+
+    # roofline: the same number of eager batched steps right after, PCG
+    # launches bracketed by HIP events on the batch's stream
+    for g in A:
+        g.statistik(reset=True)
+    st.schritte(dt, steps)
+    _, ms_pcg = st.zeiten()
+    stat_e = [g.statistik() for g in A]
+    algo = 0
+    for (s, _), q in zip(satz, stat_e):
+        b = pcg_bytes(s, args.praezision)
+        algo += q["cg_iterationen"] * b["iteration"] + q["loesungen"] * b["start"]
+    achieved = algo / (ms_pcg * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_pcg_stapel",
+                "ms_pcg_je_schritt": round(ms_pcg / steps, 3)}
+    st.close()
+
+    # the same systems, one after another, plain gc_step per system and step
+    B = kontexte()
+    for g in B:
+        g.schritte(dt, warmup)
+    t_c = time.perf_counter()
+    for _ in range(steps):
+        for g in B:
+            g.schritte(dt, 1)
+    t_d = time.perf_counter()
+    seq = steps / (t_d - t_c)
+    print(f"[bench] sequential: {steps} steps of {len(B)} systems in {(t_d - t_c) * 1e3:.1f} ms", file=sys.stderr,
+          flush=True)
+    out = {
+        "metric": "implicit steps/s @40k verts, PCG rtol 1e-6; batched/adjoint vs plain GPU vs CPU",
+        "value": round(value, 3), "unit": "batched steps/s (64 garments per step)", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(wall_ms / steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if args.praezision == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": "stapel64: 64 garments, points log-uniform in [2k, 60k], latex or woven, stiffness "
+                               "x10^U(-0.5,0.5), friction U(0.1,0.5), shared 256^3 SDF, "
+                               + ("self-collision on, " if selbst else "self-collision off, ") + "dt 1/120, rtol 1e-6",
+                   "punkte": sum(punkte), "systeme": len(satz), "graph": True, "praezision": args.praezision,
+                   "threads": threads, "bloecke": int(nb.sum()), "autotune_ms_loesung": round(wahl["ms_loesung"], 3),
+                   "sequentiell_steps_s": round(seq, 3), "stapel_gegen_sequentiell": round(value / seq, 2),
+                   "cg_iterationen_je_schritt": round(iters / steps, 1), "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": None,
     }
     print(json.dumps(out))
     return 0

@@ -102,6 +102,16 @@ SIGNATUREN = [
     ("gc_read_schnitte", C.c_int, [C.c_void_p, C.c_int32, U32, D, D, D, D, U32]),
     ("gc_read_stoesse", C.c_int, [C.c_void_p, C.c_int32, I32, U32, D, D, D, D, I32]),
     ("gc_get_v", C.c_int, [C.c_void_p, D]),
+    ("gc_add_sdf_von", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, D, D]),
+    ("gc_batch_create", C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]),
+    ("gc_batch_destroy", None, [C.c_void_p]),
+    ("gc_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("gc_batch_step", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    ("gc_batch_step_graph", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    ("gc_batch_autotune", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_double)]),
+    ("gc_batch_bloecke", C.c_int, [C.c_void_p, I32, C.POINTER(C.c_int32)]),
+    ("gc_batch_zeiten", C.c_int, [C.c_void_p, D, D]),
 ]
 
 _lib = None

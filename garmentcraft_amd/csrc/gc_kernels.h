@@ -167,18 +167,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
   __syncthreads();
 }
 
+// rang, nb: this block's rank among the nb blocks that share gs and slots (the
+// whole grid for one system, a group of blocks in a batched launch).
 template <int NV>
-__device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch) {
+__device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
+                                               unsigned rang, unsigned nb) {
   block_sum<NV>(v, sh);
   ++epoch;
-  const unsigned nb = gridDim.x;
   Slot* buf = slots + (epoch & 1u) * GC_MAX_BLOECKE;
   __shared__ int s_fehler;
   // block_sum ended with a workgroup barrier: every wave's stores happen-before
   // thread 0's agent-scope release below, which makes them visible to the
   // other blocks (release cumulativity of the HSA memory model).
   if (threadIdx.x == 0) {
-    for (int k = 0; k < NV; ++k) __hip_atomic_store(&buf[blockIdx.x].v[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < NV; ++k) __hip_atomic_store(&buf[rang].v[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(&gs->zaehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned ziel = nb * epoch;
@@ -622,23 +624,22 @@ __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const 
 // durchgang 1 is the second solve after a release; it returns at once unless
 // a point was released.
 // ---------------------------------------------------------------------------
+// The solve of one system by the nb blocks of rank 0..nb-1 that share gs and
+// slots; lds_roh is the block's dynamic LDS.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
-  const SysDev<T>& s = *sp;
-  extern __shared__ __align__(16) unsigned char lds_roh[];
+__device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Slot* slots, int durchgang, int rang, int nb,
+                                           unsigned char* lds_roh, double* sh) {
   V3<T>* slot = (V3<T>*)lds_roh;
   V3<T>* ul = slot + s.kachel_slots_max;
-  __shared__ double sh[16 * 2];
   if (durchgang == 1) {
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
-  const int nb = gridDim.x;
   unsigned epoch = 0;
   auto zeilen = [&](int t, int* lo, int* hi) { *lo = s.kachel_zeilen[2 * t]; *hi = s.kachel_zeilen[2 * t + 1]; };
   const T th2 = T(s.h2);
 
   // start: held along the normal get the target normal velocity (implizit.rs:375-384)
-  for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+  for (int t = rang; t < s.n_kacheln; t += nb) {
     int lo, hi;
     zeilen(t, &lo, &hi);
     for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
@@ -655,11 +656,11 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
   }
   {
     double nix[1] = {0.0};
-    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch)) return;
+    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
   }
   // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z) (implizit.rs:385-391)
   double part[2] = {0.0, 0.0};
-  for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+  for (int t = rang; t < s.n_kacheln; t += nb) {
     int lo, hi;
     zeilen(t, &lo, &hi);
     const uint32_t basis = s.off[GC_KAT * lo];
@@ -687,7 +688,7 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
     }
     __syncthreads();
   }
-  if (!grid_allreduce<2>(part, sh, slots, gs, epoch)) return;
+  if (!grid_allreduce<2>(part, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
   const double bb = part[0];
   double delta = part[1];
   const double ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
@@ -710,7 +711,7 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
     const bool erste = it == 0;
     // ap = S(A p), ca = p . ap (implizit.rs:401-405)
     double ca[1] = {0.0};
-    for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+    for (int t = rang; t < s.n_kacheln; t += nb) {
       int lo, hi;
       zeilen(t, &lo, &hi);
       const uint32_t basis = s.off[GC_KAT * lo];
@@ -732,13 +733,13 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
       }
       __syncthreads();
     }
-    if (!grid_allreduce<1>(ca, sh, slots, gs, epoch)) return;
+    if (!grid_allreduce<1>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
     if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
     const double alpha = delta / ca[0];
     // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
     double neu[1] = {0.0};
     const T ta = T(alpha);
-    for (int t = blockIdx.x; t < s.n_kacheln; t += nb) {
+    for (int t = rang; t < s.n_kacheln; t += nb) {
       int lo, hi;
       zeilen(t, &lo, &hi);
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
@@ -750,7 +751,7 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
         neu[0] += (double)dot(ri, zi);
       }
     }
-    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch)) return;
+    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
     beta = neu[0] / delta;
     delta = neu[0];
     cur ^= 1;
@@ -765,7 +766,7 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
       }
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (rang == 0 && threadIdx.x == 0) {
     gs->iterationen = it;
     gs->nicht_konv = delta > ziel ? 1 : 0;
     gs->rest = sqrt(delta / gmax(bb, s.boden));
@@ -778,6 +779,32 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
     if (grenze) gs->rundungsgrenze += 1;
     gs->rest_max = gmax(gs->rest_max, gs->rest);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
+  extern __shared__ __align__(16) unsigned char lds_roh[];
+  __shared__ double sh[16 * 2];
+  pcg_loesen<T>(*sp, gs, slots, durchgang, (int)blockIdx.x, (int)gridDim.x, lds_roh, sh);
+}
+
+// Batched solves: block b works for system tab[b].sys as rank tab[b].rang of
+// its tab[b].nb blocks; every system runs its own iterations, reductions and
+// stopping test, all in one persistent launch (all blocks co-resident).
+template <typename T>
+struct StapelBlock {
+  const SysDev<T>* sys;
+  GridState* gs;
+  Slot* slots;
+  int32_t rang, nb;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_pcg_stapel(const StapelBlock<T>* __restrict__ tab, int durchgang) {
+  extern __shared__ __align__(16) unsigned char lds_roh[];
+  __shared__ double sh[16 * 2];
+  const StapelBlock<T> e = tab[blockIdx.x];
+  pcg_loesen<T>(*e.sys, e.gs, e.slots, durchgang, e.rang, e.nb, lds_roh, sh);
 }
 
 // Release pass (implizit.rs:916-942): normal force from the residual

@@ -749,27 +749,51 @@ static int launch_pcg(gc_ctx* c, int durchgang) {
   return 0;
 }
 
+// A substep in three parts around its two PCG launches, so that a batch can
+// put the solves of many systems into one launch: before (positions at the
+// start, self-collision, build, warm start), between (release pass 0), after
+// (release pass 1, friction and update, impacts).
 template <typename T>
-// ev: four events (before/after each PCG launch) or nullptr
-static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
+static int launch_vor_pcg(gc_ctx* c, int hat_vorher) {
+  const SysDev<T>& s = sys<T>(c);
+  if (s.selbst) hipLaunchKernelGGL(k_sk_kopie_x, dim3(sk_gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, c->sk_dev);
+  if (launch_bau<T>(c)) return 1;
+  hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, dsys<T>(c), hat_vorher);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int launch_zwischen_pcg(gc_ctx* c) {
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(sys<T>(c).n)), dim3(BUILD_THREADS), 0, c->stream, dsys<T>(c), c->gs, 0);
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int launch_nach_pcg(gc_ctx* c) {
   const SysDev<T>& s = sys<T>(c);
   const SysDev<T>* sp = dsys<T>(c);
-  hipStream_t st = c->stream;
-  if (s.selbst) hipLaunchKernelGGL(k_sk_kopie_x, dim3(sk_gitter(s.n)), dim3(BUILD_THREADS), 0, st, c->sk_dev);
-  if (launch_bau<T>(c)) return 1;
-  hipLaunchKernelGGL(k_warm<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, hat_vorher);
-  if (ev) HIPCK(c, hipEventRecord(ev[0], st));
-  if (launch_pcg<T>(c, 0)) return 1;
-  if (ev) HIPCK(c, hipEventRecord(ev[1], st));
-  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 0);
-  if (ev) HIPCK(c, hipEventRecord(ev[2], st));
-  if (launch_pcg<T>(c, 1)) return 1;
-  if (ev) HIPCK(c, hipEventRecord(ev[3], st));
-  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs, 1);
-  hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs);
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs, 1);
+  hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs);
   if (s.selbst && launch_selbst_stoss<T>(c)) return 1;
   HIPCK(c, hipGetLastError());
   return 0;
+}
+
+template <typename T>
+// ev: four events (before/after each PCG launch) or nullptr
+static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
+  hipStream_t st = c->stream;
+  if (launch_vor_pcg<T>(c, hat_vorher)) return 1;
+  if (ev) HIPCK(c, hipEventRecord(ev[0], st));
+  if (launch_pcg<T>(c, 0)) return 1;
+  if (ev) HIPCK(c, hipEventRecord(ev[1], st));
+  if (launch_zwischen_pcg<T>(c)) return 1;
+  if (ev) HIPCK(c, hipEventRecord(ev[2], st));
+  if (launch_pcg<T>(c, 1)) return 1;
+  if (ev) HIPCK(c, hipEventRecord(ev[3], st));
+  return launch_nach_pcg<T>(c);
 }
 
 static int pruefe_gitter(gc_ctx* c) {
@@ -867,6 +891,418 @@ static int schritte_graph(gc_ctx* c, double h, int n) {
   c->ms_pcg = std::nan("");
   c->built = false;
   return pruefe_gitter(c);
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many systems, one persistent PCG launch per solve pass
+// ---------------------------------------------------------------------------
+struct gc_batch {
+  std::vector<gc_ctx*> ctx;
+  std::string err;
+  hipStream_t stream = nullptr;
+  int praezision = GC_FP32;
+  int cus = 0;
+  int threads = 1024;
+  size_t lds = 0;
+  int max_bloecke = 0;             // co-resident blocks of the batched kernel
+  std::vector<double> gewicht;     // per system: tiles x iterations per solve (last measured)
+  std::vector<int> bloecke;        // per system: blocks in the batched launch
+  void* tab = nullptr;             // StapelBlock<T>[sum of bloecke]
+  int n_tab = 0;
+  hipGraphExec_t graph = nullptr;
+  std::vector<hipEvent_t> ev;
+  double ms_schritt = 0.0, ms_pcg = 0.0;
+  // the systems' own kernels (build, release pass, update) run on several
+  // branch streams, forked from and joined back into the batch's stream
+  std::vector<hipStream_t> zweige;
+  std::vector<hipEvent_t> gabel;   // per phase: fork event, then one join event per branch
+};
+
+static int bfehler(gc_batch* b, const std::string& m) {
+  b->err = m;
+  return 1;
+}
+
+#define BHIPCK(b, call)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);                         \
+      return 1;                                                                             \
+    }                                                                                       \
+  } while (0)
+
+// While a batch call runs, system k works on branch stream k mod branches.
+struct StromTausch {
+  gc_batch* b;
+  std::vector<hipStream_t> alt;
+  explicit StromTausch(gc_batch* bb) : b(bb) {
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+      alt.push_back(b->ctx[k]->stream);
+      b->ctx[k]->stream = b->zweige[k % b->zweige.size()];
+    }
+  }
+  ~StromTausch() {
+    for (size_t k = 0; k < b->ctx.size(); ++k) b->ctx[k]->stream = alt[k];
+  }
+};
+
+// Blocks per system in proportion to its weight (tiles x iterations), at least
+// one and at most one per tile, summing to at most `gesamt`.
+static void bloecke_verteilen(gc_batch* b, int gesamt) {
+  const size_t n = b->ctx.size();
+  double sw = 0.0;
+  for (size_t k = 0; k < n; ++k) sw += b->gewicht[k];
+  std::vector<int> nb(n);
+  int summe = 0;
+  for (size_t k = 0; k < n; ++k) {
+    const int kacheln = std::max(1, b->ctx[k]->n_kacheln);
+    int q = (int)std::floor((double)gesamt * b->gewicht[k] / std::max(sw, 1e-300));
+    q = std::min(std::max(q, 1), std::min(kacheln, GC_MAX_BLOECKE));
+    nb[k] = q;
+    summe += q;
+  }
+  // hand out what is left to the systems with the most weight per block
+  while (summe < gesamt) {
+    int best = -1;
+    double bw = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+      if (nb[k] >= std::min(std::max(1, b->ctx[k]->n_kacheln), GC_MAX_BLOECKE)) continue;
+      const double w = b->gewicht[k] / nb[k];
+      if (w > bw) { bw = w; best = (int)k; }
+    }
+    if (best < 0) break;
+    ++nb[best];
+    ++summe;
+  }
+  while (summe > gesamt) {  // only when there are more systems than blocks allow
+    int best = -1;
+    double bw = 1e300;
+    for (size_t k = 0; k < n; ++k) {
+      if (nb[k] <= 1) continue;
+      const double w = b->gewicht[k] / nb[k];
+      if (w < bw) { bw = w; best = (int)k; }
+    }
+    if (best < 0) break;
+    --nb[best];
+    --summe;
+  }
+  b->bloecke = nb;
+}
+
+template <typename T>
+static int tabelle_laden(gc_batch* b) {
+  std::vector<StapelBlock<T>> tab;
+  for (size_t k = 0; k < b->ctx.size(); ++k) {
+    gc_ctx* c = b->ctx[k];
+    for (int r = 0; r < b->bloecke[k]; ++r)
+      tab.push_back(StapelBlock<T>{dsys<T>(c), c->gs, c->slots, (int32_t)r, (int32_t)b->bloecke[k]});
+  }
+  if ((int)tab.size() > b->max_bloecke) return bfehler(b, "batch needs more blocks than can be co-resident");
+  if (b->tab) (void)hipFree(b->tab);
+  b->tab = nullptr;
+  BHIPCK(b, hipMalloc(&b->tab, tab.size() * sizeof(StapelBlock<T>)));
+  BHIPCK(b, hipMemcpyAsync(b->tab, tab.data(), tab.size() * sizeof(StapelBlock<T>), hipMemcpyHostToDevice, b->stream));
+  BHIPCK(b, hipStreamSynchronize(b->stream));
+  b->n_tab = (int)tab.size();
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return 0;
+}
+
+template <typename T>
+static int batch_form(gc_batch* b, int threads, int gesamt) {
+  size_t lds = 16;
+  for (gc_ctx* c : b->ctx) lds = std::max(lds, c->pcg_lds);
+  BHIPCK(b, hipFuncSetAttribute((const void*)k_pcg_stapel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int occ = 0;
+  BHIPCK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg_stapel<T>, threads, lds));
+  if (occ < 1) return bfehler(b, "batched PCG kernel does not fit on a CU");
+  b->threads = threads;
+  b->lds = lds;
+  b->max_bloecke = occ * b->cus;
+  const int n = (int)b->ctx.size();
+  if (n > b->max_bloecke) return bfehler(b, "more systems than co-resident blocks");
+  bloecke_verteilen(b, std::max(n, std::min(gesamt > 0 ? gesamt : b->max_bloecke, b->max_bloecke)));
+  return tabelle_laden<T>(b);
+}
+
+template <typename T>
+static int batch_pcg(gc_batch* b, int durchgang) {
+  hipLaunchKernelGGL(k_pcg_stapel<T>, dim3(b->n_tab), dim3(b->threads), b->lds, b->stream, (const StapelBlock<T>*)b->tab,
+                     durchgang);
+  BHIPCK(b, hipGetLastError());
+  return 0;
+}
+
+// Branch streams wait for the batch stream (phase ph's fork event) ...
+static int gabeln(gc_batch* b, int ph) {
+  const size_t z = b->zweige.size();
+  hipEvent_t* e = &b->gabel[(size_t)ph * (z + 1)];
+  BHIPCK(b, hipEventRecord(e[0], b->stream));
+  for (size_t q = 0; q < z; ++q) BHIPCK(b, hipStreamWaitEvent(b->zweige[q], e[0], 0));
+  return 0;
+}
+// ... and the batch stream waits for every branch.
+static int vereinen(gc_batch* b, int ph) {
+  const size_t z = b->zweige.size();
+  hipEvent_t* e = &b->gabel[(size_t)ph * (z + 1)];
+  for (size_t q = 0; q < z; ++q) {
+    BHIPCK(b, hipEventRecord(e[1 + q], b->zweige[q]));
+    BHIPCK(b, hipStreamWaitEvent(b->stream, e[1 + q], 0));
+  }
+  return 0;
+}
+
+template <typename T>
+static int batch_schritt(gc_batch* b, const hipEvent_t* ev) {
+  if (gabeln(b, 0)) return 1;
+  for (gc_ctx* c : b->ctx)
+    if (launch_vor_pcg<T>(c, c->hat_vorher ? 1 : 0)) return bfehler(b, c->err);
+  if (vereinen(b, 0)) return 1;
+  if (ev) BHIPCK(b, hipEventRecord(ev[0], b->stream));
+  if (batch_pcg<T>(b, 0)) return 1;
+  if (ev) BHIPCK(b, hipEventRecord(ev[1], b->stream));
+  if (gabeln(b, 1)) return 1;
+  for (gc_ctx* c : b->ctx)
+    if (launch_zwischen_pcg<T>(c)) return bfehler(b, c->err);
+  if (vereinen(b, 1)) return 1;
+  if (ev) BHIPCK(b, hipEventRecord(ev[2], b->stream));
+  if (batch_pcg<T>(b, 1)) return 1;
+  if (ev) BHIPCK(b, hipEventRecord(ev[3], b->stream));
+  if (gabeln(b, 2)) return 1;
+  for (gc_ctx* c : b->ctx)
+    if (launch_nach_pcg<T>(c)) return bfehler(b, c->err);
+  return vereinen(b, 2);
+}
+
+static int batch_sync(gc_batch* b) {
+  for (hipStream_t z : b->zweige) BHIPCK(b, hipStreamSynchronize(z));
+  BHIPCK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+template <typename T>
+static int batch_vorbereiten(gc_batch* b, double h) {
+  for (gc_ctx* c : b->ctx) {
+    if (!c->topo || !c->hat_param) return bfehler(b, "every system of a batch needs topology and parameters");
+    sys<T>(c).h = h;
+    sys<T>(c).h2 = h * h;
+    if (selbst_vorbereiten<T>(c) || sys_hoch<T>(c)) return bfehler(b, c->err);
+  }
+  return 0;
+}
+
+static int batch_ev(gc_batch* b, size_t anzahl) {
+  while (b->ev.size() < anzahl) {
+    hipEvent_t e;
+    BHIPCK(b, hipEventCreate(&e));
+    b->ev.push_back(e);
+  }
+  return 0;
+}
+
+template <typename T>
+static int batch_pruefen(gc_batch* b) {
+  if (batch_sync(b)) return 1;
+  for (gc_ctx* c : b->ctx) {
+    c->built = false;
+    if (pruefe_gitter(c)) return bfehler(b, c->err);
+  }
+  return 0;
+}
+
+template <typename T>
+static int batch_schritte(gc_batch* b, double h, int n) {
+  StromTausch tausch(b);
+  if (batch_vorbereiten<T>(b, h)) return 1;
+  if (batch_ev(b, 4 * (size_t)n + 2)) return 1;
+  BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
+  for (int k = 0; k < n; ++k) {
+    if (batch_schritt<T>(b, &b->ev[2 + 4 * k])) return 1;
+    for (gc_ctx* c : b->ctx) c->hat_vorher = true;
+  }
+  BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
+  BHIPCK(b, hipEventSynchronize(b->ev[1]));
+  float ms = 0.f;
+  BHIPCK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->ms_schritt = ms;
+  double pcg = 0.0;
+  for (int k = 0; k < n; ++k) {
+    float m0 = 0.f, m1 = 0.f;
+    BHIPCK(b, hipEventElapsedTime(&m0, b->ev[2 + 4 * k], b->ev[3 + 4 * k]));
+    BHIPCK(b, hipEventElapsedTime(&m1, b->ev[4 + 4 * k], b->ev[5 + 4 * k]));
+    pcg += (double)m0 + (double)m1;
+  }
+  b->ms_pcg = pcg;
+  return batch_pruefen<T>(b);
+}
+
+template <typename T>
+static int batch_schritte_graph(gc_batch* b, double h, int n) {
+  if (n <= 0) return 0;
+  int done = 0;
+  bool alle = true;
+  for (gc_ctx* c : b->ctx) alle = alle && c->hat_vorher;
+  if (!alle) {  // the first substep starts from h*g, the captured one from dv_vorher
+    if (batch_schritte<T>(b, h, 1)) return 1;
+    done = 1;
+    if (n == 1) return 0;
+  }
+  StromTausch tausch(b);
+  if (batch_vorbereiten<T>(b, h)) return 1;
+  if (!b->graph) {
+    hipGraph_t g;
+    BHIPCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_schritt<T>(b, nullptr);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (rc) return 1;
+    if (e2 != hipSuccess) return bfehler(b, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return bfehler(b, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+  }
+  if (batch_ev(b, 2)) return 1;
+  BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
+  for (int k = done; k < n; ++k) BHIPCK(b, hipGraphLaunch(b->graph, b->stream));
+  BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
+  BHIPCK(b, hipEventSynchronize(b->ev[1]));
+  float ms = 0.f;
+  BHIPCK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->ms_schritt = ms;
+  b->ms_pcg = std::nan("");
+  return batch_pruefen<T>(b);
+}
+
+// Weights from the last measured iterations per solve of every system.
+static int batch_gewichte_messen(gc_batch* b) {
+  for (size_t k = 0; k < b->ctx.size(); ++k) {
+    GridState g;
+    BHIPCK(b, hipMemcpy(&g, b->ctx[k]->gs, sizeof(GridState), hipMemcpyDeviceToHost));
+    const double it = g.loesungen > 0 ? (double)g.cg_iterationen / (double)g.loesungen : 1.0;
+    b->gewicht[k] = (double)std::max(1, b->ctx[k]->n_kacheln) * std::max(it, 1.0);
+  }
+  return 0;
+}
+
+// Autotuner: builds every system once, then times the batched first solve from
+// the same warm start for each candidate launch shape (threads per block, total
+// blocks, blocks by tiles or by tiles x measured iterations) and keeps the
+// fastest. Positions, velocities, the warm start and the counters are restored;
+// the run counters of self-collision cuts advance by the one build.
+template <typename T>
+static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* bloecke_out, double* ms_out) {
+  StromTausch tausch(b);
+  if (batch_vorbereiten<T>(b, h)) return 1;
+  const size_t n = b->ctx.size();
+  std::vector<void*> tmp;
+  auto neu = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(bytes, 1)) != hipSuccess) return nullptr;
+    tmp.push_back(q);
+    return q;
+  };
+  struct Gesichert { void *x, *v, *dv; GridState alt, start; };
+  std::vector<Gesichert> g(n);
+  int rc = batch_sync(b);
+  auto ende = [&]() {
+    for (void* q : tmp) (void)hipFree(q);
+    return rc;
+  };
+  for (size_t k = 0; k < n && !rc; ++k) {
+    gc_ctx* c = b->ctx[k];
+    const SysDev<T>& s = sys<T>(c);
+    g[k].x = neu(s.n * sizeof(V3<double>));
+    g[k].v = neu(s.n * sizeof(V3<double>));
+    g[k].dv = neu(s.n * sizeof(V3<T>));
+    if (!g[k].x || !g[k].v || !g[k].dv) { rc = bfehler(b, "hipMalloc failed (autotune)"); break; }
+    if (hipMemcpyAsync(g[k].x, s.x, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(g[k].v, s.v, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(&g[k].alt, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, b->stream) != hipSuccess)
+      rc = bfehler(b, "hipMemcpy failed (autotune)");
+  }
+  if (rc) return ende();
+  if (gabeln(b, 0)) { rc = 1; return ende(); }
+  for (gc_ctx* c : b->ctx)
+    if (launch_vor_pcg<T>(c, c->hat_vorher ? 1 : 0)) { rc = bfehler(b, c->err); return ende(); }
+  if (vereinen(b, 0) || batch_sync(b)) { rc = 1; return ende(); }
+  for (size_t k = 0; k < n && !rc; ++k) {
+    const SysDev<T>& s = sys<T>(b->ctx[k]);
+    if (hipMemcpyAsync(g[k].dv, s.dv, s.n * sizeof(V3<T>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(&g[k].start, b->ctx[k]->gs, sizeof(GridState), hipMemcpyDeviceToHost, b->stream) != hipSuccess)
+      rc = bfehler(b, "hipMemcpy failed (autotune)");
+  }
+  if (rc || hipStreamSynchronize(b->stream) != hipSuccess) { rc = rc ? rc : bfehler(b, "autotune copy failed"); return ende(); }
+  if (batch_ev(b, 2)) { rc = 1; return ende(); }
+  auto versuch = [&](int threads, int gesamt, double* ms) -> int {
+    if (batch_form<T>(b, threads, gesamt)) return 1;
+    for (size_t k = 0; k < n; ++k) {
+      const SysDev<T>& s = sys<T>(b->ctx[k]);
+      BHIPCK(b, hipMemcpyAsync(s.dv, g[k].dv, s.n * sizeof(V3<T>), hipMemcpyDeviceToDevice, b->stream));
+      BHIPCK(b, hipMemcpyAsync(b->ctx[k]->gs, &g[k].start, sizeof(GridState), hipMemcpyHostToDevice, b->stream));
+    }
+    BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
+    if (batch_pcg<T>(b, 0)) return 1;
+    BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
+    BHIPCK(b, hipEventSynchronize(b->ev[1]));
+    float m = 0.f;
+    BHIPCK(b, hipEventElapsedTime(&m, b->ev[0], b->ev[1]));
+    *ms = m;
+    for (gc_ctx* c : b->ctx)
+      if (pruefe_gitter(c)) return bfehler(b, c->err);
+    return 0;
+  };
+  // weights by tiles; one solve measures every system's iterations
+  std::vector<double> nach_kacheln(n), nach_arbeit(n);
+  for (size_t k = 0; k < n; ++k) nach_kacheln[k] = (double)std::max(1, b->ctx[k]->n_kacheln);
+  b->gewicht = nach_kacheln;
+  double ms = 0.0;
+  if (versuch(1024, 0, &ms)) { rc = 1; return ende(); }
+  for (size_t k = 0; k < n; ++k) {
+    GridState q;
+    if (hipMemcpy(&q, b->ctx[k]->gs, sizeof(GridState), hipMemcpyDeviceToHost) != hipSuccess) { rc = bfehler(b, "hipMemcpy"); return ende(); }
+    nach_arbeit[k] = nach_kacheln[k] * std::max(1.0, (double)q.iterationen);
+  }
+  int best_t = 1024, best_b = 0;
+  int best_g = 0;
+  double best_ms = 1e300;
+  const int threads_kand[3] = {256, 512, 1024};
+  const double anteil[4] = {0.25, 0.5, 0.75, 1.0};
+  for (int gw = 0; gw < 2; ++gw)
+    for (int ti = 0; ti < 3; ++ti)
+      for (int ai = 0; ai < 4; ++ai) {
+        b->gewicht = gw ? nach_arbeit : nach_kacheln;
+        // blocks available at this thread count
+        int occ = 0;
+        size_t lds = 16;
+        for (gc_ctx* c : b->ctx) lds = std::max(lds, c->pcg_lds);
+        if (hipFuncSetAttribute((const void*)k_pcg_stapel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg_stapel<T>, threads_kand[ti], lds) != hipSuccess ||
+            occ < 1)
+          continue;
+        const int gesamt = std::max((int)n, (int)std::lround(anteil[ai] * occ * b->cus));
+        double m = 0.0;
+        if (versuch(threads_kand[ti], gesamt, &m)) { rc = 1; return ende(); }
+        if (m < best_ms) { best_ms = m; best_t = threads_kand[ti]; best_b = gesamt; best_g = gw; }
+      }
+  b->gewicht = best_g ? nach_arbeit : nach_kacheln;
+  if (batch_form<T>(b, best_t, best_b)) { rc = 1; return ende(); }
+  for (size_t k = 0; k < n && !rc; ++k) {
+    gc_ctx* c = b->ctx[k];
+    const SysDev<T>& s = sys<T>(c);
+    if (hipMemcpyAsync(s.x, g[k].x, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(s.v, g[k].v, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(c->gs, &g[k].alt, sizeof(GridState), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+      rc = bfehler(b, "hipMemcpy failed (autotune restore)");
+    c->built = false;
+  }
+  if (!rc && hipStreamSynchronize(b->stream) != hipSuccess) rc = bfehler(b, "autotune restore failed");
+  if (threads_out) *threads_out = best_t;
+  if (bloecke_out) *bloecke_out = b->n_tab;
+  if (ms_out) *ms_out = best_ms;
+  return ende();
 }
 
 // ---------------------------------------------------------------------------
@@ -1565,6 +2001,146 @@ int gc_zeiten(gc_ctx* c, double* ms_schritt, double* ms_pcg) {
   if (!c) return 1;
   if (ms_schritt) *ms_schritt = c->ms_schritt;
   if (ms_pcg) *ms_pcg = c->ms_pcg;
+  return 0;
+}
+
+int gc_add_sdf_von(gc_ctx* c, const gc_ctx* quelle, int32_t k, const double versatz[3], const double v[3]) {
+  if (!c) return -1;
+  if (!quelle || k < 0 || k >= (int32_t)quelle->koerper.size() || quelle->koerper[(size_t)k].k.art != GC_SDF) {
+    c->err = "gc_add_sdf_von: no such grid body in the source context";
+    return -1;
+  }
+  if (c->koerper.size() >= GC_MAX_KOERPER) {
+    c->err = "too many bodies";
+    return -1;
+  }
+  Koerperhost kh;
+  kh.k = quelle->koerper[(size_t)k].k;  // same device grid; not owned here
+  for (int q = 0; q < 3; ++q) {
+    kh.k.punkt[q] = versatz[q];
+    kh.k.geschwindigkeit[q] = v[q];
+  }
+  c->koerper.push_back(kh);
+  alle_param(c);
+  return (int)c->koerper.size() - 1;
+}
+
+gc_batch* gc_batch_create(gc_ctx* const* ctx, int32_t n, int32_t threads, int32_t bloecke, char* err, int32_t err_len) {
+  auto melde = [&](const std::string& m) {
+    if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", m.c_str());
+  };
+  if (!ctx || n <= 0) {
+    melde("gc_batch_create: no systems");
+    return nullptr;
+  }
+  for (int32_t k = 0; k < n; ++k) {
+    if (!ctx[k] || !ctx[k]->topo || !ctx[k]->hat_param) {
+      melde("gc_batch_create: every system needs topology and parameters");
+      return nullptr;
+    }
+    if (ctx[k]->opt.praezision != ctx[0]->opt.praezision || ctx[k]->opt.geraet != ctx[0]->opt.geraet) {
+      melde("gc_batch_create: systems differ in precision or device");
+      return nullptr;
+    }
+    for (int32_t q = 0; q < k; ++q)
+      if (ctx[q] == ctx[k]) {
+        melde("gc_batch_create: a system is listed twice");
+        return nullptr;
+      }
+  }
+  if (threads <= 0) threads = 1024;
+  if (threads > 1024 || threads < 64 || (threads & (threads - 1))) {
+    melde("threads must be a power of two in [64, 1024]");
+    return nullptr;
+  }
+  gc_batch* b = new gc_batch();
+  b->ctx.assign(ctx, ctx + n);
+  b->praezision = ctx[0]->opt.praezision;
+  b->cus = ctx[0]->cus;
+  b->gewicht.resize((size_t)n);
+  for (int32_t k = 0; k < n; ++k) b->gewicht[(size_t)k] = (double)std::max(1, ctx[k]->n_kacheln);
+  for (int32_t k = 0; k < n; ++k)
+    if (hipStreamSynchronize(ctx[k]->stream) != hipSuccess) {
+      melde("gc_batch_create: a system's stream failed");
+      delete b;
+      return nullptr;
+    }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    melde("hipStreamCreate failed");
+    delete b;
+    return nullptr;
+  }
+  const int n_zweige = std::min<int>(n, 8);
+  for (int q = 0; q < n_zweige; ++q) {
+    hipStream_t z = nullptr;
+    if (hipStreamCreateWithFlags(&z, hipStreamNonBlocking) != hipSuccess) {
+      melde("hipStreamCreate failed");
+      gc_batch_destroy(b);
+      return nullptr;
+    }
+    b->zweige.push_back(z);
+  }
+  for (int q = 0; q < 3 * (n_zweige + 1); ++q) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      melde("hipEventCreate failed");
+      gc_batch_destroy(b);
+      return nullptr;
+    }
+    b->gabel.push_back(e);
+  }
+  int rc = b->praezision == GC_FP32 ? batch_form<float>(b, threads, bloecke) : batch_form<double>(b, threads, bloecke);
+  if (rc) {
+    melde(b->err);
+    gc_batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void gc_batch_destroy(gc_batch* b) {
+  if (!b) return;
+  for (hipStream_t z : b->zweige) (void)hipStreamSynchronize(z);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->gabel) (void)hipEventDestroy(e);
+  if (b->tab) (void)hipFree(b->tab);
+  for (hipStream_t z : b->zweige) (void)hipStreamDestroy(z);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char* gc_batch_last_error(const gc_batch* b) { return b ? b->err.c_str() : "no batch"; }
+
+int gc_batch_step(gc_batch* b, double h, int32_t n) {
+  if (!b) return 1;
+  return b->praezision == GC_FP32 ? batch_schritte<float>(b, h, n) : batch_schritte<double>(b, h, n);
+}
+
+int gc_batch_step_graph(gc_batch* b, double h, int32_t n) {
+  if (!b) return 1;
+  return b->praezision == GC_FP32 ? batch_schritte_graph<float>(b, h, n) : batch_schritte_graph<double>(b, h, n);
+}
+
+int gc_batch_autotune(gc_batch* b, double h, int32_t* threads, int32_t* bloecke, double* ms) {
+  if (!b) return 1;
+  return b->praezision == GC_FP32 ? batch_autotune<float>(b, h, threads, bloecke, ms)
+                                  : batch_autotune<double>(b, h, threads, bloecke, ms);
+}
+
+int gc_batch_bloecke(gc_batch* b, int32_t* bloecke, int32_t* threads) {
+  if (!b) return 1;
+  if (bloecke)
+    for (size_t k = 0; k < b->bloecke.size(); ++k) bloecke[k] = b->bloecke[k];
+  if (threads) *threads = b->threads;
+  return 0;
+}
+
+int gc_batch_zeiten(gc_batch* b, double* ms_schritt, double* ms_pcg) {
+  if (!b) return 1;
+  if (ms_schritt) *ms_schritt = b->ms_schritt;
+  if (ms_pcg) *ms_pcg = b->ms_pcg;
   return 0;
 }
 

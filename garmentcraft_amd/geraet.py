@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
-from typing import Optional
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -56,7 +56,8 @@ class GeraetSzene:
     FP64 reductions)."""
 
     def __init__(self, s: Szene, param: Optional[ImplizitParameter] = None, praezision: str = "fp64",
-                 bloecke: int = 0, threads: int = 0, geraet: int = 0, lds_kb: int = 0):
+                 bloecke: int = 0, threads: int = 0, geraet: int = 0, lds_kb: int = 0,
+                 koerper_quelle: Optional["GeraetSzene"] = None):
         self.lib = L.lib()
         opt = L.Optionen(L.GC_FP32 if praezision == "fp32" else L.GC_FP64, geraet, bloecke, threads, lds_kb)
         err = C.create_string_buffer(512)
@@ -71,8 +72,15 @@ class GeraetSzene:
         self._szene_param = s.param
         self.set_param(self.param)
         self._topologie(s)
-        for kb in s.koerper:
-            self.add_koerper(kb)
+        # grid bodies of koerper_quelle (same index) are shared, not uploaded again
+        self._quelle = koerper_quelle
+        for k, kb in enumerate(s.koerper):
+            if koerper_quelle is not None and isinstance(kb, GitterSdf):
+                i = self.lib.gc_add_sdf_von(self.h, koerper_quelle.h, k, _p(_f64(kb.versatz)), _p(_f64(kb.geschwindigkeit)))
+                if i < 0:
+                    raise L.GcFehler("gc_add_sdf_von: " + self.lib.gc_last_error(self.h).decode())
+            else:
+                self.add_koerper(kb)
         self.set_state(s.x, s.v)
 
     # -- plumbing -----------------------------------------------------------
@@ -296,3 +304,58 @@ class GeraetSzene:
         o = np.zeros((self.n, 3))
         self._ck(self.lib.gc_get_v(self.h, _p(o)), "gc_get_v")
         return o
+
+
+class GeraetStapel:
+    """Many device systems stepped as one (gc_batch_*): every substep builds all
+    of them, solves them in one persistent PCG launch per solve pass (each
+    system with its own group of blocks, reductions and stopping test) and
+    updates all of them. The systems stay usable for reading state and
+    statistics; they must not be stepped on their own while the batch runs."""
+
+    def __init__(self, systeme: Sequence[GeraetSzene], threads: int = 0, bloecke: int = 0):
+        self.lib = L.lib()
+        self.systeme = list(systeme)
+        arr = (C.c_void_p * len(self.systeme))(*[g.h.value for g in self.systeme])
+        err = C.create_string_buffer(512)
+        h = self.lib.gc_batch_create(arr, len(self.systeme), threads, bloecke, err, 512)
+        if not h:
+            raise L.GcFehler("gc_batch_create: " + err.value.decode())
+        self.h = C.c_void_p(h)
+
+    def _ck(self, rc: int, was: str) -> None:
+        if rc != 0:
+            raise L.GcFehler(f"{was}: {self.lib.gc_batch_last_error(self.h).decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None:
+            self.lib.gc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def schritte(self, h: float, n: int = 1) -> None:
+        self._ck(self.lib.gc_batch_step(self.h, h, n), "gc_batch_step")
+
+    def schritte_graph(self, h: float, n: int) -> None:
+        self._ck(self.lib.gc_batch_step_graph(self.h, h, n), "gc_batch_step_graph")
+
+    def autotune(self, h: float) -> dict:
+        t = C.c_int32(); b = C.c_int32(); ms = C.c_double()
+        self._ck(self.lib.gc_batch_autotune(self.h, h, C.byref(t), C.byref(b), C.byref(ms)), "gc_batch_autotune")
+        return {"threads": t.value, "bloecke": b.value, "ms_loesung": ms.value}
+
+    def bloecke(self):
+        nb = np.zeros(len(self.systeme), dtype=np.int32)
+        t = C.c_int32()
+        self._ck(self.lib.gc_batch_bloecke(self.h, _p(nb, L.I32), C.byref(t)), "gc_batch_bloecke")
+        return nb, t.value
+
+    def zeiten(self):
+        a = C.c_double(); b = C.c_double()
+        self._ck(self.lib.gc_batch_zeiten(self.h, C.byref(a), C.byref(b)), "gc_batch_zeiten")
+        return a.value, b.value

@@ -442,3 +442,65 @@ def drape(cfg: Optional[DrapeKonfig] = None, sdf: Optional[GitterSdf] = None, mi
         "min_winkel_grad": float(mw.min()), "anteil_unter_25_grad": float(np.mean(mw < 25.0)),
     }
     return s, bericht
+
+
+@dataclass
+class StapelKonfig:
+    """Batched variants: independent garments of uneven sizes on one body."""
+    anzahl: int = 64
+    v_min: int = 2000
+    v_max: int = 60000
+    seed: int = 20261204
+    kante_mm: float = 10.0
+
+
+# points per mm^2 of the drape's panels at 10 mm edges (40,758 points on 28
+# cells of 352 mm)
+_DRAPE_DICHTE = 40758.0 / (28 * 352.0 * 352.0)
+
+
+def stapel_variante(cfg: StapelKonfig, k: int) -> Tuple[DrapeKonfig, dict]:
+    """Instance k of the batch: the drape's 7 x 4 panel grid scaled to a point
+    count log-uniform in [v_min, v_max], latex or woven at random, stiffnesses
+    (stretch, shear, bending) scaled by one factor log-uniform over one decade,
+    friction uniform in [0.1, 0.5]. Deterministic in (seed, k)."""
+    rng = np.random.default_rng([cfg.seed, k])
+    v_ziel = math.exp(rng.uniform(math.log(cfg.v_min), math.log(cfg.v_max)))
+    zelle = math.sqrt(v_ziel / (28.0 * _DRAPE_DICHTE)) * cfg.kante_mm / 10.0
+    faktor = 10.0 ** rng.uniform(-0.5, 0.5)
+    mu = rng.uniform(0.1, 0.5)
+    if rng.random() < 0.5:
+        mat = latex(1.0 * faktor, 0.5e-3, 950.0, 1e-6 * faktor, reibung=mu)
+        art = "latex"
+    else:
+        mat = gewebe(5.0 * faktor, 3.0 * faktor, 0.5 * faktor, 0.5e-3, 0.20, 1.2e-6 * faktor, 0.8e-6 * faktor,
+                     0.9e-6 * faktor, mu)
+        art = "gewebe"
+    d = DrapeKonfig(seed=cfg.seed + 7919 * (k + 1), zelle_mm=zelle, kante_mm=cfg.kante_mm,
+                    jitter_mm=30.0 * zelle / 352.0, versatz_mm=(0.0, -0.5 * zelle), material=mat)
+    return d, {"v_ziel": v_ziel, "material": art, "faktor": faktor, "reibung": mu, "zelle_mm": zelle}
+
+
+def _stapel_eins(args):
+    cfg, k = args
+    d, info = stapel_variante(cfg, k)
+    s, bericht = drape(d, mit_koerper=False)
+    bericht.update(info)
+    return s, bericht
+
+
+def stapel(cfg: Optional[StapelKonfig] = None, sdf: Optional[GitterSdf] = None, prozesse: int = 0) -> List[Tuple[Szene, dict]]:
+    """The batch's scenes, each on the same body ``sdf`` (built once here if
+    not given). prozesse > 1 builds the meshes in that many worker processes."""
+    cfg = cfg or StapelKonfig()
+    sdf = sdf if sdf is not None else koerper_sdf()
+    auftraege = [(cfg, k) for k in range(cfg.anzahl)]
+    if prozesse > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=prozesse) as ex:
+            out = list(ex.map(_stapel_eins, auftraege))
+    else:
+        out = [_stapel_eins(a) for a in auftraege]
+    for s, _ in out:
+        s.koerper = [sdf]
+    return out

@@ -156,6 +156,11 @@ int gc_add_sdf(gc_ctx* ctx, const float* werte, const int32_t dims[3], const dou
 /* Move a body to an absolute offset (cylinder: axis point; SDF: versatz), as
  * `Koerper::verschoben` does per substep (koerper.rs:122-133). */
 int gc_set_koerper_lage(gc_ctx* ctx, int32_t koerper, const double lage[3]);
+/* A body that uses the device grid of body k of another context (a batch of
+ * garments on one body keeps one copy of its SDF). The source context must
+ * outlive this one. No counterpart in the reference, which has one scene per
+ * solver. Returns the body index or -1. */
+int gc_add_sdf_von(gc_ctx* ctx, const gc_ctx* quelle, int32_t k, const double versatz[3], const double geschwindigkeit[3]);
 
 /* State: positions and velocities (Szene::x, Szene::v). */
 int gc_set_state(gc_ctx* ctx, const double* x, const double* v);
@@ -230,6 +235,38 @@ int gc_read_statistik(gc_ctx* ctx, gc_statistik* out, int32_t reset);
 /* Per-kernel device timings of the last gc_step/gc_step_graph call (HIP events
  * on the stream the kernels run on): total ms and PCG-kernel ms. */
 int gc_zeiten(gc_ctx* ctx, double* ms_schritt, double* ms_pcg);
+
+/* Batches. The reference steps one `ImplizitCg` per scene (implizit.rs:899-1038);
+ * a batch steps many independent systems (contexts with topology and
+ * parameters, same precision and device) as one: each substep runs every
+ * system's build, then ONE persistent PCG launch in which each system owns a
+ * group of blocks with its own reductions, iterations and stopping test, the
+ * release passes, a second such launch, and every system's update. Per system
+ * the substep is `ImplizitCg::schritt`, as gc_step computes it; only the
+ * grouping of the reductions' block partials follows the batch's launch shape.
+ * threads: power of two in [64, 1024] (0: 1024); bloecke: total blocks (0: all
+ * co-resident ones). The contexts must outlive the batch and are not to be
+ * stepped on their own while a batch call runs. NULL on failure with the
+ * reason in err. */
+typedef struct gc_batch gc_batch;
+gc_batch* gc_batch_create(gc_ctx* const* ctx, int32_t n, int32_t threads, int32_t bloecke, char* err, int32_t err_len);
+void gc_batch_destroy(gc_batch* b);
+const char* gc_batch_last_error(const gc_batch* b);
+/* n substeps of length h for every system of the batch, eager launches. */
+int gc_batch_step(gc_batch* b, double h, int32_t n);
+/* The same batched substep captured once into a HIP graph and replayed n times. */
+int gc_batch_step_graph(gc_batch* b, double h, int32_t n);
+/* Chooses the launch shape (threads per block; total blocks; blocks per system
+ * in proportion to its tiles or to tiles x measured iterations) by timing the
+ * batched solve of the current substep from one warm start per candidate; the
+ * state is left as it was. Returns the chosen threads, total blocks and the
+ * solve time in ms. */
+int gc_batch_autotune(gc_batch* b, double h, int32_t* threads, int32_t* bloecke, double* ms);
+/* Blocks per system (n values) and threads per block of the current shape. */
+int gc_batch_bloecke(gc_batch* b, int32_t* bloecke, int32_t* threads);
+/* Device time of the last gc_batch_step(_graph) call: total ms and the batched
+ * PCG launches' ms (NaN for graph replays). */
+int gc_batch_zeiten(gc_batch* b, double* ms_schritt, double* ms_pcg);
 
 #ifdef __cplusplus
 }
