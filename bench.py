@@ -60,9 +60,15 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bloecke", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--lds-kb", type=int, default=0, help="LDS a PCG tile may fill (KiB); 0 = the library's choice")
     ap.add_argument("--ohne-selbstkollision", action="store_true", help="self-collision off (the plain drape)")
-    ap.add_argument("--workload", default="drape40k", choices=["drape40k", "stapel64"],
-                    help="drape40k: the 40k-point drape; stapel64: 64 garments of 2k-60k points in one batch")
+    ap.add_argument("--workload", default="drape40k", choices=["drape40k", "stapel64", "adjoint20k", "gross250k"],
+                    help="drape40k: the 40k-point drape; stapel64: 64 garments of 2k-60k points in one batch; "
+                         "adjoint20k: gradient of a 60-step 20k-point drape rollout w.r.t. per-panel stiffnesses "
+                         "and friction; gross250k: the 250k-point drape, FP32 bulk against all-FP64")
+    ap.add_argument("--vorlauf", type=int, default=120, help="adjoint20k: settling steps before the rollout starts")
+    ap.add_argument("--fd", type=int, default=0, help="adjoint20k: central differences of the FP64 forward for this "
+                                                      "many parameters (largest gradients first, plus friction)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -72,6 +78,10 @@ def main() -> int:
 
     if args.workload == "stapel64":
         return stapel_lauf(args)
+    if args.workload == "adjoint20k":
+        return adjoint_lauf(args)
+    if args.workload == "gross250k":
+        return gross_lauf(args)
 
     from garmentcraft_amd import szenen
     from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
@@ -83,7 +93,8 @@ def main() -> int:
     dt = s.param.bild_dt
     selbst = not args.ohne_selbstkollision
     param = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0, selbstkollision=selbst)
-    g = GeraetSzene(s, param, praezision=args.praezision, bloecke=args.bloecke, threads=args.threads)
+    g = GeraetSzene(s, param, praezision=args.praezision, bloecke=args.bloecke, threads=args.threads,
+                    lds_kb=args.lds_kb)
     print(f"[bench] device: {g.name()}", file=sys.stderr, flush=True)
 
     step = g.schritte_graph if args.graph else g.schritte
@@ -267,6 +278,196 @@ def stapel_lauf(args) -> int:
                    "cg_iterationen_je_schritt": round(iters / steps, 1), "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def adjoint_lauf(args) -> int:
+    """The adjoint rollout: a 20k-point drape of the drape's 28 panels (cells
+    scaled to 20k points), 60 steps of 1/120 s from the flat start, checkpoint
+    every 10 steps, FP32 bulk with FP64 reductions, rtol 1e-6, self-collision
+    off (the adjoint does not differentiate it). Target: the final positions
+    with per-panel stretch/shear/bend factors and the friction coefficient
+    scaled by U(0.7, 1.3). Loss: mean squared point distance at the final step.
+    value = adjoint rollout steps/s: steps / (forward with checkpoints + reverse
+    pass, which re-runs each segment and steps the adjoint back)."""
+    from garmentcraft_amd import szenen
+    from garmentcraft_amd.adjungiert import PanelMaterial, Rollout, verlust, verlust_gradient
+    from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
+
+    schritte = 60 if args.steps == 300 else args.steps
+    alle = 10
+    h = 1.0 / 120.0
+    t0 = time.time()
+    sdf = szenen.koerper_sdf()
+    zelle = 352.0 * (20000.0 / 40758.0) ** 0.5
+    cfg = szenen.DrapeKonfig(zelle_mm=zelle, jitter_mm=30.0 * zelle / 352.0, versatz_mm=(0.0, -0.5 * zelle))
+    s, bericht = szenen.drape(cfg, sdf=sdf)
+    print(f"[bench] adjoint scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    mat = PanelMaterial(s)
+    rng = np.random.default_rng(20261205)
+    th0 = mat.theta0()
+    th_ziel = th0 * rng.uniform(0.7, 1.3, mat.n_param)
+
+    # the rollout starts from the drape settled on the body (the FP64 form,
+    # args.vorlauf steps of the flat drop): through the drop's impact the
+    # rollout is chaotic (FP32 and FP64 forms part by centimetres) and no
+    # gradient describes it
+    x_s, v_s, dv_s = s.x, s.v, None
+    if args.vorlauf > 0:
+        p = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-8, naht_k=500.0, selbstkollision=False)
+        gv = GeraetSzene(s, p, praezision="fp64")
+        gv.schritte(h, args.vorlauf)
+        x_s, v_s = gv.lage()
+        dv_s = None
+        gv.close()
+        print(f"[bench] settled {args.vorlauf} steps", file=sys.stderr, flush=True)
+
+    def kontext(praezision, tol):
+        p = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=tol, naht_k=500.0, selbstkollision=False)
+        g = GeraetSzene(s, p, praezision=praezision)
+        return g, Rollout(g, s, mat, h, schritte, alle=alle, x0=x_s, v0=v_s, dv_vorher=dv_s)
+
+    g32, r32 = kontext(args.praezision, 1e-6)
+    ziel = r32.vorwaerts(th_ziel)
+    print(f"[bench] target rollout done", file=sys.stderr, flush=True)
+    r32.vorwaerts(th0)  # warm-up of both passes
+    r32.rueckwaerts(verlust_gradient(r32.g.lage()[0], ziel))
+    g32.statistik(reset=True)
+    t_a = time.perf_counter()
+    x = r32.vorwaerts(th0)
+    t_b = time.perf_counter()
+    d32 = r32.rueckwaerts(verlust_gradient(x, ziel))
+    t_c = time.perf_counter()
+    st = g32.statistik()
+    ast = r32.statistik
+    L0 = verlust(x, ziel)
+    # plain steps of the same rollout, no checkpoints
+    g32.laden(-1)
+    t_p = time.perf_counter()
+    g32.schritte(h, schritte)
+    t_q = time.perf_counter()
+    plain = schritte / (t_q - t_p)
+    print(f"[bench] forward {1e3 * (t_b - t_a):.1f} ms, reverse {1e3 * (t_c - t_b):.1f} ms, plain {1e3 * (t_q - t_p):.1f} ms; "
+          f"loss {L0:.6e}; adjoint solves {ast}", file=sys.stderr, flush=True)
+
+    # FP64 reference gradient: the all-FP64 adjoint on the same rollout
+    g64, r64 = kontext("fp64", 1e-6)
+    x64 = r64.vorwaerts(th0)
+    d64 = r64.rueckwaerts(verlust_gradient(x64, ziel))
+    rel32 = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+    print(f"[bench] fp32 vs fp64 adjoint gradient: relative {rel32:.3e}; positions {np.max(np.abs(x - x64)):.3e} m",
+          file=sys.stderr, flush=True)
+    fd = None
+    if args.fd > 0:
+        # central differences (relative step 1e-4) of the FP64 forward at
+        # rtol 1e-10 against the FP64 adjoint at the same tolerance
+        g_fd, r_fd = kontext("fp64", 1e-10)
+        xf = r_fd.vorwaerts(th0)
+        dfd_adj = r_fd.rueckwaerts(verlust_gradient(xf, ziel))
+        wahl = list(np.argsort(-np.abs(dfd_adj[:-1]))[: max(0, args.fd - 1)]) + [mat.n_param - 1]
+        zeilen = []
+        for k in wahl:
+            hk = 1e-4 * abs(th0[k])
+            tp, tm = th0.copy(), th0.copy()
+            tp[k] += hk
+            tm[k] -= hk
+            fdk = (verlust(r_fd.vorwaerts(tp), ziel) - verlust(r_fd.vorwaerts(tm), ziel)) / (2 * hk)
+            zeilen.append({"param": int(k), "fd": fdk, "adjoint_fp64": float(dfd_adj[k]), "adjoint_fp32": float(d32[k]),
+                           "rel": abs(dfd_adj[k] - fdk) / max(abs(fdk), 1e-300)})
+            print(f"[bench] fd theta[{k}]: {fdk:.6e} adjoint {dfd_adj[k]:.6e} (fp32 at 1e-6: {d32[k]:.6e})",
+                  file=sys.stderr, flush=True)
+        v_fd = np.array([z["fd"] for z in zeilen])
+        v_ad = np.array([z["adjoint_fp64"] for z in zeilen])
+        fd = {"parameter": zeilen, "rel_norm": float(np.linalg.norm(v_ad - v_fd) / np.linalg.norm(v_fd)),
+              "rtol": 1e-10, "schritt": "1e-4 relative"}
+    value = schritte / (t_c - t_a)
+    out = {
+        "metric": "implicit steps/s @40k verts, PCG rtol 1e-6; batched/adjoint vs plain GPU vs CPU",
+        "value": round(value, 3), "unit": "adjoint rollout steps/s (forward with checkpoints + reverse)",
+        "n_gpus": 1, "steps": schritte, "warmup": 1, "ms_per_step": round(1e3 * (t_c - t_a) / schritte, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if args.praezision == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"adjoint20k: 28-panel woven drape scaled to 20k points, settled {args.vorlauf} steps, "
+                               f"then {schritte} steps of 1/120 s, checkpoint every 10, rtol 1e-6, self-collision off; "
+                               "gradient of the final mean squared point distance w.r.t. 28x3 per-panel stiffness "
+                               "factors and friction",
+                   "punkte": s.n, "parameter": mat.n_param, "verlust": L0,
+                   "plain_steps_s": round(plain, 3), "adjoint_gegen_plain": round(value / plain, 3),
+                   "ms_vorwaerts": round(1e3 * (t_b - t_a), 1), "ms_rueckwaerts": round(1e3 * (t_c - t_b), 1),
+                   "cg_iterationen_je_schritt": round(st["cg_iterationen"] / schritte, 1),
+                   "adjoint_cg_iterationen_je_schritt": round(ast["cg_iterationen"] / schritte, 1),
+                   "adjoint_abweichungen": ast["abweichungen"], "adjoint_eintritte": ast["eintritte"],
+                   "fp32_gegen_fp64_gradient": rel32, "fd": fd, "parallelism": "1 GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def gross_lauf(args) -> int:
+    """The 250k-point drape: the 28 panels meshed at 4 mm (~250k points,
+    ~490k triangles), body SDF 384^3 at 2 mm, self-collision on, friction 0.3,
+    100 steps of 1/240 s at rtol 1e-7, stepped in lockstep in the FP32-bulk
+    form and the all-FP64 form; per step the FP32 form's position error
+    against the FP64 form (which equals the FP64 oracle to 1e-9 m on the
+    drape tests)."""
+    from garmentcraft_amd import szenen
+    from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
+
+    schritte = 100 if args.steps == 300 else args.steps
+    h = 1.0 / 240.0
+    t0 = time.time()
+    sdf = szenen.koerper_sdf(aufloesung=384, spacing=0.002)
+    s, bericht = szenen.drape(szenen.DrapeKonfig(kante_mm=4.0), sdf=sdf)
+    print(f"[bench] scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    selbst = not args.ohne_selbstkollision
+    p = ImplizitParameter(schritte_je_bild=1, cg_max=20000, cg_toleranz=1e-7, naht_k=500.0, selbstkollision=selbst)
+    g = {pr: GeraetSzene(s, p, praezision=pr) for pr in ("fp32", "fp64")}
+    ms = {"fp32": 0.0, "fp64": 0.0}
+    ms_pcg = {"fp32": 0.0, "fp64": 0.0}
+    fehler_max, fehler_mittel = [], []
+    for k in range(schritte):
+        for pr in ("fp32", "fp64"):
+            t_a = time.perf_counter()
+            g[pr].schritte(h, 1)
+            ms[pr] += 1e3 * (time.perf_counter() - t_a)
+            ms_pcg[pr] += g[pr].zeiten()[1]
+        d = np.linalg.norm(g["fp32"].lage()[0] - g["fp64"].lage()[0], axis=1)
+        fehler_max.append(float(d.max()))
+        fehler_mittel.append(float(d.mean()))
+        if k % 10 == 9 or k == schritte - 1:
+            print(f"[bench] step {k + 1}: fp32 {ms['fp32']:.0f} ms, fp64 {ms['fp64']:.0f} ms so far; position error "
+                  f"max {d.max():.3e} m mean {d.mean():.3e} m", file=sys.stderr, flush=True)
+    st = {pr: g[pr].statistik() for pr in g}
+    roof = {}
+    for pr in g:
+        b = pcg_bytes(s, pr)
+        algo = st[pr]["cg_iterationen"] * b["iteration"] + st[pr]["loesungen"] * b["start"]
+        ach = algo / (ms_pcg[pr] * 1e-3) / 1e9
+        roof[pr] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_pcg",
+                    "bytes_per_iteration": b["iteration"],
+                    "us_per_iteration": round(ms_pcg[pr] * 1e3 / max(st[pr]["cg_iterationen"], 1), 3)}
+    value = schritte / (ms["fp32"] * 1e-3)
+    out = {
+        "metric": "implicit steps/s @40k verts, PCG rtol 1e-6; batched/adjoint vs plain GPU vs CPU",
+        "value": round(value, 3), "unit": "steps/s (FP32 bulk, 250k points)", "n_gpus": 1, "steps": schritte,
+        "warmup": 0, "ms_per_step": round(ms["fp32"] / schritte, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "gross250k: 28 woven panels at 4 mm edges, SDF 384^3 at 2 mm, friction 0.3, "
+                               + ("self-collision on, " if selbst else "self-collision off, ") + "dt 1/240, rtol 1e-7",
+                   "punkte": s.n, "dreiecke": int(s.dreiecke.shape[0]),
+                   "fp64_steps_s": round(schritte / (ms["fp64"] * 1e-3), 3),
+                   "fp32_gegen_fp64_schneller": round(ms["fp64"] / ms["fp32"], 3),
+                   "positionsfehler_max_m": max(fehler_max), "positionsfehler_ende_max_m": fehler_max[-1],
+                   "positionsfehler_ende_mittel_m": fehler_mittel[-1],
+                   "positionsfehler_je_schritt_max_m": [float(f"{e:.3e}") for e in fehler_max],
+                   "cg_iterationen_je_schritt": {pr: round(st[pr]["cg_iterationen"] / schritte, 1) for pr in g},
+                   "nicht_konvergiert": {pr: st[pr]["nicht_konvergiert"] for pr in g},
+                   "rundungsgrenze_fp32": st["fp32"]["cg_rundungsgrenze"], "parallelism": "1 GPU"},
+        "roofline": roof["fp32"], "roofline_fp64": roof["fp64"], "cpu_baseline": None,
     }
     print(json.dumps(out))
     return 0

@@ -66,6 +66,11 @@ class Statistik_c(C.Structure):
     ]
 
 
+class AdjointStat_c(C.Structure):
+    _fields_ = [("schritte", C.c_int64), ("cg_iterationen", C.c_int64), ("nicht_konvergiert", C.c_int64),
+                ("abweichungen", C.c_int64), ("eintritte", C.c_int64), ("rest_max", C.c_double)]
+
+
 # (name, restype, argtypes) for every entry point of include/gcsim.h
 SIGNATUREN = [
     ("gc_create", C.c_void_p, [C.POINTER(Optionen), C.c_char_p, C.c_int32]),
@@ -112,6 +117,14 @@ SIGNATUREN = [
                                     C.POINTER(C.c_double)]),
     ("gc_batch_bloecke", C.c_int, [C.c_void_p, I32, C.POINTER(C.c_int32)]),
     ("gc_batch_zeiten", C.c_int, [C.c_void_p, D, D]),
+    ("gc_adjoint_step", C.c_int, [C.c_void_p, C.c_double]),
+    ("gc_adjoint_set", C.c_int, [C.c_void_p, D, D]),
+    ("gc_adjoint_get", C.c_int, [C.c_void_p, D, D]),
+    ("gc_adjoint_gradient", C.c_int, [C.c_void_p, D, D, D, D, D, C.c_int32]),
+    ("gc_adjoint_statistik", C.c_int, [C.c_void_p, C.POINTER(AdjointStat_c), C.c_int32]),
+    ("gc_set_steifigkeit", C.c_int, [C.c_void_p, D, D, D, D]),
+    ("gc_zustand_sichern", C.c_int, [C.c_void_p, C.c_int32]),
+    ("gc_zustand_laden", C.c_int, [C.c_void_p, C.c_int32]),
 ]
 
 _lib = None

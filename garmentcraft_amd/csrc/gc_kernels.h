@@ -75,6 +75,9 @@ struct SysDev {
   V3<T>* p[2];
   V3<T>* ap;
   V3<double>* dv_vorher;
+  // normal target of held points for the solve; null: vn_soll - v.n of the
+  // step (implizit.rs:375-384); set: the adjoint solve's own targets
+  const double* soll_direkt;
   // tiles of the PCG operator: row ranges whose CSR entries (and the points
   // their constraints touch) fit in LDS
   int32_t n_kacheln;
@@ -646,9 +649,9 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       const uint8_t fl = s.fflag[i];
       if (fl == 2) {
         s.dv[i] = zero3<T>();
-      } else if (fl == 1) {
+      } else if (fl != 0) {  // 1; 3 only in the adjoint's solve (see k_freigabe)
         const V3<double> n = s.fnorm64[i];
-        const double soll = s.vn_soll[i] - dot(s.v[i], n);
+        const double soll = s.soll_direkt ? s.soll_direkt[i] : s.vn_soll[i] - dot(s.v[i], n);
         const V3<T> dvf = filter_fl(fl, s.fnorm[i], s.dv[i]);
         s.dv[i] = dvf + s.fnorm[i] * T(soll);
       }
@@ -822,7 +825,16 @@ __global__ void k_freigabe(const SysDev<T>* __restrict__ sp, GridState* gs, int 
     const V3<double> rest = cast3<double>(ap) - cast3<double>(s.b[i]);
     const double fn = dot(rest, s.fnorm64[i]) / s.h;
     if (fn < 0.0) {
-      s.fflag[i] = 0;
+      // released; after the second solve (no third follows) the point keeps
+      // flag 3 = "held in the last solve, free in the update", with the
+      // normal force of that solve for the step's adjoint. Every kernel of
+      // the step treats 3 as free (the PCG never runs after this pass).
+      if (durchgang == 1) {
+        s.fflag[i] = 3;
+        s.fn_kraft[i] = fn;
+      } else {
+        s.fflag[i] = 0;
+      }
       ++frei;
       ++weg;
     } else {

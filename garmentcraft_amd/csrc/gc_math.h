@@ -332,60 +332,66 @@ struct Koerper {
 };
 #define GC_MAX_KOERPER 4
 
+GC_HD double wert(double a) { return a; }
+
 // Trilinear sample of the grid SDF and its gradient (per unit length).
-GC_HD bool sdf_sample(const Koerper& k, const double q[3], double* d, double grad[3]) {
-  double t[3];
+// Templated on the scalar S of the query point (double in the step; a dual
+// number in the adjoint, where the cell is chosen from the value).
+template <typename S>
+GC_HD bool sdf_sample(const Koerper& k, const S q[3], S* d, S grad[3]) {
+  S t[3];
   int i[3];
   for (int a = 0; a < 3; ++a) {
-    double u = (q[a] - k.origin[a]) / k.spacing;
-    if (!(u >= 0.0) || !(u <= (double)(k.dims[a] - 1))) return false;
-    int c = (int)floor(u);
+    S u = (q[a] - S(k.origin[a])) / S(k.spacing);
+    if (!(wert(u) >= 0.0) || !(wert(u) <= (double)(k.dims[a] - 1))) return false;
+    int c = (int)floor(wert(u));
     if (c > k.dims[a] - 2) c = k.dims[a] - 2;
     i[a] = c;
-    t[a] = u - (double)c;
+    t[a] = u - S((double)c);
   }
   const size_t sx = 1, sy = (size_t)k.dims[0], sz = (size_t)k.dims[0] * (size_t)k.dims[1];
   const float* w0 = k.werte + (size_t)i[2] * sz + (size_t)i[1] * sy + (size_t)i[0];
-  double v000 = w0[0], v100 = w0[sx], v010 = w0[sy], v110 = w0[sy + sx];
-  double v001 = w0[sz], v101 = w0[sz + sx], v011 = w0[sz + sy], v111 = w0[sz + sy + sx];
-  double tx = t[0], ty = t[1], tz = t[2];
-  double x00 = v000 + tx * (v100 - v000), x10 = v010 + tx * (v110 - v010);
-  double x01 = v001 + tx * (v101 - v001), x11 = v011 + tx * (v111 - v011);
-  double y0 = x00 + ty * (x10 - x00), y1 = x01 + ty * (x11 - x01);
+  S v000 = S((double)w0[0]), v100 = S((double)w0[sx]), v010 = S((double)w0[sy]), v110 = S((double)w0[sy + sx]);
+  S v001 = S((double)w0[sz]), v101 = S((double)w0[sz + sx]), v011 = S((double)w0[sz + sy]), v111 = S((double)w0[sz + sy + sx]);
+  S tx = t[0], ty = t[1], tz = t[2];
+  S x00 = v000 + tx * (v100 - v000), x10 = v010 + tx * (v110 - v010);
+  S x01 = v001 + tx * (v101 - v001), x11 = v011 + tx * (v111 - v011);
+  S y0 = x00 + ty * (x10 - x00), y1 = x01 + ty * (x11 - x01);
   *d = y0 + tz * (y1 - y0);
-  double gx0 = (v100 - v000) + ty * ((v110 - v010) - (v100 - v000));
-  double gx1 = (v101 - v001) + ty * ((v111 - v011) - (v101 - v001));
-  grad[0] = (gx0 + tz * (gx1 - gx0)) / k.spacing;
-  double gy0 = x10 - x00, gy1 = x11 - x01;
-  grad[1] = (gy0 + tz * (gy1 - gy0)) / k.spacing;
-  grad[2] = (y1 - y0) / k.spacing;
+  S gx0 = (v100 - v000) + ty * ((v110 - v010) - (v100 - v000));
+  S gx1 = (v101 - v001) + ty * ((v111 - v011) - (v101 - v001));
+  grad[0] = (gx0 + tz * (gx1 - gx0)) / S(k.spacing);
+  S gy0 = x10 - x00, gy1 = x11 - x01;
+  grad[1] = (gy0 + tz * (gy1 - gy0)) / S(k.spacing);
+  grad[2] = (y1 - y0) / S(k.spacing);
   return true;
 }
 
 // koerper.rs:69-91 (cylinder) and the same query on the grid SDF.
-GC_HD bool kontakt(const Koerper& k, V3<double> x, double abstand, V3<double>* normale, double* tiefe, V3<double>* vk) {
+template <typename S>
+GC_HD bool kontakt(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* tiefe, V3<double>* vk) {
   *vk = V3<double>{k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
   if (k.art == GC_ZYLINDER) {
-    V3<double> ach = {k.achse[0], k.achse[1], k.achse[2]};
-    V3<double> a = ach * (1.0 / norm(ach));
-    V3<double> d = x - V3<double>{k.punkt[0], k.punkt[1], k.punkt[2]};
-    V3<double> quer = d - a * dot(d, a);
-    double r = norm(quer);
-    double t = k.radius + abstand - r;
-    if (t <= 0.0 || r < 1e-12) return false;
-    *normale = quer * (1.0 / r);
+    V3<S> ach = {S(k.achse[0]), S(k.achse[1]), S(k.achse[2])};
+    V3<S> a = ach * (S(1.0) / norm(ach));
+    V3<S> d = x - V3<S>{S(k.punkt[0]), S(k.punkt[1]), S(k.punkt[2])};
+    V3<S> quer = d - a * dot(d, a);
+    S r = norm(quer);
+    S t = S(k.radius) + abstand - r;
+    if (wert(t) <= 0.0 || wert(r) < 1e-12) return false;
+    *normale = quer * (S(1.0) / r);
     *tiefe = t;
     return true;
   }
-  double q[3] = {x.x - k.punkt[0], x.y - k.punkt[1], x.z - k.punkt[2]};
-  double dd, g[3];
+  S q[3] = {x.x - S(k.punkt[0]), x.y - S(k.punkt[1]), x.z - S(k.punkt[2])};
+  S dd, g[3];
   if (!sdf_sample(k, q, &dd, g)) return false;
-  double t = abstand - dd;
-  if (t <= 0.0) return false;
-  V3<double> gv = {g[0], g[1], g[2]};
-  double lg = norm(gv);
-  if (!(lg > 1e-12)) return false;
-  *normale = gv * (1.0 / lg);
+  S t = abstand - dd;
+  if (wert(t) <= 0.0) return false;
+  V3<S> gv = {g[0], g[1], g[2]};
+  S lg = norm(gv);
+  if (!(wert(lg) > 1e-12)) return false;
+  *normale = gv * (S(1.0) / lg);
   *tiefe = t;
   return true;
 }
@@ -396,7 +402,9 @@ GC_HD bool kontakt(const Koerper& k, V3<double> x, double abstand, V3<double>* n
 // outside -> inside of the sampled distance at half-spacing samples along the
 // path, refined by 60 bisections; the normal is the normalised gradient there.
 // A path that starts inside does not enter; a cylinder has none (koerper.rs:101).
-GC_HD bool durchstoss(const Koerper& k, V3<double> a, V3<double> b, V3<double>* p, V3<double>* n, V3<double>* vk) {
+// t_hit: the entry's parameter along the path (p = a + (b - a) t_hit).
+GC_HD bool durchstoss(const Koerper& k, V3<double> a, V3<double> b, V3<double>* p, V3<double>* n, V3<double>* vk,
+                      double* t_hit = nullptr) {
   if (k.art == GC_ZYLINDER) return false;
   const V3<double> d = b - a;
   if (dot(d, d) < 1e-24) return false;
@@ -443,6 +451,7 @@ GC_HD bool durchstoss(const Koerper& k, V3<double> a, V3<double> b, V3<double>* 
       else hi = m;
     }
     *p = a + d * hi;
+    if (t_hit) *t_hit = hi;
     const double q[3] = {p->x - k.punkt[0], p->y - k.punkt[1], p->z - k.punkt[2]};
     double dd, g[3];
     sdf_sample(k, q, &dd, g);

@@ -6,6 +6,7 @@
 //   reset, build edges / bending / membranes, contacts, per-point gather,
 //   warm start, PCG, release, PCG (only after a release), release, update.
 #include "../../include/gcsim.h"
+#include "gc_adjoint.h"
 #include "gc_selbst.h"
 
 #include <hip/hip_runtime.h>
@@ -93,6 +94,26 @@ struct gc_ctx {
   uint32_t* sk_fe_tq = nullptr;              // per spring entry: its point on side q
   // graph of one substep
   hipGraphExec_t graph = nullptr;
+  // adjoint of the step (allocated on first use): FP64 buffers, the adjoint
+  // right-hand side and solution in the bulk type, a copy of the system
+  // description whose b, dv and normal targets are the adjoint's, and a grid
+  // state of its own (the step's statistics stay the step's)
+  bool adj_bereit = false;
+  AdjDev adj{};
+  void* adj_b = nullptr;
+  void* adj_mu = nullptr;
+  void* adj_sys_dev = nullptr;
+  std::vector<unsigned char> adj_sys_host;
+  GridState* gs_adj = nullptr;
+  long long adj_schritte = 0;
+  // saved states (x, v, warm start) by slot number, for checkpointed rollouts
+  struct Zustand {
+    V3<double>* x = nullptr;
+    V3<double>* v = nullptr;
+    V3<double>* dvv = nullptr;
+    bool hat_vorher = false;
+  };
+  std::map<int, Zustand> zustaende;
 };
 
 #define HIPCK(ctx, call)                                                                      \
@@ -1439,6 +1460,196 @@ static int records_lesen(gc_ctx* c, double* richtung, double* quer, double* grad
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// adjoint of the step (reverse mode over one substep, gc_adjoint.h)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int adj_vorbereiten(gc_ctx* c) {
+  if (c->adj_bereit) return 0;
+  const SysDev<T>& s = sys<T>(c);
+  const size_t n = (size_t)c->n;
+  AdjDev& a = c->adj;
+  a.gx = dalloc<V3<double>>(c, n);
+  a.gv = dalloc<V3<double>>(c, n);
+  a.x0 = dalloc<V3<double>>(c, n);
+  a.v0 = dalloc<V3<double>>(c, n);
+  a.gx_p = dalloc<V3<double>>(c, n);
+  a.gv_p = dalloc<V3<double>>(c, n);
+  a.gdv = dalloc<V3<double>>(c, n);
+  a.ziel = dalloc<double>(c, n);
+  a.nu = dalloc<double>(c, n);
+  a.e_kante = dalloc<V3<double>>(c, 2 * (size_t)s.nk);
+  a.e_biegung = dalloc<V3<double>>(c, 4 * (size_t)s.nb);
+  a.e_membran = dalloc<V3<double>>(c, 3 * (size_t)s.nm);
+  a.d_kante = dalloc<double>(c, s.nk);
+  a.d_biegung = dalloc<double>(c, s.nb);
+  a.d_membran = dalloc<double>(c, 3 * (size_t)s.nm);
+  a.d_reibung = dalloc<double>(c, n);
+  a.zaehler = dalloc<unsigned long long>(c, 2);
+  c->adj_b = dalloc<V3<T>>(c, n);
+  c->adj_mu = dalloc<V3<T>>(c, n);
+  c->adj_sys_dev = dalloc<SysDev<T>>(c, 1);
+  c->gs_adj = dalloc<GridState>(c, 1);
+  if (!a.gx || !a.gv || !a.x0 || !a.v0 || !a.gx_p || !a.gv_p || !a.gdv || !a.ziel || !a.nu || !a.e_kante || !a.e_biegung ||
+      !a.e_membran || !a.d_kante || !a.d_biegung || !a.d_membran || !a.d_reibung || !a.zaehler || !c->adj_b || !c->adj_mu ||
+      !c->adj_sys_dev || !c->gs_adj)
+    return fehler(c, "hipMalloc failed (adjoint)");
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->adj_bereit = true;
+  return 0;
+}
+
+// One step from the context's state (as gc_step(h, 1) computes it), then its
+// reverse: the adjoint seed (dL/dx, dL/dv of the state after the step)
+// becomes dL/dx, dL/dv of the state before it, and the step's parameter
+// gradients are added to the accumulators.
+template <typename T>
+static int adj_schritt(gc_ctx* c, double h) {
+  if (!c->topo || !c->hat_param) return fehler(c, "gc_adjoint_step: topology and parameters must be set first");
+  if (!(h > 0.0)) return fehler(c, "h must be > 0");
+  if (selbst_an(c))
+    return fehler(c, "the adjoint does not differentiate cloth self-collision: switch it off (gc_param.selbstkollision = 0)");
+  if (c->param.normalkraft_aus_last)
+    return fehler(c, "the adjoint takes the normal force from the residual (gc_param.normalkraft_aus_last = 0)");
+  if (adj_vorbereiten<T>(c)) return 1;
+  SysDev<T>& s = sys<T>(c);
+  s.h = h;
+  s.h2 = h * h;
+  if (sys_hoch<T>(c)) return 1;
+  AdjDev& a = c->adj;
+  hipStream_t st = c->stream;
+  const size_t n = (size_t)c->n;
+  HIPCK(c, hipMemcpyAsync(a.x0, s.x, n * sizeof(V3<double>), hipMemcpyDeviceToDevice, st));
+  HIPCK(c, hipMemcpyAsync(a.v0, s.v, n * sizeof(V3<double>), hipMemcpyDeviceToDevice, st));
+  if (launch_schritt<T>(c, c->hat_vorher ? 1 : 0, nullptr)) return 1;
+  c->hat_vorher = true;
+  c->built = false;
+  // the adjoint solve: the step's system with right-hand side dL/d(dv),
+  // solution mu and the normal targets of the adjoint; a purely relative
+  // stopping test (the seed's scale is arbitrary)
+  SysDev<T> sa = s;
+  sa.b = (V3<T>*)c->adj_b;
+  sa.dv = (V3<T>*)c->adj_mu;
+  sa.soll_direkt = a.ziel;
+  sa.boden = 0.0;
+  c->adj_sys_host.assign((const unsigned char*)&sa, (const unsigned char*)&sa + sizeof(sa));
+  HIPCK(c, hipMemcpyAsync(c->adj_sys_dev, c->adj_sys_host.data(), sizeof(sa), hipMemcpyHostToDevice, st));
+  const SysDev<T>* sp = dsys<T>(c);
+  const SysDev<T>* sap = (const SysDev<T>*)c->adj_sys_dev;
+  const V3<T>* mu = (const V3<T>*)c->adj_mu;
+  hipLaunchKernelGGL(k_adj_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, (V3<T>*)c->adj_b);
+  HIPCK(c, hipMemsetAsync(c->adj_mu, 0, n * sizeof(V3<T>), st));
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->gs_adj);
+  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, st, sap, c->gs_adj, c->slots, 0);
+  hipLaunchKernelGGL(k_adj_nu<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
+  if (s.nk) hipLaunchKernelGGL(k_adj_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
+  if (s.nb) hipLaunchKernelGGL(k_adj_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
+  if (s.nm) hipLaunchKernelGGL(k_adj_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
+  hipLaunchKernelGGL(k_adj_sammeln<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipStreamSynchronize(st));
+  ++c->adj_schritte;
+  if (pruefe_gitter(c)) return 1;
+  unsigned fe = 0;
+  HIPCK(c, hipMemcpy(&fe, &c->gs_adj->fehler, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (fe) {
+    unsigned null = 0;
+    HIPCK(c, hipMemcpy(&c->gs_adj->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
+    return fehler(c, "grid barrier wait exceeded its spin limit (adjoint solve)");
+  }
+  return 0;
+}
+
+// New stiffness and friction values on an uploaded topology (the zero
+// pattern of the edge stiffnesses stays: a k = 0 edge is carried by a membrane
+// and not on the device).
+template <typename T>
+static int steifigkeit_setzen(gc_ctx* c, const double* kk, const double* bk, const double* md, const double* rb) {
+  SysDev<T>& s = sys<T>(c);
+  if (kk) {
+    int nz = 0, nz_dev = 0;
+    for (int e = 0; e < c->n_kanten_in; ++e) nz += kk[e] != 0.0 ? 1 : 0;
+    for (int q = 0; q < c->nk; ++q) nz_dev += c->k_quelle[q] >= 0 ? 1 : 0;
+    for (int q = 0; q < c->nk; ++q)
+      if (c->k_quelle[q] >= 0 && kk[c->k_quelle[q]] == 0.0) nz = -1;
+    if (nz != nz_dev) return fehler(c, "gc_set_steifigkeit: the edges with k = 0 must stay the topology's");
+    for (int q = 0; q < c->nk; ++q)
+      if (c->k_quelle[q] >= 0) c->k_k[q] = kk[c->k_quelle[q]];
+    if (hoch(c, const_cast<double*>(s.k_k), c->k_k.data(), c->k_k.size())) return 1;
+  }
+  if (bk) {
+    c->b_k.assign(bk, bk + c->b_k.size());
+    if (hoch(c, const_cast<double*>(s.b_k), c->b_k.data(), c->b_k.size())) return 1;
+  }
+  if (md) {
+    const size_t nm = c->m_p.size() / 3;
+    std::vector<Membran<T>> mel(nm);
+    std::vector<Membran<double>> mel64(nm);
+    for (size_t e = 0; e < nm; ++e) {
+      for (int q = 0; q < 3; ++q) c->m_d[12 * e + 9 + q] = md[3 * e + q];
+      const double* d = &c->m_d[12 * e];
+      Membran<double>& m = mel64[e];
+      m.flaeche = d[0];
+      for (int q = 0; q < 4; ++q) m.r[q] = d[1 + q];
+      m.kette[0] = d[5]; m.kette[1] = d[6];
+      m.schuss[0] = d[7]; m.schuss[1] = d[8];
+      m.ek = d[9]; m.es = d[10]; m.g = d[11];
+      Membran<T>& t = mel[e];
+      t.flaeche = T(m.flaeche);
+      for (int q = 0; q < 4; ++q) t.r[q] = T(m.r[q]);
+      t.kette[0] = T(m.kette[0]); t.kette[1] = T(m.kette[1]);
+      t.schuss[0] = T(m.schuss[0]); t.schuss[1] = T(m.schuss[1]);
+      t.ek = T(m.ek); t.es = T(m.es); t.g = T(m.g);
+    }
+    if (hoch(c, const_cast<Membran<T>*>(s.m_el), mel.data(), nm)) return 1;
+    if ((const void*)s.m_el64 != (const void*)s.m_el && hoch(c, const_cast<Membran<double>*>(s.m_el64), mel64.data(), nm)) return 1;
+  }
+  if (rb) {
+    c->reib.assign(rb, rb + c->n);
+    if (hoch(c, const_cast<double*>(s.reib), c->reib.data(), c->reib.size())) return 1;
+  }
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+template <typename T>
+static int adj_gradient_lesen(gc_ctx* c, double* d_kanten, double* d_naehte, double* d_biegung, double* d_membran,
+                              double* d_reibung, int reset) {
+  if (adj_vorbereiten<T>(c)) return 1;
+  const SysDev<T>& s = sys<T>(c);
+  const AdjDev& a = c->adj;
+  std::vector<double> dk(s.nk), db(s.nb), dm(3 * (size_t)s.nm), dr(c->n);
+  hipStream_t st = c->stream;
+  if (s.nk) HIPCK(c, hipMemcpyAsync(dk.data(), a.d_kante, dk.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (s.nb) HIPCK(c, hipMemcpyAsync(db.data(), a.d_biegung, db.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (s.nm) HIPCK(c, hipMemcpyAsync(dm.data(), a.d_membran, dm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipMemcpyAsync(dr.data(), a.d_reibung, dr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipStreamSynchronize(st));
+  if (d_kanten) std::fill(d_kanten, d_kanten + c->n_kanten_in, 0.0);
+  if (d_naehte) std::fill(d_naehte, d_naehte + c->n_naehte, 0.0);
+  for (int q = 0; q < s.nk; ++q) {
+    const int e = c->k_quelle[q];
+    if (e >= 0 && d_kanten) d_kanten[e] = dk[q];
+    if (e < 0 && d_naehte) d_naehte[-(e + 1)] = dk[q];
+  }
+  if (d_biegung) std::copy(db.begin(), db.end(), d_biegung);
+  if (d_membran) std::copy(dm.begin(), dm.end(), d_membran);
+  if (d_reibung) std::copy(dr.begin(), dr.end(), d_reibung);
+  if (reset) {
+    if (s.nk) HIPCK(c, hipMemsetAsync(a.d_kante, 0, dk.size() * sizeof(double), st));
+    if (s.nb) HIPCK(c, hipMemsetAsync(a.d_biegung, 0, db.size() * sizeof(double), st));
+    if (s.nm) HIPCK(c, hipMemsetAsync(a.d_membran, 0, dm.size() * sizeof(double), st));
+    HIPCK(c, hipMemsetAsync(a.d_reibung, 0, dr.size() * sizeof(double), st));
+    HIPCK(c, hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+static V3<double>* zustand_x(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.x : c->sd.x; }
+static V3<double>* zustand_v(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.v : c->sd.v; }
+static V3<double>* zustand_dvv(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.dv_vorher : c->sd.dv_vorher; }
+
 extern "C" {
 
 gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
@@ -2141,6 +2352,116 @@ int gc_batch_zeiten(gc_batch* b, double* ms_schritt, double* ms_pcg) {
   if (!b) return 1;
   if (ms_schritt) *ms_schritt = b->ms_schritt;
   if (ms_pcg) *ms_pcg = b->ms_pcg;
+  return 0;
+}
+
+// --- adjoint and saved states ----------------------------------------------
+
+int gc_adjoint_step(gc_ctx* c, double h) {
+  if (!c) return 1;
+  return c->opt.praezision == GC_FP32 ? adj_schritt<float>(c, h) : adj_schritt<double>(c, h);
+}
+
+int gc_adjoint_set(gc_ctx* c, const double* gx, const double* gv) {
+  if (!c || !c->topo) return 1;
+  int rc = c->opt.praezision == GC_FP32 ? adj_vorbereiten<float>(c) : adj_vorbereiten<double>(c);
+  if (rc) return rc;
+  const size_t bytes = (size_t)c->n * sizeof(V3<double>);
+  if (gx) HIPCK(c, hipMemcpyAsync(c->adj.gx, gx, bytes, hipMemcpyHostToDevice, c->stream));
+  else HIPCK(c, hipMemsetAsync(c->adj.gx, 0, bytes, c->stream));
+  if (gv) HIPCK(c, hipMemcpyAsync(c->adj.gv, gv, bytes, hipMemcpyHostToDevice, c->stream));
+  else HIPCK(c, hipMemsetAsync(c->adj.gv, 0, bytes, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_adjoint_get(gc_ctx* c, double* gx, double* gv) {
+  if (!c || !c->topo) return 1;
+  if (!c->adj_bereit) return fehler(c, "no adjoint state yet (gc_adjoint_set)");
+  const size_t bytes = (size_t)c->n * sizeof(V3<double>);
+  if (gx) HIPCK(c, hipMemcpyAsync(gx, c->adj.gx, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (gv) HIPCK(c, hipMemcpyAsync(gv, c->adj.gv, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_adjoint_gradient(gc_ctx* c, double* d_kanten, double* d_naehte, double* d_biegung, double* d_membran,
+                        double* d_reibung, int32_t reset) {
+  if (!c || !c->topo) return 1;
+  return c->opt.praezision == GC_FP32 ? adj_gradient_lesen<float>(c, d_kanten, d_naehte, d_biegung, d_membran, d_reibung, reset)
+                                      : adj_gradient_lesen<double>(c, d_kanten, d_naehte, d_biegung, d_membran, d_reibung, reset);
+}
+
+int gc_adjoint_statistik(gc_ctx* c, gc_adjoint_stat* out, int32_t reset) {
+  if (!c || !c->topo) return 1;
+  if (!c->adj_bereit) {
+    if (out) std::memset(out, 0, sizeof(*out));
+    return 0;
+  }
+  GridState g;
+  unsigned long long z[2];
+  HIPCK(c, hipMemcpyAsync(&g, c->gs_adj, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(z, c->adj.zaehler, sizeof(z), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    out->schritte = c->adj_schritte;
+    out->cg_iterationen = g.cg_iterationen;
+    out->nicht_konvergiert = g.nicht_konvergiert;
+    out->abweichungen = (int64_t)z[0];
+    out->eintritte = (int64_t)z[1];
+    out->rest_max = g.rest_max;
+  }
+  if (reset) {
+    c->adj_schritte = 0;
+    HIPCK(c, hipMemsetAsync(c->gs_adj, 0, sizeof(GridState), c->stream));
+    HIPCK(c, hipMemsetAsync(c->adj.zaehler, 0, sizeof(z), c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int gc_set_steifigkeit(gc_ctx* c, const double* kanten_k, const double* biegung_k, const double* membran_ek_es_g,
+                       const double* reibung) {
+  if (!c || !c->topo) return 1;
+  return c->opt.praezision == GC_FP32 ? steifigkeit_setzen<float>(c, kanten_k, biegung_k, membran_ek_es_g, reibung)
+                                      : steifigkeit_setzen<double>(c, kanten_k, biegung_k, membran_ek_es_g, reibung);
+}
+
+int gc_zustand_sichern(gc_ctx* c, int32_t slot) {
+  if (!c || !c->topo) return 1;
+  gc_ctx::Zustand& z = c->zustaende[slot];
+  const size_t n = (size_t)c->n;
+  if (!z.x) {
+    z.x = dalloc<V3<double>>(c, n);
+    z.v = dalloc<V3<double>>(c, n);
+    z.dvv = dalloc<V3<double>>(c, n);
+    if (!z.x || !z.v || !z.dvv) {
+      c->zustaende.erase(slot);
+      return fehler(c, "hipMalloc failed (saved state)");
+    }
+  }
+  const size_t bytes = n * sizeof(V3<double>);
+  HIPCK(c, hipMemcpyAsync(z.x, zustand_x(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(z.v, zustand_v(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(z.dvv, zustand_dvv(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+  z.hat_vorher = c->hat_vorher;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_zustand_laden(gc_ctx* c, int32_t slot) {
+  if (!c || !c->topo) return 1;
+  auto it = c->zustaende.find(slot);
+  if (it == c->zustaende.end()) return fehler(c, "gc_zustand_laden: no state saved in this slot");
+  const gc_ctx::Zustand& z = it->second;
+  const size_t bytes = (size_t)c->n * sizeof(V3<double>);
+  HIPCK(c, hipMemcpyAsync(zustand_x(c), z.x, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(zustand_v(c), z.v, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(zustand_dvv(c), z.dvv, bytes, hipMemcpyDeviceToDevice, c->stream));
+  c->hat_vorher = z.hat_vorher;
+  c->built = false;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  alle_param(c);
   return 0;
 }
 

@@ -305,6 +305,47 @@ class GeraetSzene:
         self._ck(self.lib.gc_get_v(self.h, _p(o)), "gc_get_v")
         return o
 
+    # -- material values, saved states, adjoint ----------------------------
+    def set_steifigkeit(self, kanten_k=None, biegung_k=None, membran_ek_es_g=None, reibung=None) -> None:
+        """New stiffnesses / friction on the uploaded topology (None keeps a group)."""
+        keep = [None if a is None else _f64(a) for a in (kanten_k, biegung_k, membran_ek_es_g, reibung)]
+        ptr = [None if a is None else _p(a) for a in keep]
+        self._ck(self.lib.gc_set_steifigkeit(self.h, *ptr), "gc_set_steifigkeit")
+
+    def sichern(self, slot: int) -> None:
+        self._ck(self.lib.gc_zustand_sichern(self.h, slot), "gc_zustand_sichern")
+
+    def laden(self, slot: int) -> None:
+        self._ck(self.lib.gc_zustand_laden(self.h, slot), "gc_zustand_laden")
+
+    def adjoint_setzen(self, gx=None, gv=None) -> None:
+        a = None if gx is None else _f64(gx)
+        b = None if gv is None else _f64(gv)
+        self._ck(self.lib.gc_adjoint_set(self.h, None if a is None else _p(a), None if b is None else _p(b)),
+                 "gc_adjoint_set")
+
+    def adjoint_lesen(self):
+        gx = np.zeros((self.n, 3)); gv = np.zeros((self.n, 3))
+        self._ck(self.lib.gc_adjoint_get(self.h, _p(gx), _p(gv)), "gc_adjoint_get")
+        return gx, gv
+
+    def adjoint_schritt(self, h: float) -> None:
+        self._ck(self.lib.gc_adjoint_step(self.h, h), "gc_adjoint_step")
+
+    def adjoint_gradient(self, s: Szene, reset: bool = False) -> dict:
+        """Accumulated parameter gradients: per input edge, seam, bending
+        constraint, membrane (E_k, E_s, G) and point (friction)."""
+        dk = np.zeros(s.abstand_punkte.shape[0]); dn = np.zeros(s.naht_punkte.shape[0])
+        db = np.zeros(s.biegung_punkte.shape[0]); dm = np.zeros((s.membran_punkte.shape[0], 3)); dr = np.zeros(self.n)
+        self._ck(self.lib.gc_adjoint_gradient(self.h, _p(dk), _p(dn), _p(db), _p(dm), _p(dr), 1 if reset else 0),
+                 "gc_adjoint_gradient")
+        return {"kanten": dk, "naehte": dn, "biegung": db, "membran": dm, "reibung": dr}
+
+    def adjoint_statistik(self, reset: bool = False) -> dict:
+        st = L.AdjointStat_c()
+        self._ck(self.lib.gc_adjoint_statistik(self.h, C.byref(st), 1 if reset else 0), "gc_adjoint_statistik")
+        return {f: getattr(st, f) for f, _ in L.AdjointStat_c._fields_}
+
 
 class GeraetStapel:
     """Many device systems stepped as one (gc_batch_*): every substep builds all

@@ -268,6 +268,55 @@ int gc_batch_bloecke(gc_batch* b, int32_t* bloecke, int32_t* threads);
  * PCG launches' ms (NaN for graph replays). */
 int gc_batch_zeiten(gc_batch* b, double* ms_schritt, double* ms_pcg);
 
+/* --- adjoint of the step and saved states --------------------------------- */
+
+/* No counterpart in the reference, which has no gradient: its step
+ * (ImplizitCg::schritt, implizit.rs:899-1038) run in reverse so that material
+ * stiffnesses and friction can be fitted to target drapes.
+ *
+ * gc_adjoint_step runs one substep of length h from the context's state,
+ * exactly as gc_step(ctx, h, 1), and then its reverse: the context's adjoint
+ * seed, dL/dx and dL/dv of the state AFTER the step, is replaced by dL/dx and
+ * dL/dv of the state BEFORE it, and the step's parameter gradients are added
+ * to the accumulators. The discrete decisions of the step (contacts after the
+ * release pass, stick or slip, path entries, push-outs) are held as the step
+ * took them; within them the gradient is exact (per element and per point by
+ * dual numbers over the step's own functions; one adjoint solve with the
+ * step's modified PCG, reductions in FP64). Refused while cloth
+ * self-collision or normalkraft_aus_last is on. A rollout runs its steps
+ * forward saving states (gc_zustand_sichern), then for each step from the
+ * last: gc_zustand_laden of the state before it, gc_adjoint_step. */
+int gc_adjoint_step(gc_ctx* ctx, double h);
+/* Seed / read the adjoint state: dL/dx and dL/dv (double[3n] each; NULL sets zero). */
+int gc_adjoint_set(gc_ctx* ctx, const double* gx, const double* gv);
+int gc_adjoint_get(gc_ctx* ctx, double* gx, double* gv);
+/* Accumulated parameter gradients since the last reset: per input edge dL/dk
+ * (0 for the k = 0 edges a membrane carries), per seam dL/dk (its stiffness is
+ * gc_param.naht_k), per bending constraint dL/dk, per membrane
+ * dL/d(E_k, E_s, G) (3 per element, the membran_daten fields 9-11), per point
+ * dL/d(friction coefficient). Any pointer may be NULL. */
+int gc_adjoint_gradient(gc_ctx* ctx, double* d_kanten, double* d_naehte, double* d_biegung, double* d_membran,
+                        double* d_reibung, int32_t reset);
+typedef struct gc_adjoint_stat {
+  int64_t schritte;           /* adjoint steps since the last reset */
+  int64_t cg_iterationen;     /* iterations of their adjoint solves */
+  int64_t nicht_konvergiert;  /* adjoint solves that stopped above the tolerance */
+  int64_t abweichungen;       /* points whose recomputed update differed from the step's (0 expected) */
+  int64_t eintritte;          /* path entries differentiated */
+  double rest_max;            /* largest relative residual of an adjoint solve */
+} gc_adjoint_stat;
+int gc_adjoint_statistik(gc_ctx* ctx, gc_adjoint_stat* out, int32_t reset);
+/* New material values on the uploaded topology: per input edge k (the k = 0
+ * pattern must stay), per bending constraint k, per membrane (E_k, E_s, G),
+ * per point friction coefficient; NULL leaves a group as it is. The builder
+ * writes these values when it builds the scene (szene.rs:425-461, 602-647). */
+int gc_set_steifigkeit(gc_ctx* ctx, const double* kanten_k, const double* biegung_k, const double* membran_ek_es_g,
+                       const double* reibung);
+/* Save / restore the state (x, v and the warm start of the next step) in
+ * device memory under a slot number, e.g. every k steps of a rollout. */
+int gc_zustand_sichern(gc_ctx* ctx, int32_t slot);
+int gc_zustand_laden(gc_ctx* ctx, int32_t slot);
+
 #ifdef __cplusplus
 }
 #endif
