@@ -67,6 +67,7 @@ def main() -> int:
                          "adjoint20k: gradient of a 60-step 20k-point drape rollout w.r.t. per-panel stiffnesses "
                          "and friction; gross250k: the 250k-point drape, FP32 bulk against all-FP64")
     ap.add_argument("--vorlauf", type=int, default=120, help="adjoint20k: settling steps before the rollout starts")
+    ap.add_argument("--fd-schritt", type=float, default=1e-4, help="adjoint20k: relative step of the central differences")
     ap.add_argument("--fd", type=int, default=0, help="adjoint20k: central differences of the FP64 forward for this "
                                                       "many parameters (largest gradients first, plus friction)")
     args = ap.parse_args()
@@ -352,6 +353,28 @@ def adjoint_lauf(args) -> int:
     print(f"[bench] forward {1e3 * (t_b - t_a):.1f} ms, reverse {1e3 * (t_c - t_b):.1f} ms, plain {1e3 * (t_q - t_p):.1f} ms; "
           f"loss {L0:.6e}; adjoint solves {ast}", file=sys.stderr, flush=True)
 
+    # one step's vector-Jacobian product from the same state in both forms
+    # (same discrete decisions): a random seed dL/dx' back to dL/dx, dL/dv
+    # and the parameter gradients
+    vjp = {}
+    seed = np.random.default_rng(5).standard_normal((s.n, 3))
+    for pr in ("fp32", "fp64"):
+        gq, rq = kontext(pr, 1e-6)
+        rq._setzen(th0)
+        gq.laden(-1)
+        gq.adjoint_gradient(s, reset=True)
+        gq.adjoint_setzen(seed, None)
+        gq.adjoint_schritt(h)
+        gx_, gv_ = gq.adjoint_lesen()
+        vjp[pr] = (gx_, gv_, mat.zurueck(gq.adjoint_gradient(s, reset=True)), gq.lage()[0])
+        gq.close()
+    def _rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    vjp_rel = {"dL_dx": _rel(vjp["fp32"][0], vjp["fp64"][0]), "dL_dv": _rel(vjp["fp32"][1], vjp["fp64"][1]),
+               "dL_dtheta": _rel(vjp["fp32"][2], vjp["fp64"][2]),
+               "x_nach_schritt_m": float(np.max(np.abs(vjp["fp32"][3] - vjp["fp64"][3])))}
+    print(f"[bench] one-step VJP fp32 vs fp64: {vjp_rel}", file=sys.stderr, flush=True)
+
     # FP64 reference gradient: the all-FP64 adjoint on the same rollout
     g64, r64 = kontext("fp64", 1e-6)
     x64 = r64.vorwaerts(th0)
@@ -369,7 +392,7 @@ def adjoint_lauf(args) -> int:
         wahl = list(np.argsort(-np.abs(dfd_adj[:-1]))[: max(0, args.fd - 1)]) + [mat.n_param - 1]
         zeilen = []
         for k in wahl:
-            hk = 1e-4 * abs(th0[k])
+            hk = args.fd_schritt * abs(th0[k])
             tp, tm = th0.copy(), th0.copy()
             tp[k] += hk
             tm[k] -= hk
@@ -381,7 +404,7 @@ def adjoint_lauf(args) -> int:
         v_fd = np.array([z["fd"] for z in zeilen])
         v_ad = np.array([z["adjoint_fp64"] for z in zeilen])
         fd = {"parameter": zeilen, "rel_norm": float(np.linalg.norm(v_ad - v_fd) / np.linalg.norm(v_fd)),
-              "rtol": 1e-10, "schritt": "1e-4 relative"}
+              "rtol": 1e-10, "schritt": f"{args.fd_schritt:g} relative"}
     value = schritte / (t_c - t_a)
     out = {
         "metric": "implicit steps/s @40k verts, PCG rtol 1e-6; batched/adjoint vs plain GPU vs CPU",
@@ -399,7 +422,8 @@ def adjoint_lauf(args) -> int:
                    "cg_iterationen_je_schritt": round(st["cg_iterationen"] / schritte, 1),
                    "adjoint_cg_iterationen_je_schritt": round(ast["cg_iterationen"] / schritte, 1),
                    "adjoint_abweichungen": ast["abweichungen"], "adjoint_eintritte": ast["eintritte"],
-                   "fp32_gegen_fp64_gradient": rel32, "fd": fd, "parallelism": "1 GPU"},
+                   "fp32_gegen_fp64_gradient": rel32, "ein_schritt_fp32_gegen_fp64": vjp_rel, "fd": fd,
+                   "parallelism": "1 GPU"},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -428,18 +452,23 @@ def gross_lauf(args) -> int:
     ms = {"fp32": 0.0, "fp64": 0.0}
     ms_pcg = {"fp32": 0.0, "fp64": 0.0}
     fehler_max, fehler_mittel = [], []
+    iters_alt = {pr: 0 for pr in g}
     for k in range(schritte):
+        zeile = []
         for pr in ("fp32", "fp64"):
             t_a = time.perf_counter()
             g[pr].schritte(h, 1)
-            ms[pr] += 1e3 * (time.perf_counter() - t_a)
+            dt_ms = 1e3 * (time.perf_counter() - t_a)
+            ms[pr] += dt_ms
             ms_pcg[pr] += g[pr].zeiten()[1]
+            it = g[pr].statistik()["cg_iterationen"]
+            zeile.append(f"{pr} {dt_ms:.0f} ms {it - iters_alt[pr]} it")
+            iters_alt[pr] = it
         d = np.linalg.norm(g["fp32"].lage()[0] - g["fp64"].lage()[0], axis=1)
         fehler_max.append(float(d.max()))
         fehler_mittel.append(float(d.mean()))
-        if k % 10 == 9 or k == schritte - 1:
-            print(f"[bench] step {k + 1}: fp32 {ms['fp32']:.0f} ms, fp64 {ms['fp64']:.0f} ms so far; position error "
-                  f"max {d.max():.3e} m mean {d.mean():.3e} m", file=sys.stderr, flush=True)
+        print(f"[bench] step {k + 1}: {', '.join(zeile)}; position error max {d.max():.3e} m mean {d.mean():.3e} m",
+              file=sys.stderr, flush=True)
     st = {pr: g[pr].statistik() for pr in g}
     roof = {}
     for pr in g:

@@ -137,6 +137,23 @@ struct alignas(64) Slot {
   double v[4];
 };
 
+// Arrival counters of the grid barrier, after the two partial buffers of a
+// system's slot array: one per group of GC_GRUPPE blocks, one for the group
+// leaders, one release flag, each on a 128-byte line of its own. A slot array
+// holds GC_SLOTS entries.
+#define GC_GRUPPE 8
+#define GC_BARR_GRUPPEN (GC_MAX_BLOECKE / GC_GRUPPE)
+#define GC_BARR_BASIS (2 * GC_MAX_BLOECKE)
+#define GC_BARR_ZAEHLER (GC_BARR_GRUPPEN + 2)
+#define GC_SLOTS (GC_BARR_BASIS + 2 * GC_BARR_ZAEHLER)
+__device__ __forceinline__ unsigned* barr_zaehler(Slot* slots, unsigned k) {
+  return (unsigned*)&slots[GC_BARR_BASIS + 2 * k];
+}
+// Clears the barrier counters (one block; a fill kernel's work, no memset).
+__device__ __forceinline__ void barriere_leeren(Slot* slots) {
+  for (unsigned k = threadIdx.x; k < GC_BARR_ZAEHLER; k += blockDim.x) *barr_zaehler(slots, k) = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // Grid-wide all-reduce: block partials go into per-block slots, thread 0
 // releases and arrives on one counter that only grows within a launch (a
@@ -172,6 +189,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
 
 // rang, nb: this block's rank among the nb blocks that share gs and slots (the
 // whole grid for one system, a group of blocks in a batched launch).
+//
+// Arrival is a two-level tree: each block releases its partial and adds one
+// to its group's counter (groups of GC_GRUPPE consecutive ranks); the last
+// arrival of a group (seen from the counter's old value) acquires the group's
+// partials and adds one to the leaders' counter; the last leader acquires
+// them all and publishes the barrier's epoch in the release flag, which every
+// block polls. No counter sees more than GC_GRUPPE (or nb / GC_GRUPPE)
+// arrivals, where a single counter serialised all nb of them. The counters
+// grow monotonically within a launch and are cleared before each.
 template <int NV>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
                                                unsigned rang, unsigned nb) {
@@ -184,12 +210,25 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
   // other blocks (release cumulativity of the HSA memory model).
   if (threadIdx.x == 0) {
     for (int k = 0; k < NV; ++k) __hip_atomic_store(&buf[rang].v[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE, g = rang / GC_GRUPPE;
+    const unsigned gsz = nb - g * GC_GRUPPE < GC_GRUPPE ? nb - g * GC_GRUPPE : GC_GRUPPE;
+    unsigned* flagge = barr_zaehler(slots, GC_BARR_GRUPPEN + 1);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(&gs->zaehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned ziel = nb * epoch;
+    const unsigned alt = __hip_atomic_fetch_add(barr_zaehler(slots, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (alt == gsz * epoch - 1u) {  // last of its group
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const unsigned alt2 =
+          __hip_atomic_fetch_add(barr_zaehler(slots, GC_BARR_GRUPPEN), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (alt2 == ng * epoch - 1u) {  // last group: everyone has arrived
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(flagge, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     unsigned spins = 0;
     int fehler = 0;
-    while (__hip_atomic_load(&gs->zaehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ziel) {
+    while (__hip_atomic_load(flagge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
       if (++spins > (1u << 22)) { fehler = 1; break; }
       if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { fehler = 1; break; }
     }
@@ -521,14 +560,19 @@ __global__ void k_sammeln(const SysDev<T>* __restrict__ sp) {
 // Per-step reset of the shared counters (a fill kernel, so a captured step
 // records no memset node). The barrier error flag is sticky: the host reads
 // and clears it after every call, so a failure in any substep is reported.
-__global__ void k_reset(GridState* gs) {
+__global__ void k_reset(GridState* gs, Slot* slots) {
   if (threadIdx.x == 0) {
     gs->zaehler = 0;
     gs->frei[0] = 0;
     gs->frei[1] = 0;
     gs->n_fest = 0;
   }
+  barriere_leeren(slots);
 }
+
+// Clears a system's barrier counters before a PCG launch that no k_reset or
+// release pass precedes.
+__global__ void k_barriere_leeren(Slot* slots) { barriere_leeren(slots); }
 
 // Warm start (implizit.rs:906): dv of the previous step, h*g on the first.
 template <typename T>
@@ -813,11 +857,14 @@ __global__ void __launch_bounds__(1024) k_pcg_stapel(const StapelBlock<T>* __res
 // Release pass (implizit.rs:916-942): normal force from the residual
 // A*dv - b; a held point that pulls is released.
 template <typename T>
-__global__ void k_freigabe(const SysDev<T>* __restrict__ sp, GridState* gs, int durchgang) {
+__global__ void k_freigabe(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
   const SysDev<T>& s = *sp;
   if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   if (__hip_atomic_load(&gs->n_fest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) gs->zaehler = 0;
+  if (blockIdx.x == 0) {  // the next PCG launch starts with clear barrier counters
+    if (threadIdx.x == 0) gs->zaehler = 0;
+    barriere_leeren(slots);
+  }
   unsigned frei = 0, weg = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     if (s.fflag[i] != 1) continue;

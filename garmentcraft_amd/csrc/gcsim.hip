@@ -751,7 +751,7 @@ static int launch_bau(gc_ctx* c) {
   const SysDev<T>& s = sys<T>(c);
   const SysDev<T>* sp = dsys<T>(c);
   hipStream_t st = c->stream;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->gs);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->gs, c->slots);
   if (s.selbst && launch_selbst_bau<T>(c)) return 1;
   if (s.nk) hipLaunchKernelGGL(k_bau_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sp);
   if (s.nb) hipLaunchKernelGGL(k_bau_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, sp);
@@ -786,7 +786,7 @@ static int launch_vor_pcg(gc_ctx* c, int hat_vorher) {
 
 template <typename T>
 static int launch_zwischen_pcg(gc_ctx* c) {
-  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(sys<T>(c).n)), dim3(BUILD_THREADS), 0, c->stream, dsys<T>(c), c->gs, 0);
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(sys<T>(c).n)), dim3(BUILD_THREADS), 0, c->stream, dsys<T>(c), c->gs, c->slots, 0);
   HIPCK(c, hipGetLastError());
   return 0;
 }
@@ -795,7 +795,7 @@ template <typename T>
 static int launch_nach_pcg(gc_ctx* c) {
   const SysDev<T>& s = sys<T>(c);
   const SysDev<T>* sp = dsys<T>(c);
-  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs, 1);
+  hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs, c->slots, 1);
   hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs);
   if (s.selbst && launch_selbst_stoss<T>(c)) return 1;
   HIPCK(c, hipGetLastError());
@@ -1263,6 +1263,7 @@ static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* 
       const SysDev<T>& s = sys<T>(b->ctx[k]);
       BHIPCK(b, hipMemcpyAsync(s.dv, g[k].dv, s.n * sizeof(V3<T>), hipMemcpyDeviceToDevice, b->stream));
       BHIPCK(b, hipMemcpyAsync(b->ctx[k]->gs, &g[k].start, sizeof(GridState), hipMemcpyHostToDevice, b->stream));
+      hipLaunchKernelGGL(k_barriere_leeren, dim3(1), dim3(256), 0, b->stream, b->ctx[k]->slots);
     }
     BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
     if (batch_pcg<T>(b, 0)) return 1;
@@ -1377,7 +1378,7 @@ static int loese(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
   SysDev<T>& s = sys<T>(c);
   if (hoch_vec<T>(c, s.dv, dv)) return 1;
   if (sys_hoch<T>(c)) return 1;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, c->stream, c->gs);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, c->stream, c->gs, c->slots);
   if (launch_pcg<T>(c, 0)) return 1;
   GridState g;
   HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
@@ -1540,7 +1541,7 @@ static int adj_schritt(gc_ctx* c, double h) {
   const V3<T>* mu = (const V3<T>*)c->adj_mu;
   hipLaunchKernelGGL(k_adj_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, (V3<T>*)c->adj_b);
   HIPCK(c, hipMemsetAsync(c->adj_mu, 0, n * sizeof(V3<T>), st));
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->gs_adj);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->gs_adj, c->slots);
   hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, st, sap, c->gs_adj, c->slots, 0);
   hipLaunchKernelGGL(k_adj_nu<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
   if (s.nk) hipLaunchKernelGGL(k_adj_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
@@ -1707,7 +1708,7 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
     return nullptr;
   }
   c->gs = dalloc<GridState>(c, 1);
-  c->slots = dalloc<Slot>(c, 2 * GC_MAX_BLOECKE);
+  c->slots = dalloc<Slot>(c, GC_SLOTS);
   if (!c->gs || !c->slots) {
     melde("hipMalloc failed");
     frei_alles(c);
