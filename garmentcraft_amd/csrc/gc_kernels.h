@@ -125,6 +125,7 @@ struct GridState {
   unsigned int selbst_ueberlauf;  // a self-collision list exceeded its capacity
   long long naehe_pd, naehe_kk, selbst_stoesse, durchdringungen, gebiete, durchstoesse;
   long long rundungsgrenze;  // FP32 solves stopped at their rounding floor
+  long long neustarts;       // FP32 true-residual restarts
   // result of the last solve
   int iterationen;
   int nicht_konv;
@@ -705,125 +706,142 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     double nix[1] = {0.0};
     if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
   }
-  // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z) (implizit.rs:385-391)
-  double part[2] = {0.0, 0.0};
-  for (int t = rang; t < s.n_kacheln; t += nb) {
-    int lo, hi;
-    zeilen(t, &lo, &hi);
-    const uint32_t basis = s.off[GC_KAT * lo];
-    const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-    for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = s.dv[s.kachel_v[v]];
-    __syncthreads();
-    kachel_beitraege(s, t, slot, ul);
-    __syncthreads();
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return s.dv[j]; });
-      const V3<T> ap = hp * th2 + s.dv[i] * s.mdiag[i];
-      const uint8_t fl = s.fflag[i];
-      const V3<T> n = s.fnorm[i];
-      const V3<T> bi = s.b[i];
-      const V3<T> ri = filter_fl(fl, n, bi - ap);
-      const T* P = s.pinv + 9 * (size_t)i;
-      if (fl != 2) {
-        const V3<T> fb = filter_fl(fl, n, bi);
-        part[0] += (double)dot(fb, m3_mal(P, fb));
-      }
-      const V3<T> zi = m3_mal(P, ri);
-      part[1] += (double)dot(ri, filter_fl(fl, n, zi));
-      s.r[i] = ri;
-      s.z[i] = zi;
-    }
-    __syncthreads();
-  }
-  if (!grid_allreduce<2>(part, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
-  const double bb = part[0];
-  double delta = part[1];
-  const double ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
+  // FP32 true-residual restarts. The FP32 product A p is exact only to a few
+  // ulps, so with a stiff system the recursively updated residual drifts from
+  // b - A dv: it stalls near what FP32 can reach and can then grow without
+  // bound. When delta stops improving (400 iterations without a new best) or
+  // grows 1e4x above its best, the solve starts again from the current dv
+  // with the true residual (residual replacement), at most 3 times; after
+  // that it ends as not converged and is counted at its rounding floor. The
+  // FP64 build runs the reference's loop unchanged (no restart).
+  double bb = 0.0, delta = 0.0, ziel = 0.0;
   int it = 0;
-  double beta = 0.0;
-  int cur = 0;
-  // FP32 rounding floor: the FP32 product A p is exact only to a few ulps, so
-  // with a stiff system the recursive residual can stall near the tolerance
-  // and then grow without bound. Once the best delta has come within 100x of
-  // the target, 200 iterations without a new best or a delta 100x above it end
-  // the FP32 solve (not converged); ordinary CG oscillation passes both tests.
-  // The FP64 build keeps the reference's loop unchanged.
-  double dmin = delta;
-  int it_min = 0;
   bool grenze = false;
-  while (delta > ziel && it < s.cg_max) {
-    V3<T>* pn = s.p[cur];
-    const V3<T>* po = s.p[cur ^ 1];
-    const T tb = T(beta);
-    const bool erste = it == 0;
-    // ap = S(A p), ca = p . ap (implizit.rs:401-405)
-    double ca[1] = {0.0};
+  int neustarts = 0;
+  for (bool erster_start = true;; erster_start = false) {
+    // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z) (implizit.rs:385-391)
+    double part[2] = {0.0, 0.0};
     for (int t = rang; t < s.n_kacheln; t += nb) {
       int lo, hi;
       zeilen(t, &lo, &hi);
       const uint32_t basis = s.off[GC_KAT * lo];
       const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung(s, s.kachel_v[v], po, tb, erste);
+      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = s.dv[s.kachel_v[v]];
       __syncthreads();
       kachel_beitraege(s, t, slot, ul);
-      if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung(s, j, po, tb, erste); });
       __syncthreads();
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return s.dv[j]; });
+        const V3<T> ap = hp * th2 + s.dv[i] * s.mdiag[i];
         const uint8_t fl = s.fflag[i];
-        const V3<T> pi = richtung(s, (uint32_t)i, po, tb, erste);
-        const V3<T> hk = kachel_zeile(s, i, basis, slot);
-        const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk) : hk;
-        const V3<T> ap = filter_fl(fl, s.fnorm[i], hp * th2 + pi * s.mdiag[i]);
-        pn[i] = pi;
-        s.ap[i] = ap;
-        ca[0] += (double)dot(pi, ap);
+        const V3<T> n = s.fnorm[i];
+        const V3<T> bi = s.b[i];
+        const V3<T> ri = filter_fl(fl, n, bi - ap);
+        const T* P = s.pinv + 9 * (size_t)i;
+        if (fl != 2) {
+          const V3<T> fb = filter_fl(fl, n, bi);
+          part[0] += (double)dot(fb, m3_mal(P, fb));
+        }
+        const V3<T> zi = m3_mal(P, ri);
+        part[1] += (double)dot(ri, filter_fl(fl, n, zi));
+        s.r[i] = ri;
+        s.z[i] = zi;
       }
       __syncthreads();
     }
-    if (!grid_allreduce<1>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
-    if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
-    const double alpha = delta / ca[0];
-    // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
-    double neu[1] = {0.0};
-    const T ta = T(alpha);
-    for (int t = rang; t < s.n_kacheln; t += nb) {
-      int lo, hi;
-      zeilen(t, &lo, &hi);
-      for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        s.dv[i] = s.dv[i] + pn[i] * ta;
-        const V3<T> ri = s.r[i] - s.ap[i] * ta;
-        s.r[i] = ri;
-        const V3<T> zi = m3_mal(s.pinv + 9 * (size_t)i, ri);
-        s.z[i] = zi;
-        neu[0] += (double)dot(ri, zi);
+    if (!grid_allreduce<2>(part, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    delta = part[1];
+    if (erster_start) {
+      bb = part[0];
+      ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
+    }
+    double beta = 0.0;
+    int cur = 0;
+    const int it_start = it;
+    double dmin = delta;
+    int it_min = it;
+    bool neu_starten = false;
+    while (delta > ziel && it < s.cg_max) {
+      V3<T>* pn = s.p[cur];
+      const V3<T>* po = s.p[cur ^ 1];
+      const T tb = T(beta);
+      const bool erste = it == it_start;
+      // ap = S(A p), ca = p . ap (implizit.rs:401-405)
+      double ca[1] = {0.0};
+      for (int t = rang; t < s.n_kacheln; t += nb) {
+        int lo, hi;
+        zeilen(t, &lo, &hi);
+        const uint32_t basis = s.off[GC_KAT * lo];
+        const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
+        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung(s, s.kachel_v[v], po, tb, erste);
+        __syncthreads();
+        kachel_beitraege(s, t, slot, ul);
+        if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung(s, j, po, tb, erste); });
+        __syncthreads();
+        for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+          const uint8_t fl = s.fflag[i];
+          const V3<T> pi = richtung(s, (uint32_t)i, po, tb, erste);
+          const V3<T> hk = kachel_zeile(s, i, basis, slot);
+          const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk) : hk;
+          const V3<T> ap = filter_fl(fl, s.fnorm[i], hp * th2 + pi * s.mdiag[i]);
+          pn[i] = pi;
+          s.ap[i] = ap;
+          ca[0] += (double)dot(pi, ap);
+        }
+        __syncthreads();
+      }
+      if (!grid_allreduce<1>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+      if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
+      const double alpha = delta / ca[0];
+      // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
+      double neu[1] = {0.0};
+      const T ta = T(alpha);
+      for (int t = rang; t < s.n_kacheln; t += nb) {
+        int lo, hi;
+        zeilen(t, &lo, &hi);
+        for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+          s.dv[i] = s.dv[i] + pn[i] * ta;
+          const V3<T> ri = s.r[i] - s.ap[i] * ta;
+          s.r[i] = ri;
+          const V3<T> zi = m3_mal(s.pinv + 9 * (size_t)i, ri);
+          s.z[i] = zi;
+          neu[0] += (double)dot(ri, zi);
+        }
+      }
+      if (!grid_allreduce<1>(neu, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+      beta = neu[0] / delta;
+      delta = neu[0];
+      cur ^= 1;
+      ++it;
+      if constexpr (sizeof(T) == 4) {
+        if (delta < dmin) {
+          dmin = delta;
+          it_min = it;
+        } else if (delta > 1e4 * dmin || it - it_min >= 400 || !isfinite(delta)) {
+          neu_starten = true;
+          break;
+        }
       }
     }
-    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
-    beta = neu[0] / delta;
-    delta = neu[0];
-    cur ^= 1;
-    ++it;
-    if constexpr (sizeof(T) == 4) {
-      if (delta < dmin) {
-        dmin = delta;
-        it_min = it;
-      } else if (dmin <= 100.0 * ziel && (delta > 100.0 * dmin || it - it_min >= 200)) {
-        grenze = true;
-        break;
-      }
+    if (!neu_starten) break;
+    if (neustarts == 3) {
+      grenze = true;
+      break;
     }
+    ++neustarts;
   }
   if (rang == 0 && threadIdx.x == 0) {
     gs->iterationen = it;
-    gs->nicht_konv = delta > ziel ? 1 : 0;
+    gs->nicht_konv = delta <= ziel ? 0 : 1;  // a NaN delta counts as not converged
     gs->rest = sqrt(delta / gmax(bb, s.boden));
     gs->delta = delta;
     gs->ziel = ziel;
     gs->bb = bb;
     gs->loesungen += 1;
     gs->cg_iterationen += it;
-    if (delta > ziel) gs->nicht_konvergiert += 1;
+    if (!(delta <= ziel)) gs->nicht_konvergiert += 1;
     if (grenze) gs->rundungsgrenze += 1;
+    gs->neustarts += neustarts;
     gs->rest_max = gmax(gs->rest_max, gs->rest);
   }
 }

@@ -2087,11 +2087,13 @@ int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
     out->durchdringungsgebiete = g.gebiete;
     out->durchstoesse = g.durchstoesse;
     out->cg_rundungsgrenze = g.rundungsgrenze;
+    out->cg_neustarts = g.neustarts;
   }
   if (reset) {
     g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
     g.naehe_pd = g.naehe_kk = g.selbst_stoesse = g.durchdringungen = g.gebiete = g.durchstoesse = 0;
     g.rundungsgrenze = 0;
+    g.neustarts = 0;
     g.rest_max = 0.0;
     g.spins_max = 0;
     HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));

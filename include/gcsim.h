@@ -127,6 +127,7 @@ typedef struct gc_statistik {
   int64_t durchdringungsgebiete;
   int64_t durchstoesse;
   int64_t cg_rundungsgrenze;   /* FP32 solves ended at their rounding floor (no FP64 counterpart) */
+  int64_t cg_neustarts;        /* FP32 true-residual restarts of the PCG (no FP64 counterpart) */
 } gc_statistik;
 
 /* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):
