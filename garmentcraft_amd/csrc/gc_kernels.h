@@ -140,12 +140,12 @@ struct alignas(64) Slot {
 
 // Arrival counters of the grid barrier, after the two partial buffers of a
 // system's slot array: one per group of GC_GRUPPE blocks, one for the group
-// leaders, one release flag, each on a 128-byte line of its own. A slot array
-// holds GC_SLOTS entries.
+// leaders, and one release flag per group, each on a 128-byte line of its
+// own. A slot array holds GC_SLOTS entries.
 #define GC_GRUPPE 8
 #define GC_BARR_GRUPPEN (GC_MAX_BLOECKE / GC_GRUPPE)
 #define GC_BARR_BASIS (2 * GC_MAX_BLOECKE)
-#define GC_BARR_ZAEHLER (GC_BARR_GRUPPEN + 2)
+#define GC_BARR_ZAEHLER (2 * GC_BARR_GRUPPEN + 1)
 #define GC_SLOTS (GC_BARR_BASIS + 2 * GC_BARR_ZAEHLER)
 __device__ __forceinline__ unsigned* barr_zaehler(Slot* slots, unsigned k) {
   return (unsigned*)&slots[GC_BARR_BASIS + 2 * k];
@@ -166,87 +166,95 @@ __device__ __forceinline__ void barriere_leeren(Slot* slots) {
 // can only overwrite parity p after every block has published epoch e+1,
 // i.e. after everyone has read epoch e-1.
 // ---------------------------------------------------------------------------
-// Deterministic block sum: xor butterfly inside the wave (every lane ends
-// with the same value, IEEE addition being commutative), then the wave sums in
-// wave order. sh: 16 * NV doubles.
+// Block sums are deterministic: an xor butterfly inside each wave (every lane
+// ends with the same value, IEEE addition being commutative), then the wave
+// sums in wave order.
+// rang, nb: this block's rank among the nb blocks that share gs and slots (the
+// whole grid for one system, a group of blocks in a batched launch).
+//
+// The block's partial is its waves' sums in wave order; wave 0 alone then
+// arrives, waits, sums the nb partials and hands the result to the block
+// through LDS (two workgroup barriers per grid barrier).
+//
+// Arrival is a two-level tree: each block writes back its stores (release,
+// the L2 write-back to the coherent level) and adds one to its group's
+// counter (groups of GC_GRUPPE consecutive ranks); the last arrival of a group
+// (seen from the counter's old value) adds one to the leaders' counter; the
+// last leader publishes the barrier's epoch in every group's release flag;
+// a block polls its group's flag. No counter or flag sees more than GC_GRUPPE
+// (or nb / GC_GRUPPE) blocks, where a single counter serialised all nb
+// arrivals and a single flag all nb polls. The counters
+// grow monotonically within a launch and are cleared before each. There is no
+// acquire (no L2 invalidation): everything a block reads that another block
+// wrote in this launch (partials, counters, dv / z / p of points it does not
+// own) is read with agent-scope loads, which the program order after the
+// observed flag puts after every writer's write-back.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* sh) {
+__device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
+                                               unsigned rang, unsigned nb) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
-  __syncthreads();
   if (lane == 0)
     for (int k = 0; k < NV; ++k) sh[wave * NV + k] = v[k];
+  // every wave's stores and wave sums are in; sh's previous readers (wave 0
+  // below, the broadcast readers at the end) finished before this barrier
   __syncthreads();
-  for (int k = 0; k < NV; ++k) {
-    double a = 0.0;
-    for (int w = 0; w < nw; ++w) a += sh[w * NV + k];
-    v[k] = a;
-  }
-  __syncthreads();
-}
-
-// rang, nb: this block's rank among the nb blocks that share gs and slots (the
-// whole grid for one system, a group of blocks in a batched launch).
-//
-// Arrival is a two-level tree: each block releases its partial and adds one
-// to its group's counter (groups of GC_GRUPPE consecutive ranks); the last
-// arrival of a group (seen from the counter's old value) acquires the group's
-// partials and adds one to the leaders' counter; the last leader acquires
-// them all and publishes the barrier's epoch in the release flag, which every
-// block polls. No counter sees more than GC_GRUPPE (or nb / GC_GRUPPE)
-// arrivals, where a single counter serialised all nb of them. The counters
-// grow monotonically within a launch and are cleared before each.
-template <int NV>
-__device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
-                                               unsigned rang, unsigned nb) {
-  block_sum<NV>(v, sh);
   ++epoch;
   Slot* buf = slots + (epoch & 1u) * GC_MAX_BLOECKE;
+  double* aus = sh + 16 * NV;
   __shared__ int s_fehler;
-  // block_sum ended with a workgroup barrier: every wave's stores happen-before
-  // thread 0's agent-scope release below, which makes them visible to the
-  // other blocks (release cumulativity of the HSA memory model).
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NV; ++k) __hip_atomic_store(&buf[rang].v[k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE, g = rang / GC_GRUPPE;
-    const unsigned gsz = nb - g * GC_GRUPPE < GC_GRUPPE ? nb - g * GC_GRUPPE : GC_GRUPPE;
-    unsigned* flagge = barr_zaehler(slots, GC_BARR_GRUPPEN + 1);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned alt = __hip_atomic_fetch_add(barr_zaehler(slots, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (alt == gsz * epoch - 1u) {  // last of its group
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const unsigned alt2 =
-          __hip_atomic_fetch_add(barr_zaehler(slots, GC_BARR_GRUPPEN), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (alt2 == ng * epoch - 1u) {  // last group: everyone has arrived
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(flagge, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    unsigned spins = 0;
+  if (wave == 0) {
     int fehler = 0;
-    while (__hip_atomic_load(flagge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-      if (++spins > (1u << 22)) { fehler = 1; break; }
-      if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { fehler = 1; break; }
+    if (lane == 0) {
+      for (int k = 0; k < NV; ++k) {
+        double a = 0.0;
+        for (int w = 0; w < nw; ++w) a += sh[w * NV + k];
+        __hip_atomic_store(&buf[rang].v[k], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE, g = rang / GC_GRUPPE;
+      const unsigned gsz = nb - g * GC_GRUPPE < GC_GRUPPE ? nb - g * GC_GRUPPE : GC_GRUPPE;
+      unsigned* flagge = barr_zaehler(slots, GC_BARR_GRUPPEN + 1 + g);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const unsigned alt = __hip_atomic_fetch_add(barr_zaehler(slots, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (alt == gsz * epoch - 1u) {  // last of its group
+        const unsigned alt2 =
+            __hip_atomic_fetch_add(barr_zaehler(slots, GC_BARR_GRUPPEN), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (alt2 == ng * epoch - 1u)  // last group: everyone has arrived; wake every group
+          for (unsigned q = 0; q < ng; ++q)
+            __hip_atomic_store(barr_zaehler(slots, GC_BARR_GRUPPEN + 1 + q), epoch, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      }
+      unsigned spins = 0;
+      while (__hip_atomic_load(flagge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        if (++spins > (1u << 22)) { fehler = 1; break; }
+        if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { fehler = 1; break; }
+      }
+      if (fehler) __hip_atomic_store(&gs->fehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_max(&gs->spins_max, spins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (fehler) __hip_atomic_store(&gs->fehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_max(&gs->spins_max, spins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    s_fehler = fehler;
+    fehler = __shfl(fehler, 0, 64);
+    // the nb partials, summed in the same order by every block: lane l takes
+    // l, l + 64, ..., then the wave's butterfly
+    double acc[NV];
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    if (!fehler)
+      for (unsigned b = lane; b < nb; b += 64)
+        for (int k = 0; k < NV; ++k) acc[k] += __hip_atomic_load(&buf[b].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (lane == 0) {
+      for (int k = 0; k < NV; ++k) aus[k] = acc[k];
+      s_fehler = fehler;
+    }
   }
   __syncthreads();
   if (s_fehler) return false;
-  // every block sums the nb partials in the same order
-  double acc[NV];
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-    for (int k = 0; k < NV; ++k) acc[k] += __hip_atomic_load(&buf[b].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV>(acc, sh);
-  for (int k = 0; k < NV; ++k) v[k] = acc[k];
+  for (int k = 0; k < NV; ++k) v[k] = aus[k];
   return true;
 }
 
@@ -600,6 +608,21 @@ __device__ __forceinline__ V3<T> filter_fl(uint8_t fl, V3<T> n, V3<T> v) {
   return v - n * dot(v, n);
 }
 
+// Coherent load of a vector that another block of the grid wrote during this
+// launch (the PCG's dv, z, p). Agent-scope loads are served by the level that
+// is coherent across the XCDs, where the writer's release at the grid barrier
+// (its L2 write-back) has put the data; so the barrier needs no acquire (no L2
+// invalidation), and the read-only operator records stay cached in each XCD's
+// L2 from one iteration to the next. Every read of such data across blocks
+// goes through this function; a block's own rows are read with plain loads.
+template <typename T>
+__device__ __forceinline__ V3<T> lade_k(const V3<T>* a, uint32_t j) {
+  const T* q = (const T*)(a + j);
+  return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+          __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+          __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+}
+
 template <typename T>
 __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T>* slot, const V3<T>* u) {
   const uint32_t e0 = s.kachel_el_off[t], e1 = s.kachel_el_off[t + 1];
@@ -655,11 +678,13 @@ __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_
 // The search direction of point j, p = S(z + beta p_old) (implizit.rs:420-422);
 // the first direction is S(z) (implizit.rs:390). The owner and every tile that
 // reads the point evaluate this same expression, so they agree to the bit.
-template <typename T>
+// K = true for a point another block owns: its z and p are read with
+// coherent loads (see lade_k).
+template <bool K, typename T>
 __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const V3<T>* po, T tb, bool erste) {
   const uint8_t fl = s.fflag[j];
-  const V3<T> z = s.z[j];
-  const V3<T> q = erste ? z : z + po[j] * tb;
+  V3<T> q = K ? lade_k(s.z, j) : s.z[j];
+  if (!erste) q = q + (K ? lade_k(po, j) : po[j]) * tb;
   return filter_fl(fl, s.fnorm[j], q);
 }
 
@@ -726,12 +751,12 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       zeilen(t, &lo, &hi);
       const uint32_t basis = s.off[GC_KAT * lo];
       const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = s.dv[s.kachel_v[v]];
+      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = lade_k(s.dv, s.kachel_v[v]);
       __syncthreads();
       kachel_beitraege(s, t, slot, ul);
       __syncthreads();
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return s.dv[j]; });
+        const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return lade_k(s.dv, j); });
         const V3<T> ap = hp * th2 + s.dv[i] * s.mdiag[i];
         const uint8_t fl = s.fflag[i];
         const V3<T> n = s.fnorm[i];
@@ -773,14 +798,14 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         zeilen(t, &lo, &hi);
         const uint32_t basis = s.off[GC_KAT * lo];
         const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung(s, s.kachel_v[v], po, tb, erste);
+        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, s.kachel_v[v], po, tb, erste);
         __syncthreads();
         kachel_beitraege(s, t, slot, ul);
-        if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung(s, j, po, tb, erste); });
+        if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); });
         __syncthreads();
         for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
           const uint8_t fl = s.fflag[i];
-          const V3<T> pi = richtung(s, (uint32_t)i, po, tb, erste);
+          const V3<T> pi = richtung<false>(s, (uint32_t)i, po, tb, erste);
           const V3<T> hk = kachel_zeile(s, i, basis, slot);
           const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk) : hk;
           const V3<T> ap = filter_fl(fl, s.fnorm[i], hp * th2 + pi * s.mdiag[i]);
@@ -849,7 +874,7 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
   extern __shared__ __align__(16) unsigned char lds_roh[];
-  __shared__ double sh[16 * 2];
+  __shared__ double sh[16 * 2 + 2];
   pcg_loesen<T>(*sp, gs, slots, durchgang, (int)blockIdx.x, (int)gridDim.x, lds_roh, sh);
 }
 
@@ -867,7 +892,7 @@ struct StapelBlock {
 template <typename T>
 __global__ void __launch_bounds__(1024) k_pcg_stapel(const StapelBlock<T>* __restrict__ tab, int durchgang) {
   extern __shared__ __align__(16) unsigned char lds_roh[];
-  __shared__ double sh[16 * 2];
+  __shared__ double sh[16 * 2 + 2];
   const StapelBlock<T> e = tab[blockIdx.x];
   pcg_loesen<T>(*e.sys, e.gs, e.slots, durchgang, e.rang, e.nb, lds_roh, sh);
 }

@@ -381,7 +381,7 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, c->pcg_lds));
   if (occ < 1) return fehler(c, "PCG kernel does not fit on a CU with this LDS budget");
   c->pcg_max_blocks = std::min(GC_MAX_BLOECKE, occ * c->cus);
-  int want = c->opt.bloecke > 0 ? c->opt.bloecke : 64;
+  int want = c->opt.bloecke > 0 ? c->opt.bloecke : 128;
   want = std::min(want, c->n_kacheln);
   c->pcg_blocks = std::max(1, std::min(want, c->pcg_max_blocks));
   return 0;
@@ -1690,7 +1690,7 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
   gc_ctx* c = new gc_ctx();
   c->opt = o;
   c->cus = p.multiProcessorCount;
-  c->lds_budget = (size_t)(o.lds_kb > 0 ? o.lds_kb : 128) * 1024;
+  c->lds_budget = (size_t)(o.lds_kb > 0 ? o.lds_kb : 64) * 1024;
   if (c->lds_budget + 1024 > (size_t)p.sharedMemPerBlock) {
     melde("lds_kb exceeds the LDS of a workgroup");
     return nullptr;
