@@ -83,6 +83,7 @@ struct SysDev {
   int32_t n_kacheln;
   const int32_t* kachel_zeilen;   // lo, hi per tile
   const uint32_t* kachel_el_off;  // element-list offsets (n_kacheln + 1)
+  const uint32_t* kachel_kat;     // per tile: first bending entry, first membrane entry (edges come first)
   const uint32_t* kachel_el;      // category << 30 | constraint
   const uint16_t* kachel_lv;      // 4 tile-local point indices per listed element
   const uint16_t* kachel_pos;     // 4 LDS slots per listed element, 0xFFFF = point not in the tile
@@ -623,46 +624,68 @@ __device__ __forceinline__ V3<T> lade_k(const V3<T>* a, uint32_t j) {
           __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
 }
 
+// One constraint's contributions to the tile's own points, each into its LDS
+// slot (pos = 0xFFFF: that corner is not in the tile).
+template <typename T>
+__device__ __forceinline__ void membran_beitrag(const Membran<T>& el, const Hessen<T>& hh, uint2 lw, uint2 pw, V3<T>* slot,
+                                                const V3<T>* u) {
+  const V3<T> d3[3] = {u[lw.x & 0xFFFFu], u[lw.x >> 16], u[lw.y & 0xFFFFu]};
+  V3<T> o[3];
+  mal_h_aus_alle(el, hh, d3, o);
+  const uint32_t pos[3] = {pw.x & 0xFFFFu, pw.x >> 16, pw.y & 0xFFFFu};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (pos[k] != 0xFFFFu) slot[pos[k]] = o[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void biege_beitrag(const V3<T> (&g)[4], T kt, uint2 lw, uint2 pw, V3<T>* slot, const V3<T>* u) {
+  const uint32_t lv[4] = {lw.x & 0xFFFFu, lw.x >> 16, lw.y & 0xFFFFu, lw.y >> 16};
+  const uint32_t pos[4] = {pw.x & 0xFFFFu, pw.x >> 16, pw.y & 0xFFFFu, pw.y >> 16};
+  T gp = T(0);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) gp += dot(g[l], u[lv[l]]);
+  const T kg = kt * gp;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    if (pos[m] != 0xFFFFu) slot[pos[m]] = g[m] * kg;
+}
+
+// Stage B of a tile: every listed constraint's contributions. The tile's list
+// is sorted by category (edges, bending, membranes; kachel_kat gives the
+// boundaries), so each loop runs one kind of constraint without a branch on
+// the kind. (Two constraints per thread and pass, loads first, was measured
+// slower: 7.7 -> 10.0 us per iteration for this stage on the drape.)
 template <typename T>
 __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T>* slot, const V3<T>* u) {
-  const uint32_t e0 = s.kachel_el_off[t], e1 = s.kachel_el_off[t + 1];
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const uint32_t ref = s.kachel_el[e];
-    const uint32_t kat = ref >> 30, c = ref & 0x3FFFFFFFu;
+  const uint32_t e0 = s.kachel_el_off[t], e3 = s.kachel_el_off[t + 1];
+  const uint32_t e1 = s.kachel_kat[2 * t], e2 = s.kachel_kat[2 * t + 1];
+  const uint32_t bd = blockDim.x;
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += bd) {
+    const uint32_t c = s.kachel_el[e] & 0x3FFFFFFFu;
     const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
     const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    const uint32_t lv[4] = {lw.x & 0xFFFFu, lw.x >> 16, lw.y & 0xFFFFu, lw.y >> 16};
-    const uint32_t pos[4] = {pw.x & 0xFFFFu, pw.x >> 16, pw.y & 0xFFFFu, pw.y >> 16};
-    if (kat == 2u) {
-      const Membran<T> el = s.m_el[c];
-      const Hessen<T> hh = s.m_h[c];
-      const V3<T> d3[3] = {u[lv[0]], u[lv[1]], u[lv[2]]};
-      V3<T> o[3];
-      mal_h_aus_alle(el, hh, d3, o);
+    const V3<T> n = s.k_n[c];
+    const T q = s.k_q[c], kk = s.k_kk[c];
+    const V3<T> d = u[lw.x & 0xFFFFu] - u[lw.x >> 16];
+    const V3<T> bd3 = (n * (dot(n, d) * (T(1) - q)) + d * q) * kk;
+    if ((pw.x & 0xFFFFu) != 0xFFFFu) slot[pw.x & 0xFFFFu] = bd3;
+    if ((pw.x >> 16) != 0xFFFFu) slot[pw.x >> 16] = bd3 * T(-1);
+  }
+  for (uint32_t e = e1 + threadIdx.x; e < e2; e += bd) {
+    const uint32_t c = s.kachel_el[e] & 0x3FFFFFFFu;
+    const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
+    const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
+    V3<T> g[4];
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (pos[k] != 0xFFFFu) slot[pos[k]] = o[k];
-    } else if (kat == 1u) {
-      const V3<T>* gr = s.b_grad + 4 * (size_t)c;
-      V3<T> g[4];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) g[l] = gr[l];
-      const T kt = s.b_kt[c];
-      T gp = T(0);
-#pragma unroll
-      for (int l = 0; l < 4; ++l) gp += dot(g[l], u[lv[l]]);
-      const T kg = kt * gp;
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-        if (pos[m] != 0xFFFFu) slot[pos[m]] = g[m] * kg;
-    } else {
-      const V3<T> n = s.k_n[c];
-      const T q = s.k_q[c], kk = s.k_kk[c];
-      const V3<T> d = u[lv[0]] - u[lv[1]];
-      const V3<T> bd = (n * (dot(n, d) * (T(1) - q)) + d * q) * kk;
-      if (pos[0] != 0xFFFFu) slot[pos[0]] = bd;
-      if (pos[1] != 0xFFFFu) slot[pos[1]] = bd * T(-1);
-    }
+    for (int l = 0; l < 4; ++l) g[l] = s.b_grad[4 * (size_t)c + l];
+    biege_beitrag(g, s.b_kt[c], lw, pw, slot, u);
+  }
+  for (uint32_t e = e2 + threadIdx.x; e < e3; e += bd) {
+    const uint32_t c = s.kachel_el[e] & 0x3FFFFFFFu;
+    const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
+    const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
+    membran_beitrag(s.m_el[c], s.m_h[c], lw, pw, slot, u);
   }
 }
 

@@ -277,7 +277,7 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   // every (constraint, own point).
   const size_t budget = c->lds_budget / sizeof(V3<T>);
   std::vector<int32_t> kz;
-  std::vector<uint32_t> k_off{0}, k_el, v_off{0}, k_v;
+  std::vector<uint32_t> k_off{0}, k_el, v_off{0}, k_v, k_kat;
   std::vector<uint16_t> k_pos, k_lv;
   size_t max_slots = 0, max_v = 0;
   {
@@ -323,8 +323,11 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
       }
       max_slots = std::max(max_slots, slots_hier);
       max_v = std::max(max_v, pkt.size());
+      uint32_t kat_start[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
       for (size_t q = 0; q < liste.size();) {
         std::array<uint16_t, 4> pos = liste[q].second;
+        const uint32_t kat_q = (uint32_t)(liste[q].first >> 32);
+        if (kat_start[kat_q] == 0xFFFFFFFFu) kat_start[kat_q] = (uint32_t)k_el.size();
         size_t r = q + 1;
         for (; r < liste.size() && liste[r].first == liste[q].first; ++r)
           for (int m = 0; m < 4; ++m)
@@ -341,6 +344,15 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
         }
         q = r;
       }
+      // the list is sorted by category: edges, bending, membranes; per tile
+      // the first bending and the first membrane entry
+      {
+        const uint32_t ende = (uint32_t)k_el.size();
+        uint32_t b1 = kat_start[1] != 0xFFFFFFFFu ? kat_start[1] : (kat_start[2] != 0xFFFFFFFFu ? kat_start[2] : ende);
+        uint32_t b2 = kat_start[2] != 0xFFFFFFFFu ? kat_start[2] : ende;
+        k_kat.push_back(b1);
+        k_kat.push_back(b2);
+      }
       for (uint32_t v : pkt) k_v.push_back(v);
       kz.push_back(lo);
       kz.push_back(hi);
@@ -353,13 +365,15 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   c->n_kacheln = (int)(kz.size() / 2);
   int32_t* d_kz = dalloc<int32_t>(c, kz.size());
   uint32_t* d_koff = dalloc<uint32_t>(c, k_off.size());
+  uint32_t* d_kkat = dalloc<uint32_t>(c, k_kat.size());
   uint32_t* d_kel = dalloc<uint32_t>(c, k_el.size());
   uint16_t* d_kpos = dalloc<uint16_t>(c, k_pos.size());
   uint16_t* d_klv = dalloc<uint16_t>(c, k_lv.size());
   uint32_t* d_voff = dalloc<uint32_t>(c, v_off.size());
   uint32_t* d_kv = dalloc<uint32_t>(c, k_v.size());
-  if (!d_kz || !d_koff || !d_kel || !d_kpos || !d_klv || !d_voff || !d_kv) return fehler(c, "hipMalloc failed (tiles)");
+  if (!d_kz || !d_koff || !d_kkat || !d_kel || !d_kpos || !d_klv || !d_voff || !d_kv) return fehler(c, "hipMalloc failed (tiles)");
   if (hoch(c, d_kz, kz.data(), kz.size()) || hoch(c, d_koff, k_off.data(), k_off.size()) ||
+      hoch(c, d_kkat, k_kat.data(), k_kat.size()) ||
       hoch(c, d_kel, k_el.data(), k_el.size()) || hoch(c, d_kpos, k_pos.data(), k_pos.size()) ||
       hoch(c, d_klv, k_lv.data(), k_lv.size()) || hoch(c, d_voff, v_off.data(), v_off.size()) ||
       hoch(c, d_kv, k_v.data(), k_v.size()))
@@ -367,6 +381,7 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   s.n_kacheln = c->n_kacheln;
   s.kachel_zeilen = d_kz;
   s.kachel_el_off = d_koff;
+  s.kachel_kat = d_kkat;
   s.kachel_el = d_kel;
   s.kachel_pos = d_kpos;
   s.kachel_lv = d_klv;
