@@ -417,34 +417,104 @@ __device__ __forceinline__ void sk_je_eintrag(const SelbstDev& s, const Quader& 
       }
 }
 
-// Exclusive scan of `anzahl` counters in one block (out[anzahl] = total);
-// optionally stores the total in a counter slot.
-__global__ void __launch_bounds__(1024) k_sk_scan(uint32_t* werte, int anzahl, uint32_t* kopie, uint32_t* gesamt_ziel) {
-  __shared__ uint32_t teil[1024];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int je = (anzahl + nt - 1) / nt;
-  const int a = t * je, b = min(anzahl, a + je);
-  uint32_t summe = 0;
-  for (int i = a; i < b; ++i) summe += werte[i];
-  teil[t] = summe;
-  __syncthreads();
-  for (int o = 1; o < nt; o <<= 1) {
-    const uint32_t v = t >= o ? teil[t - o] : 0u;
-    __syncthreads();
-    teil[t] += v;
-    __syncthreads();
+// Exclusive scan of `anzahl` counters (werte[anzahl] = total, optionally a
+// copy and the total in a counter slot) over many blocks, for long arrays
+// (the buckets of the hash, one count per edge or triangle): pass 1 sums each chunk of
+// SK_SCAN_CHUNK counts, pass 2 scans the chunk sums in one block and writes
+// the total, pass 3 scans inside each chunk from its offset. Every load and
+// store is coalesced; integer sums, so the result is the serial scan's.
+#define SK_SCAN_K 4
+#define SK_SCAN_CHUNK (1024 * SK_SCAN_K)
+
+// inclusive scan of one value per thread over a block of 1024 threads; tot:
+// the block's sum (every thread)
+__device__ __forceinline__ uint32_t sk_block_scan(uint32_t v, uint32_t* wsum, uint32_t* tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
   }
-  uint32_t lauf = teil[t] - summe;
-  for (int i = a; i < b; ++i) {
-    const uint32_t v = werte[i];
-    werte[i] = lauf;
-    if (kopie) kopie[i] = lauf;
+  __syncthreads();
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t u = __shfl_up(w, o, 64);
+      if (lane >= o) w += u;
+    }
+    if (lane < 16) wsum[lane] = w;  // inclusive over waves
+  }
+  __syncthreads();
+  *tot = wsum[15];
+  return v + (wave > 0 ? wsum[wave - 1] : 0u);
+}
+
+__global__ void __launch_bounds__(1024) k_sk_scan_teil(const uint32_t* werte, int anzahl, uint32_t* teil) {
+  __shared__ uint32_t wsum[16];
+  const int basis = blockIdx.x * SK_SCAN_CHUNK;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SK_SCAN_K; ++k) {
+    const int i = basis + k * 1024 + (int)threadIdx.x;
+    if (i < anzahl) s += werte[i];
+  }
+  uint32_t tot;
+  sk_block_scan(s, wsum, &tot);
+  if (threadIdx.x == 0) teil[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_sk_scan_teile(uint32_t* teil, int nteile, uint32_t* werte, int anzahl, uint32_t* kopie,
+                                                         uint32_t* gesamt_ziel) {
+  __shared__ uint32_t wsum[16];
+  uint32_t lauf = 0;
+  for (int b = 0; b < nteile; b += 1024) {
+    const int i = b + (int)threadIdx.x;
+    const uint32_t v = i < nteile ? teil[i] : 0u;
+    uint32_t tot;
+    const uint32_t inkl = sk_block_scan(v, wsum, &tot);
+    if (i < nteile) teil[i] = lauf + inkl - v;
+    lauf += tot;
+    __syncthreads();  // wsum is reused by the next round
+  }
+  if (threadIdx.x == 0) {
+    werte[anzahl] = lauf;
+    if (kopie) kopie[anzahl] = lauf;
+    if (gesamt_ziel) *gesamt_ziel = lauf;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_sk_scan_fertig(uint32_t* werte, int anzahl, const uint32_t* teil, uint32_t* kopie) {
+  __shared__ uint32_t w[SK_SCAN_CHUNK];
+  __shared__ uint32_t wsum[16];
+  const int basis = blockIdx.x * SK_SCAN_CHUNK;
+#pragma unroll
+  for (int k = 0; k < SK_SCAN_K; ++k) {
+    const int j = k * 1024 + (int)threadIdx.x, i = basis + j;
+    w[j] = i < anzahl ? werte[i] : 0u;
+  }
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SK_SCAN_K; ++k) s += w[threadIdx.x * SK_SCAN_K + k];
+  uint32_t tot;
+  uint32_t lauf = teil[blockIdx.x] + sk_block_scan(s, wsum, &tot) - s;
+#pragma unroll
+  for (int k = 0; k < SK_SCAN_K; ++k) {
+    const uint32_t v = w[threadIdx.x * SK_SCAN_K + k];
+    w[threadIdx.x * SK_SCAN_K + k] = lauf;
     lauf += v;
   }
-  if (t == nt - 1) {
-    werte[anzahl] = teil[nt - 1];
-    if (kopie) kopie[anzahl] = teil[nt - 1];
-    if (gesamt_ziel) *gesamt_ziel = teil[nt - 1];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SK_SCAN_K; ++k) {
+    const int j = k * 1024 + (int)threadIdx.x, i = basis + j;
+    if (i < anzahl) {
+      werte[i] = w[j];
+      if (kopie) kopie[i] = w[j];
+    }
   }
 }
 

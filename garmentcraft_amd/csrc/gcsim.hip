@@ -88,6 +88,8 @@ struct gc_ctx {
   uint32_t* sk_fe_fuell = nullptr;
   uint32_t* sk_fk_fuell = nullptr;
   uint32_t* sk_st_fuell = nullptr;
+  uint32_t* sk_scan_teil = nullptr;          // chunk sums of the multi-block scan
+  uint32_t sk_scan_max = 0;                  // longest array it scans
   double* sk_naht_ruhe_dev = nullptr;
   void* sk_block_t = nullptr;                // spring blocks in the bulk type
   void* sk_fe_t = nullptr;                   // per spring entry: its term in the PCG (bulk type)
@@ -628,6 +630,8 @@ static int selbst_vorbereiten(gc_ctx* c) {
   c->sk_fe_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_fe_fuell);
   c->sk_fk_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_fk_fuell);
   c->sk_st_fuell = dalloc<uint32_t>(c, (size_t)n + 1); chk(c->sk_st_fuell);
+  c->sk_scan_max = std::max<uint32_t>(c->sk_eimer_max, std::max<uint32_t>((uint32_t)n, std::max<uint32_t>(nt, ne)));
+  c->sk_scan_teil = dalloc<uint32_t>(c, c->sk_scan_max / SK_SCAN_CHUNK + 1); chk(c->sk_scan_teil);
   c->sk_block_t = dalloc<T>(c, 9 * (size_t)k.kapazitaet_federn); chk(c->sk_block_t);
   c->sk_fe_t = dalloc<V3<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_t);
   c->sk_fe_tq = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_tq);
@@ -656,14 +660,24 @@ static unsigned sk_gitter(long count) {
   return (unsigned)std::max(1L, std::min(b, 4096L));
 }
 
+// Exclusive scan of werte[0..anzahl) in place, total into werte[anzahl]
+// (and into kopie[0..anzahl] and *gesamt_ziel when given), over
+// ceil(anzahl / SK_SCAN_CHUNK) blocks.
+static void sk_scan(gc_ctx* c, uint32_t* werte, int anzahl, uint32_t* kopie, uint32_t* gesamt_ziel) {
+  hipStream_t st = c->stream;
+  const int teile = std::max(1, (anzahl + SK_SCAN_CHUNK - 1) / SK_SCAN_CHUNK);
+  hipLaunchKernelGGL(k_sk_scan_teil, dim3(teile), dim3(1024), 0, st, (const uint32_t*)werte, anzahl, c->sk_scan_teil);
+  hipLaunchKernelGGL(k_sk_scan_teile, dim3(1), dim3(1024), 0, st, c->sk_scan_teil, teile, werte, anzahl, kopie, gesamt_ziel);
+  hipLaunchKernelGGL(k_sk_scan_fertig, dim3(teile), dim3(1024), 0, st, werte, anzahl, (const uint32_t*)c->sk_scan_teil, kopie);
+}
+
 // One spatial hash over c->sk.quader[0..anzahl) with the cell at *zelle_p.
 static void sk_hash(gc_ctx* c, int anzahl, const double* zelle_p, uint32_t eimer) {
   hipStream_t st = c->stream;
   SelbstDev* sp = c->sk_dev;
   hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(eimer + 1)), dim3(BUILD_THREADS), 0, st, c->sk.eimer_fuell, (int)eimer + 1);
   hipLaunchKernelGGL(k_sk_eimer_zaehlen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, c->sk.eimer_fuell, (int)eimer, c->sk.eimer_anfang,
-                     c->sk.zaehler + SK_Z_EINTRAEGE);
+  sk_scan(c, c->sk.eimer_fuell, (int)eimer, c->sk.eimer_anfang, c->sk.zaehler + SK_Z_EINTRAEGE);
   hipLaunchKernelGGL(k_sk_eimer_fuellen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
   hipLaunchKernelGGL(k_sk_eimer_sortieren, dim3(sk_gitter(c->sk.kapazitaet_eintraege)), dim3(BUILD_THREADS), 0, st, sp, eimer);
 }
@@ -684,13 +698,13 @@ static int launch_selbst_bau(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_quader_punkte, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.n, c->sk_zellen, c->sk_eimer_p);
   hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 0);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.zahl, k.nt, (uint32_t*)nullptr, k.zaehler + SK_Z_NAEHE_PD);
+  sk_scan(c, k.zahl, k.nt, nullptr, k.zaehler + SK_Z_NAEHE_PD);
   hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 1);
   // edge-edge (:506-592)
   hipLaunchKernelGGL(k_sk_quader_kanten, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.ne, c->sk_zellen, c->sk_eimer_k);
   hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 0);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.zahl, k.ne, (uint32_t*)nullptr, k.zaehler + SK_Z_NAEHE_KK);
+  sk_scan(c, k.zahl, k.ne, nullptr, k.zaehler + SK_Z_NAEHE_KK);
   hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 1);
   hipLaunchKernelGGL(k_sk_naehe_summe, dim3(1), dim3(64), 0, st, sp);
   // springs and the damping of the approach (:713-787)
@@ -702,7 +716,7 @@ static int launch_selbst_bau(gc_ctx* c) {
   sk_hash(c, k.nt, c->sk_zellen + 1, c->sk_eimer_d);
   hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
                      c->sk_eimer_d, 0);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.zahl, k.ne, (uint32_t*)nullptr, k.zaehler + SK_Z_SCHNITTE);
+  sk_scan(c, k.zahl, k.ne, nullptr, k.zaehler + SK_Z_SCHNITTE);
   hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
                      c->sk_eimer_d, 1);
   hipLaunchKernelGGL(k_sk_schnitte_summe, dim3(1), dim3(64), 0, st, sp);
@@ -713,8 +727,8 @@ static int launch_selbst_bau(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.fe_off, k.n + 1);
   hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.fk_off, k.n + 1);
   hipLaunchKernelGGL(k_sk_feder_zaehlen, dim3(sk_gitter(k.kapazitaet_federn)), dim3(BUILD_THREADS), 0, st, sp);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.fe_off, k.n, c->sk_fe_fuell, (uint32_t*)nullptr);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.fk_off, k.n, c->sk_fk_fuell, (uint32_t*)nullptr);
+  sk_scan(c, k.fe_off, k.n, c->sk_fe_fuell, nullptr);
+  sk_scan(c, k.fk_off, k.n, c->sk_fk_fuell, nullptr);
   hipLaunchKernelGGL(k_sk_feder_pruefen, dim3(1), dim3(64), 0, st, sp);
   hipLaunchKernelGGL(k_sk_feder_fuellen, dim3(sk_gitter(k.kapazitaet_federn)), dim3(BUILD_THREADS), 0, st, sp, c->sk_fe_fuell,
                      c->sk_fk_fuell);
@@ -747,7 +761,7 @@ static int launch_selbst_stoss(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_stoss_filter, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp);
   hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.st_off, k.n + 1);
   hipLaunchKernelGGL(k_sk_stoss_zaehlen, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp);
-  hipLaunchKernelGGL(k_sk_scan, dim3(1), dim3(1024), 0, st, k.st_off, k.n, c->sk_st_fuell, (uint32_t*)nullptr);
+  sk_scan(c, k.st_off, k.n, c->sk_st_fuell, nullptr);
   hipLaunchKernelGGL(k_sk_stoss_fuellen, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_st_fuell);
   for (int it = 0; it < k.stoss_iterationen; ++it) {
     hipLaunchKernelGGL(k_sk_stoss_beitrag<T>, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp, syp);
