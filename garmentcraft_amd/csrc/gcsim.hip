@@ -1569,7 +1569,8 @@ static int adj_schritt(gc_ctx* c, double h) {
   const SysDev<T>* sap = (const SysDev<T>*)c->adj_sys_dev;
   const V3<T>* mu = (const V3<T>*)c->adj_mu;
   hipLaunchKernelGGL(k_adj_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, (V3<T>*)c->adj_b);
-  HIPCK(c, hipMemsetAsync(c->adj_mu, 0, n * sizeof(V3<T>), st));
+  // the adjoint solve starts from the previous (later) step's mu, cleared by
+  // gc_adjoint_set: the adjoint state changes little from step to step
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->gs_adj, c->slots);
   hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, st, sap, c->gs_adj, c->slots, 0);
   hipLaunchKernelGGL(k_adj_nu<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, a, mu);
@@ -2403,6 +2404,9 @@ int gc_adjoint_set(gc_ctx* c, const double* gx, const double* gv) {
   else HIPCK(c, hipMemsetAsync(c->adj.gx, 0, bytes, c->stream));
   if (gv) HIPCK(c, hipMemcpyAsync(c->adj.gv, gv, bytes, hipMemcpyHostToDevice, c->stream));
   else HIPCK(c, hipMemsetAsync(c->adj.gv, 0, bytes, c->stream));
+  // a new seed: the next adjoint solve starts from mu = 0
+  const size_t mu_bytes = (size_t)c->n * (c->opt.praezision == GC_FP32 ? sizeof(V3<float>) : sizeof(V3<double>));
+  HIPCK(c, hipMemsetAsync(c->adj_mu, 0, mu_bytes, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
