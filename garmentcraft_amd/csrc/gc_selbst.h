@@ -783,36 +783,91 @@ __global__ void k_sk_federn(const SelbstDev* __restrict__ sp) {
 }
 
 // Approach damping in list order (:762-781): each pair reads the velocities
-// the pairs before it left.
-__global__ void k_sk_daempfung(const SelbstDev* __restrict__ sp) {
+// the pairs before it left, so the pairs are applied one after another by one
+// thread. The block stages them in chunks: all threads load a chunk's pair
+// records (mu, masses, weights, normal, held flags) and put the velocities
+// of the points they touch into an LDS hash table keyed by point; thread 0
+// then applies the chunk's pairs in order on the table (the arithmetic and
+// its order are the sequential loop's), and the block writes the touched
+// velocities back before the next chunk reads them.
+#define SK_DP_CHUNK 128
+#define SK_DP_TAB 1024  // power of two, at least twice the chunk's 4 * SK_DP_CHUNK points
+__global__ void __launch_bounds__(1024) k_sk_daempfung(const SelbstDev* __restrict__ sp) {
   const SelbstDev& s = *sp;
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  __shared__ uint32_t tab_key[SK_DP_TAB];
+  __shared__ D3 tab_v[SK_DP_TAB];
+  __shared__ D3 p_n[SK_DP_CHUNK];
+  __shared__ double p_g[SK_DP_CHUNK][4];
+  __shared__ double p_mu[SK_DP_CHUNK], p_m1[SK_DP_CHUNK], p_m2[SK_DP_CHUNK];
+  __shared__ uint16_t p_pos[SK_DP_CHUNK][4];
+  __shared__ uint8_t p_fest[SK_DP_CHUNK];
   const uint32_t nn = s.zaehler[SK_Z_NAEHE];
-  for (uint32_t q0 = 0; q0 < nn; ++q0) {
-    const Naehe& p = s.naehe[q0];
-    double mu;
-    if (p.art == 0) mu = 0.5 * (s.mu[p.punkte[0]] + (s.mu[p.punkte[1]] + s.mu[p.punkte[2]] + s.mu[p.punkte[3]]) / 3.0);
-    else mu = 0.25 * (s.mu[p.punkte[0]] + s.mu[p.punkte[1]] + s.mu[p.punkte[2]] + s.mu[p.punkte[3]]);
-    const D3 n = p.normale;
-    double m1 = 0.0, m2 = 0.0;
-    D3 dv = zero3<double>();
-    for (int q = 0; q < 4; ++q) {
-      const double g = p.gewichte[q];
-      dv = dv + s.v[p.punkte[q]] * g;
-      if (g > 0.0) m1 += g * s.masse[p.punkte[q]];
-      else m2 -= g * s.masse[p.punkte[q]];
+  const uint32_t leer = 0xFFFFFFFFu;
+  for (uint32_t q0 = 0; q0 < nn; q0 += SK_DP_CHUNK) {
+    const uint32_t anzahl = nn - q0 < SK_DP_CHUNK ? nn - q0 : SK_DP_CHUNK;
+    for (uint32_t e = threadIdx.x; e < SK_DP_TAB; e += blockDim.x) tab_key[e] = leer;
+    __syncthreads();
+    // the chunk's points into the table (linear probing; at most 4 * SK_DP_CHUNK
+    // distinct keys, so the table is at most half full)
+    for (uint32_t t = threadIdx.x; t < 4 * anzahl; t += blockDim.x) {
+      const uint32_t pr = t >> 2, q = t & 3u;
+      const uint32_t i = s.naehe[q0 + pr].punkte[q];
+      uint32_t e = (i * 2654435761u) & (SK_DP_TAB - 1);
+      for (;;) {
+        const uint32_t alt = atomicCAS(&tab_key[e], leer, i);
+        if (alt == leer || alt == i) break;
+        e = (e + 1) & (SK_DP_TAB - 1);
+      }
+      p_pos[pr][q] = (uint16_t)e;
     }
-    const double an = dot(dv, n) * SK_ANNAEHERUNG;
-    if (an < 0.0 && m1 + m2 > 0.0) {
-      const D3 u = n * an + (dv - n * an) * mu;
+    // the per-pair values that do not depend on velocities (same expressions
+    // and order as the sequential loop)
+    for (uint32_t pr = threadIdx.x; pr < anzahl; pr += blockDim.x) {
+      const Naehe& p = s.naehe[q0 + pr];
+      double mu;
+      if (p.art == 0) mu = 0.5 * (s.mu[p.punkte[0]] + (s.mu[p.punkte[1]] + s.mu[p.punkte[2]] + s.mu[p.punkte[3]]) / 3.0);
+      else mu = 0.25 * (s.mu[p.punkte[0]] + s.mu[p.punkte[1]] + s.mu[p.punkte[2]] + s.mu[p.punkte[3]]);
+      double m1 = 0.0, m2 = 0.0;
+      uint8_t fest = 0;
       for (int q = 0; q < 4; ++q) {
         const double g = p.gewichte[q];
-        const uint32_t i = p.punkte[q];
-        if (s.w[i] <= 0.0) continue;
-        const double anteil = g > 0.0 ? m2 / (m1 + m2) : m1 / (m1 + m2);
-        s.v[i] = s.v[i] - u * (g * anteil);
+        p_g[pr][q] = g;
+        if (g > 0.0) m1 += g * s.masse[p.punkte[q]];
+        else m2 -= g * s.masse[p.punkte[q]];
+        if (s.w[p.punkte[q]] <= 0.0) fest |= (uint8_t)(1u << q);
       }
+      p_mu[pr] = mu;
+      p_m1[pr] = m1;
+      p_m2[pr] = m2;
+      p_n[pr] = p.normale;
+      p_fest[pr] = fest;
     }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < SK_DP_TAB; e += blockDim.x)
+      if (tab_key[e] != leer) tab_v[e] = s.v[tab_key[e]];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (uint32_t pr = 0; pr < anzahl; ++pr) {
+        const D3 n = p_n[pr];
+        const double m1 = p_m1[pr], m2 = p_m2[pr];
+        D3 dv = zero3<double>();
+        for (int q = 0; q < 4; ++q) dv = dv + tab_v[p_pos[pr][q]] * p_g[pr][q];
+        const double an = dot(dv, n) * SK_ANNAEHERUNG;
+        if (an < 0.0 && m1 + m2 > 0.0) {
+          const D3 u = n * an + (dv - n * an) * p_mu[pr];
+          for (int q = 0; q < 4; ++q) {
+            const double g = p_g[pr][q];
+            if (p_fest[pr] & (1u << q)) continue;
+            const double anteil = g > 0.0 ? m2 / (m1 + m2) : m1 / (m1 + m2);
+            const uint16_t e = p_pos[pr][q];
+            tab_v[e] = tab_v[e] - u * (g * anteil);
+          }
+        }
+      }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < SK_DP_TAB; e += blockDim.x)
+      if (tab_key[e] != leer) s.v[tab_key[e]] = tab_v[e];
+    __syncthreads();
   }
 }
 

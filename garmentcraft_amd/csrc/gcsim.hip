@@ -709,7 +709,7 @@ static int launch_selbst_bau(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_naehe_summe, dim3(1), dim3(64), 0, st, sp);
   // springs and the damping of the approach (:713-787)
   hipLaunchKernelGGL(k_sk_federn, dim3(sk_gitter(k.kapazitaet_naehe)), dim3(BUILD_THREADS), 0, st, sp);
-  hipLaunchKernelGGL(k_sk_daempfung, dim3(1), dim3(64), 0, st, sp);
+  hipLaunchKernelGGL(k_sk_daempfung, dim3(1), dim3(1024), 0, st, sp);
   // cuts, regions, push springs (:858-1059)
   hipLaunchKernelGGL(k_sk_quader_dreiecke, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp);
   hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 1);
