@@ -401,20 +401,97 @@ __global__ void k_sk_eimer_sortieren(const SelbstDev* __restrict__ sp, uint32_t 
   }
 }
 
-// f(entry, cell) for every entry in the cells of q, cells in the reference's
-// order (i, then j, then k), bucket contents in index order.
-template <typename F>
-__device__ __forceinline__ void sk_je_eintrag(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, F f) {
+// Every entry in the cells of q for one query box, in the reference's order
+// (cells by i, then j, then k; bucket contents in index order), walked by a
+// whole wave: the bucket ranges of the cells are concatenated in that order
+// and handed out 64 at a time, lane l taking entry r + l of round r. Entries
+// with index <= ab are left out (buckets are sorted, so they are a prefix
+// found by bisection). g(aktiv, entry, cell) runs on every lane of the wave
+// once per round, aktiv false on lanes past the end, so g may use wave
+// operations. q, zelle and ab must be the same on all 64 lanes, and all 64
+// lanes must call.
+template <typename G>
+__device__ __forceinline__ void sk_welle_je_eintrag(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, int64_t ab, G g) {
+  const int lane = threadIdx.x & 63;
   if (s.zaehler[SK_Z_EINTRAEGE] > s.kapazitaet_eintraege) return;  // overflow: reported, not read
+  if (!sk_bereich_ok(q, zelle)) return;
   int64_t lo[3], hi[3];
-  if (!sk_zellen(s, q, zelle, lo, hi)) return;
-  for (int64_t i = lo[0]; i <= hi[0]; ++i)
-    for (int64_t j = lo[1]; j <= hi[1]; ++j)
-      for (int64_t k = lo[2]; k <= hi[2]; ++k) {
-        const uint32_t h = sk_streue(i, j, k, eimer);
-        const int64_t c[3] = {i, j, k};
-        for (uint32_t a = s.eimer_anfang[h]; a < s.eimer_anfang[h + 1]; ++a) f(s.eintraege[a], c);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = sk_zelle(q.lo[a], zelle);
+    hi[a] = sk_zelle(q.hi[a], zelle);
+  }
+  // each extent is at most 65 cells (sk_bereich_ok), so the counts fit an int
+  const int nx = (int)(hi[0] - lo[0] + 1), ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
+  const int nc = nx * ny * nz;
+  if (nc > SK_MAX_ZELLEN) {
+    if (lane == 0) atomicAdd(&s.zaehler[SK_Z_UEBERLAUF], 1u);
+    return;
+  }
+  const int nyz = ny * nz;
+  for (int c0 = 0; c0 < nc; c0 += 64) {
+    const int c = c0 + lane;
+    uint32_t anf = 0, n = 0;
+    if (c < nc) {
+      const uint32_t h = sk_streue(lo[0] + c / nyz, lo[1] + (c / nz) % ny, lo[2] + c % nz, eimer);
+      uint32_t u = s.eimer_anfang[h];
+      const uint32_t b = s.eimer_anfang[h + 1];
+      uint32_t o = b;
+      while (u < o) {
+        const uint32_t m = (u + o) >> 1;
+        if ((int64_t)s.eintraege[m] <= ab) u = m + 1;
+        else o = m;
       }
+      anf = u;
+      n = b - u;
+    }
+    uint32_t inkl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(inkl, o, 64);
+      if (lane >= o) inkl += v;
+    }
+    const uint32_t gesamt = __shfl(inkl, 63, 64);
+    const uint32_t exkl = inkl - n;
+    for (uint32_t r = 0; r < gesamt; r += 64) {
+      const uint32_t k = r + (uint32_t)lane;
+      // the cell of candidate k: the number of lanes whose inclusive sum is <= k
+      int z = 0;
+#pragma unroll
+      for (int st = 32; st > 0; st >>= 1)
+        if (__shfl(inkl, z + st - 1, 64) <= k) z += st;
+      const uint32_t a_z = __shfl(anf, z, 64), e_z = __shfl(exkl, z, 64);
+      const bool aktiv = k < gesamt;
+      uint32_t eintrag = 0;
+      int64_t cz[3] = {0, 0, 0};
+      if (aktiv) {
+        eintrag = s.eintraege[a_z + (k - e_z)];
+        const int cc = c0 + z;
+        cz[0] = lo[0] + cc / nyz;
+        cz[1] = lo[1] + (cc / nz) % ny;
+        cz[2] = lo[2] + cc % nz;
+      }
+      g(aktiv, eintrag, (const int64_t*)cz);
+    }
+  }
+}
+
+// Folds the hits of one round into a wave's running earliest hit in lane
+// order, with the rule of a single thread walking the list: a hit replaces
+// the best unless best.zeit <= zeit. All 64 lanes call; the result is the
+// same on every lane.
+__device__ __forceinline__ void sk_welle_falten(bool treffer, double zeit, uint32_t wer, bool* hat, double* best_zeit, uint32_t* best_wer) {
+  unsigned long long m = __ballot(treffer);
+  while (m) {
+    const int l = __ffsll(m) - 1;
+    const double z = __shfl(zeit, l, 64);
+    const uint32_t w = __shfl(wer, l, 64);
+    if (!*hat || !(*best_zeit <= z)) {
+      *hat = true;
+      *best_zeit = z;
+      *best_wer = w;
+    }
+    m &= m - 1ull;
+  }
 }
 
 // Exclusive scan of `anzahl` counters (werte[anzahl] = total, optionally a
@@ -565,7 +642,23 @@ __global__ void k_sk_quader_punkte(const SelbstDev* __restrict__ sp) {
   }
 }
 
-// One triangle's point-triangle pairs; schreiben = false only counts.
+// Writes the lanes' pairs of one round at basis + *z in lane order and adds
+// the round's count to *z; schreiben = false only counts. All 64 lanes call.
+template <typename R>
+__device__ __forceinline__ void sk_welle_schreiben(bool ok, bool schreiben, const R& o, R* ziel, uint32_t kapazitaet, uint32_t basis,
+                                                   uint32_t* z) {
+  const unsigned long long m = __ballot(ok);
+  if (ok && schreiben) {
+    const uint32_t pos = basis + *z + (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
+    if (pos < kapazitaet) ziel[pos] = o;
+  }
+  *z += (uint32_t)__popcll(m);
+}
+
+// One triangle's point-triangle pairs, walked by a whole wave
+// (sk_welle_je_eintrag): the pairs are numbered in the order of the
+// candidates, as a single thread walking them would number them. schreiben =
+// false only counts. All 64 lanes call with the same t.
 __device__ void sk_punkt_dreieck(const SelbstDev& s, int t, uint32_t eimer, bool schreiben, uint32_t basis, uint32_t* anzahl) {
   const uint32_t* d = s.dreiecke + 3 * t;
   const uint32_t i0 = d[0], i1 = d[1], i2 = d[2];
@@ -580,49 +673,50 @@ __device__ void sk_punkt_dreieck(const SelbstDev& s, int t, uint32_t eimer, bool
     for (int k = 0; k < 3; ++k) en[k] = s.kantenebene[s.dreieck_kanten[3 * t + k]];
     const double sg[3] = {dot(p[2] - p[0], en[0]), dot(p[0] - p[1], en[1]), dot(p[1] - p[2], en[2])};
     const double zelle = sk_zelle_naehe(s);
-    sk_je_eintrag(s, qt, zelle, eimer, [&](uint32_t v, const int64_t* c) {
-      if (sk_partner(s, v, i0) || sk_partner(s, v, i1) || sk_partner(s, v, i2)) return;
-      if (s.w[v] <= 0.0 && s.w[i0] <= 0.0 && s.w[i1] <= 0.0 && s.w[i2] <= 0.0) return;
+    auto paar = [&](uint32_t v, const int64_t* c, Naehe& o) -> bool {
+      if (sk_partner(s, v, i0) || sk_partner(s, v, i1) || sk_partner(s, v, i2)) return false;
+      if (s.w[v] <= 0.0 && s.w[i0] <= 0.0 && s.w[i1] <= 0.0 && s.w[i2] <= 0.0) return false;
       const Quader qv = s.quader[v];
-      if (!sk_ueberlappt(qv, qt) || !sk_erste_zelle(qt, qv, zelle, c)) return;
+      if (!sk_ueberlappt(qv, qt) || !sk_erste_zelle(qt, qv, zelle, c)) return false;
       const D3 xv = s.x[v];
       const double ruhe = rt + s.r_punkt[v];  // no folded allowance bands here (band_abstand)
       const double soll = ruhe + SK_ZP;
       const double sd = dot(nt, xv - p[0]);
-      if (fabs(sd) > soll) return;
+      if (fabs(sd) > soll) return false;
       for (int k = 0; k < 3; ++k)
-        if (dot(xv - p[k], en[k]) * sg[k] < 0.0) return;
+        if (dot(xv - p[k], en[k]) * sg[k] < 0.0) return false;
       const D3 g = sk_gewichte_dreieck(p[0], p[1], p[2], xv);
       const D3 cp = (p[0] * g.x + p[1] * g.y) + p[2] * g.z;
       const D3 r = xv - cp;
       const double abstand = norm(r);
-      if (abstand > soll) return;
-      if (schreiben) {
-        const uint32_t pos = basis + z;
-        if (pos < s.kapazitaet_naehe) {
-          Naehe& o = s.naehe[pos];
-          o.art = 0;
-          o.element = (uint32_t)t;
-          o.element2 = SK_KEIN;
-          o.punkte[0] = v; o.punkte[1] = i0; o.punkte[2] = i1; o.punkte[3] = i2;
-          o.gewichte[0] = 1.0; o.gewichte[1] = -g.x; o.gewichte[2] = -g.y; o.gewichte[3] = -g.z;
-          o.normale = abstand > 1e-12 ? r * (1.0 / abstand) : nt;
-          o.abstand = abstand;
-          o.ruhe = ruhe;
-        }
-      }
-      ++z;
+      if (abstand > soll) return false;
+      o.art = 0;
+      o.element = (uint32_t)t;
+      o.element2 = SK_KEIN;
+      o.punkte[0] = v; o.punkte[1] = i0; o.punkte[2] = i1; o.punkte[3] = i2;
+      o.gewichte[0] = 1.0; o.gewichte[1] = -g.x; o.gewichte[2] = -g.y; o.gewichte[3] = -g.z;
+      o.normale = abstand > 1e-12 ? r * (1.0 / abstand) : nt;
+      o.abstand = abstand;
+      o.ruhe = ruhe;
+      return true;
+    };
+    sk_welle_je_eintrag(s, qt, zelle, eimer, -1, [&](bool aktiv, uint32_t v, const int64_t* c) {
+      Naehe o;
+      const bool ok = aktiv && paar(v, c, o);
+      sk_welle_schreiben(ok, schreiben, o, s.naehe, s.kapazitaet_naehe, basis, &z);
     });
   }
   *anzahl = z;
 }
 
-__global__ void k_sk_pd(const SelbstDev* __restrict__ sp, uint32_t eimer, int schreiben) {
+// One wave per triangle.
+__global__ void __launch_bounds__(256) k_sk_pd(const SelbstDev* __restrict__ sp, uint32_t eimer, int schreiben) {
   const SelbstDev& s = *sp;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < s.nt; t += gridDim.x * blockDim.x) {
+  const int wellen = gridDim.x * (blockDim.x >> 6);
+  for (int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < s.nt; t += wellen) {
     uint32_t z;
     sk_punkt_dreieck(s, t, eimer, schreiben != 0, schreiben ? s.zahl[t] : 0u, &z);
-    if (!schreiben) s.zahl[t] = z;
+    if (!schreiben && (threadIdx.x & 63) == 0) s.zahl[t] = z;
   }
 }
 
@@ -649,6 +743,8 @@ __device__ __forceinline__ bool sk_ist_kuerzeste(const SelbstDev& s, uint32_t e,
   return true;
 }
 
+// One edge's edge-edge pairs with the edges f > e, walked by a whole wave as in
+// sk_punkt_dreieck. All 64 lanes call with the same e.
 __device__ void sk_kante_kante(const SelbstDev& s, int e, uint32_t eimer, bool schreiben, uint32_t basis, uint32_t* anzahl) {
   const uint32_t a0 = s.kanten[2 * e], a1 = s.kanten[2 * e + 1];
   uint32_t z = 0;
@@ -656,62 +752,63 @@ __device__ void sk_kante_kante(const SelbstDev& s, int e, uint32_t eimer, bool s
     const D3 pa0 = s.x[a0], pa1 = s.x[a1];
     const Quader qe = s.quader[e];
     const double zelle = sk_zelle_naehe(s);
-    sk_je_eintrag(s, qe, zelle, eimer, [&](uint32_t f, const int64_t* c) {
-      if (f <= (uint32_t)e) return;
+    auto paar = [&](uint32_t f, const int64_t* c, Naehe& o) -> bool {
       const Quader qf = s.quader[f];
-      if (!sk_ueberlappt(qe, qf) || !sk_erste_zelle(qe, qf, zelle, c)) return;
+      if (!sk_ueberlappt(qe, qf) || !sk_erste_zelle(qe, qf, zelle, c)) return false;
       const uint32_t b0 = s.kanten[2 * f], b1 = s.kanten[2 * f + 1];
-      if (sk_partner(s, a0, b0) || sk_partner(s, a1, b0) || sk_partner(s, a0, b1) || sk_partner(s, a1, b1)) return;
-      if (s.w[a0] <= 0.0 && s.w[a1] <= 0.0 && s.w[b0] <= 0.0 && s.w[b1] <= 0.0) return;
+      if (sk_partner(s, a0, b0) || sk_partner(s, a1, b0) || sk_partner(s, a0, b1) || sk_partner(s, a1, b1)) return false;
+      if (s.w[a0] <= 0.0 && s.w[a1] <= 0.0 && s.w[b0] <= 0.0 && s.w[b1] <= 0.0) return false;
       const D3 pb0 = s.x[b0], pb1 = s.x[b1];
       const D3 da = pa1 - pa0, db = pb1 - pb0;
       const double la = norm(da), lb = norm(db);
-      if (la < SK_KANTE_MIN || lb < SK_KANTE_MIN) return;
+      if (la < SK_KANTE_MIN || lb < SK_KANTE_MIN) return false;
       const D3 ua = da * (1.0 / la), ub = db * (1.0 / lb);
       const double cw = dot(ua, ub);
-      if (fabs(cw) > 0.99999) return;
+      if (fabs(cw) > 0.99999) return false;
       const D3 r = pb0 - pa0;
       const double pp = dot(r, ua), qq = -dot(r, ub);
       const double nen = 1.0 - cw * cw;
       const double sa = (pp + cw * qq) / nen;
       const double sb = (qq + cw * pp) / nen;
-      if (!(sa >= 0.0 && sa <= la) || !(sb >= 0.0 && sb <= lb)) return;
+      if (!(sa >= 0.0 && sa <= la) || !(sb >= 0.0 && sb <= lb)) return false;
       const D3 pa = pa0 + ua * sa;
       const D3 pb = pb0 + ub * sb;
       const D3 dv = pa - pb;
       const double abstand = norm(dv);
       const double ruhe = s.r_kante[e] + s.r_kante[f];
-      if (abstand > ruhe + SK_ZP) return;
+      if (abstand > ruhe + SK_ZP) return false;
       const D3 n = sk_einheit(dv);
-      if (!sk_ist_kuerzeste(s, f, ub, n) || !sk_ist_kuerzeste(s, (uint32_t)e, ua, n * -1.0)) return;
-      if (schreiben) {
-        const uint32_t pos = basis + z;
-        if (pos < s.kapazitaet_naehe) {
-          const double sv = sa / la, tv = sb / lb;
-          Naehe& o = s.naehe[pos];
-          o.art = 1;
-          o.element = (uint32_t)e;
-          o.element2 = f;
-          o.punkte[0] = a0; o.punkte[1] = a1; o.punkte[2] = b0; o.punkte[3] = b1;
-          o.gewichte[0] = 1.0 - sv; o.gewichte[1] = sv; o.gewichte[2] = -(1.0 - tv); o.gewichte[3] = -tv;
-          o.normale = n;
-          o.abstand = abstand;
-          o.ruhe = ruhe;
-        }
-      }
-      ++z;
+      if (!sk_ist_kuerzeste(s, f, ub, n) || !sk_ist_kuerzeste(s, (uint32_t)e, ua, n * -1.0)) return false;
+      const double sv = sa / la, tv = sb / lb;
+      o.art = 1;
+      o.element = (uint32_t)e;
+      o.element2 = f;
+      o.punkte[0] = a0; o.punkte[1] = a1; o.punkte[2] = b0; o.punkte[3] = b1;
+      o.gewichte[0] = 1.0 - sv; o.gewichte[1] = sv; o.gewichte[2] = -(1.0 - tv); o.gewichte[3] = -tv;
+      o.normale = n;
+      o.abstand = abstand;
+      o.ruhe = ruhe;
+      return true;
+    };
+    // the partners f <= e, which the reference skips first, are left out of the walk
+    sk_welle_je_eintrag(s, qe, zelle, eimer, (int64_t)e, [&](bool aktiv, uint32_t f, const int64_t* c) {
+      Naehe o;
+      const bool ok = aktiv && paar(f, c, o);
+      sk_welle_schreiben(ok, schreiben, o, s.naehe, s.kapazitaet_naehe, basis, &z);
     });
   }
   *anzahl = z;
 }
 
-__global__ void k_sk_kk(const SelbstDev* __restrict__ sp, uint32_t eimer, int schreiben) {
+// One wave per edge.
+__global__ void __launch_bounds__(256) k_sk_kk(const SelbstDev* __restrict__ sp, uint32_t eimer, int schreiben) {
   const SelbstDev& s = *sp;
   const uint32_t basis0 = s.zaehler[SK_Z_NAEHE_PD];
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < s.ne; e += gridDim.x * blockDim.x) {
+  const int wellen = gridDim.x * (blockDim.x >> 6);
+  for (int e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < s.ne; e += wellen) {
     uint32_t z;
     sk_kante_kante(s, e, eimer, schreiben != 0, schreiben ? basis0 + s.zahl[e] : 0u, &z);
-    if (!schreiben) s.zahl[e] = z;
+    if (!schreiben && (threadIdx.x & 63) == 0) s.zahl[e] = z;
   }
 }
 
@@ -898,6 +995,8 @@ __global__ void __launch_bounds__(1024) k_sk_laengste(const SelbstDev* __restric
   if (threadIdx.x == 0) *aus = fmax(m[0], untergrenze);
 }
 
+// One edge's cuts through triangles, walked by a whole wave as in
+// sk_punkt_dreieck. All 64 lanes call with the same e.
 __device__ void sk_schnitte_kante(const SelbstDev& s, int e, double zelle, uint32_t eimer, bool schreiben, uint32_t basis,
                                   uint32_t* anzahl) {
   const uint32_t a0 = s.kanten[2 * e], a1 = s.kanten[2 * e + 1];
@@ -909,7 +1008,7 @@ __device__ void sk_schnitte_kante(const SelbstDev& s, int e, double zelle, uint3
     sk_quader_rein(qe, pa, SK_ZQ);
     sk_quader_rein(qe, pb, SK_ZQ);
     const D3 ehat = sk_einheit(pb - pa);
-    sk_je_eintrag(s, qe, zelle, eimer, [&](uint32_t t, const int64_t* c) {
+    auto schnitt = [&](uint32_t t, const int64_t* c, Schnitt& o) -> bool {
       const uint32_t* d = s.dreiecke + 3 * t;
       bool nachbar;
       if (s.an_naehten) {
@@ -919,13 +1018,13 @@ __device__ void sk_schnitte_kante(const SelbstDev& s, int e, double zelle, uint3
         for (int k = 0; k < 3; ++k) nachbar = nachbar || sk_partner(s, a0, d[k]) || sk_partner(s, a1, d[k]);
       }
       const Quader qt = s.quader[t];
-      if (nachbar || !sk_ueberlappt(qt, qe) || !sk_erste_zelle(qe, qt, zelle, c)) return;
+      if (nachbar || !sk_ueberlappt(qt, qe) || !sk_erste_zelle(qe, qt, zelle, c)) return false;
       const D3 n = s.n_dreieck[t];
       const D3 v[3] = {s.x[d[0]], s.x[d[1]], s.x[d[2]]};
       double tk, w[3];
-      if (!sk_kante_durch_dreieck(pa, pb, v, n, &tk, w)) return;
+      if (!sk_kante_durch_dreieck(pa, pb, v, n, &tk, w)) return false;
       const double cn = dot(n, ehat);
-      if (fabs(cn) < 1e-6) return;
+      if (fabs(cn) < 1e-6) return false;
       if (schreiben) {
         const D3 p = sk_lerp(pa, pb, tk);
         D3 g = zero3<double>();
@@ -945,30 +1044,34 @@ __device__ void sk_schnitte_kante(const SelbstDev& s, int e, double zelle, uint3
           g = g + (m - n * (dot(ehat, m) / cn));
         }
         const double lg = norm(g);
-        const uint32_t pos = basis + z;
-        if (pos < s.kapazitaet_schnitte) {
-          Schnitt& o = s.schnitte[pos];
-          o.kante = (uint32_t)e;
-          o.dreieck = t;
-          o.t = tk;
-          o.w[0] = w[0]; o.w[1] = w[1]; o.w[2] = w[2];
-          o.richtung = sk_einheit(g);
-          o.gewicht = lg / sqrt(lg * lg + 2.25);
-        }
+        o.kante = (uint32_t)e;
+        o.dreieck = t;
+        o.t = tk;
+        o.w[0] = w[0]; o.w[1] = w[1]; o.w[2] = w[2];
+        o.richtung = sk_einheit(g);
+        o.gewicht = lg / sqrt(lg * lg + 2.25);
       }
-      ++z;
+      return true;
+    };
+    sk_welle_je_eintrag(s, qe, zelle, eimer, -1, [&](bool aktiv, uint32_t t, const int64_t* c) {
+      Schnitt o;
+      const bool ok = aktiv && schnitt(t, c, o);
+      sk_welle_schreiben(ok, schreiben, o, s.schnitte, s.kapazitaet_schnitte, basis, &z);
     });
   }
   *anzahl = z;
 }
 
-__global__ void k_sk_schnitte(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer, int schreiben) {
+// One wave per edge.
+__global__ void __launch_bounds__(256) k_sk_schnitte(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer,
+                                                     int schreiben) {
   const SelbstDev& s = *sp;
   const double zelle = *zelle_p;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < s.ne; e += gridDim.x * blockDim.x) {
+  const int wellen = gridDim.x * (blockDim.x >> 6);
+  for (int e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < s.ne; e += wellen) {
     uint32_t z;
     sk_schnitte_kante(s, e, zelle, eimer, schreiben != 0, schreiben ? s.zahl[e] : 0u, &z);
-    if (!schreiben) s.zahl[e] = z;
+    if (!schreiben && (threadIdx.x & 63) == 0) s.zahl[e] = z;
   }
 }
 
@@ -1281,102 +1384,166 @@ __global__ void k_sk_quader_weg_kanten(const SelbstDev* __restrict__ sp) {
   }
 }
 
-// Earliest point through each triangle; slot t of the impact array.
-__global__ void k_sk_stoss_pd(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer) {
+// Point v through triangle d on the path of the step (:1201-1231): the time
+// the four points are coplanar and the barycentric position at that time.
+// With hat, a time not before `grenze` is rejected before the position test.
+__device__ __forceinline__ bool sk_pd_zeit(const SelbstDev& s, const uint32_t* d, uint32_t v, bool hat, double grenze, double* zeit,
+                                           double* uv) {
+  const uint32_t idx[4] = {d[0], d[1], d[2], v};
+  if (s.w[idx[0]] <= 0.0 && s.w[idx[1]] <= 0.0 && s.w[idx[2]] <= 0.0 && s.w[idx[3]] <= 0.0) return false;
+  D3 a0[4], a1[4];
+  for (int k = 0; k < 4; ++k) { a0[k] = s.x_anfang[idx[k]]; a1[k] = s.x[idx[k]]; }
+  if (!sk_koplanar_zeit(a0, a1, SK_STOSS * 0.1, zeit)) return false;
+  if (hat && grenze <= *zeit) return false;
+  D3 q[4];
+  for (int k = 0; k < 4; ++k) q[k] = sk_lerp(a0[k], a1[k], *zeit);
+  return sk_im_dreieck(q[0], q[1], q[2], q[3], uv);
+}
+// The impact record of the hit of point v through triangle t.
+__device__ __forceinline__ void sk_pd_satz(const SelbstDev& s, int t, const uint32_t* d, uint32_t v, Stoss* out) {
+  double zeit, uv[2];
+  sk_pd_zeit(s, d, v, false, 0.0, &zeit, uv);
+  const uint32_t idx[4] = {d[0], d[1], d[2], v};
+  D3 a0[4], a1[4], q[4];
+  for (int k = 0; k < 4; ++k) {
+    a0[k] = s.x_anfang[idx[k]];
+    a1[k] = s.x[idx[k]];
+    q[k] = sk_lerp(a0[k], a1[k], zeit);
+  }
+  const D3 n = sk_einheit(cross(q[1] - q[0], q[2] - q[0]));
+  const double g[3] = {1.0 - uv[0] - uv[1], uv[0], uv[1]};
+  const D3 rel = (a1[3] - a0[3]) - (((a1[0] - a0[0]) * g[0] + (a1[1] - a0[1]) * g[1]) + (a1[2] - a0[2]) * g[2]);
+  const double seite = dot(rel, n) <= 0.0 ? 1.0 : -1.0;
+  Stoss best;
+  best.art = 0;
+  best.element = (uint32_t)t;
+  best.element2 = SK_KEIN;
+  best.punkte[0] = v; best.punkte[1] = d[0]; best.punkte[2] = d[1]; best.punkte[3] = d[2];
+  best.lage[0] = uv[0];
+  best.lage[1] = uv[1];
+  best.zeit = zeit;
+  best.seite = seite;
+  best.normale = n * seite;
+  *out = best;
+}
+
+// Earliest point through each triangle; slot t of the impact array. One wave
+// per triangle walks the point hash (sk_welle_je_eintrag) and folds its hits
+// in candidate order, so the impact kept is the one a single thread walking
+// the candidates in the reference's order keeps.
+__global__ void __launch_bounds__(256) k_sk_stoss_pd(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer) {
   const SelbstDev& s = *sp;
   const double zelle = *zelle_p;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < s.nt; t += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int wellen = gridDim.x * (blockDim.x >> 6);
+  for (int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < s.nt; t += wellen) {
     const uint32_t* d = s.dreiecke + 3 * t;
-    Stoss best;
-    best.art = SK_KEIN;
-    s.zahl[t] = 0u;
+    bool hat = false;
+    double best_zeit = 0.0;
+    uint32_t best_v = 0;
     if (!(d[0] == d[1] || d[1] == d[2] || d[0] == d[2])) {
       Quader qt = sk_quader_leer();
       for (int k = 0; k < 3; ++k) sk_quader_rein(qt, s.x_anfang[d[k]], 1e-8);
       for (int k = 0; k < 3; ++k) sk_quader_rein(qt, s.x[d[k]], 1e-8);
-      sk_je_eintrag(s, qt, zelle, eimer, [&](uint32_t v, const int64_t* c) {
-        if (sk_partner(s, v, d[0]) || sk_partner(s, v, d[1]) || sk_partner(s, v, d[2])) return;
-        const Quader qv = s.quader[v];
-        if (!sk_ueberlappt(qv, qt) || !sk_erste_zelle(qt, qv, zelle, c)) return;
-        const uint32_t idx[4] = {d[0], d[1], d[2], v};
-        if (s.w[idx[0]] <= 0.0 && s.w[idx[1]] <= 0.0 && s.w[idx[2]] <= 0.0 && s.w[idx[3]] <= 0.0) return;
-        D3 a0[4], a1[4];
-        for (int k = 0; k < 4; ++k) { a0[k] = s.x_anfang[idx[k]]; a1[k] = s.x[idx[k]]; }
-        double zeit;
-        if (!sk_koplanar_zeit(a0, a1, SK_STOSS * 0.1, &zeit)) return;
-        if (best.art != SK_KEIN && best.zeit <= zeit) return;
-        D3 q[4];
-        for (int k = 0; k < 4; ++k) q[k] = sk_lerp(a0[k], a1[k], zeit);
-        double uv[2];
-        if (!sk_im_dreieck(q[0], q[1], q[2], q[3], uv)) return;
-        const D3 n = sk_einheit(cross(q[1] - q[0], q[2] - q[0]));
-        const double g[3] = {1.0 - uv[0] - uv[1], uv[0], uv[1]};
-        const D3 rel = (a1[3] - a0[3]) - (((a1[0] - a0[0]) * g[0] + (a1[1] - a0[1]) * g[1]) + (a1[2] - a0[2]) * g[2]);
-        const double seite = dot(rel, n) <= 0.0 ? 1.0 : -1.0;
-        best.art = 0;
-        best.element = (uint32_t)t;
-        best.element2 = SK_KEIN;
-        best.punkte[0] = v; best.punkte[1] = d[0]; best.punkte[2] = d[1]; best.punkte[3] = d[2];
-        best.lage[0] = uv[0];
-        best.lage[1] = uv[1];
-        best.zeit = zeit;
-        best.seite = seite;
-        best.normale = n * seite;
+      sk_welle_je_eintrag(s, qt, zelle, eimer, -1, [&](bool aktiv, uint32_t v, const int64_t* c) {
+        bool ok = false;
+        double zeit = 0.0;
+        if (aktiv && !(sk_partner(s, v, d[0]) || sk_partner(s, v, d[1]) || sk_partner(s, v, d[2]))) {
+          const Quader qv = s.quader[v];
+          if (sk_ueberlappt(qv, qt) && sk_erste_zelle(qt, qv, zelle, c)) {
+            double uv[2];
+            ok = sk_pd_zeit(s, d, v, hat, best_zeit, &zeit, uv);
+          }
+        }
+        sk_welle_falten(ok, zeit, v, &hat, &best_zeit, &best_v);
       });
     }
-    if (best.art != SK_KEIN) {
-      s.stoesse[t] = best;
-      s.zahl[t] = 1u;
+    if (lane == 0) {
+      if (hat) sk_pd_satz(s, t, d, best_v, &s.stoesse[t]);
+      s.zahl[t] = hat ? 1u : 0u;
     }
   }
 }
 
-// Earliest partner edge of each edge; slot nt + e of the impact array.
-__global__ void k_sk_stoss_kk(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer) {
+// Edges (i0, i1) and (j0, j1) crossing on the path of the step (:1233-1262):
+// the coplanarity time and the two edge parameters at that time. With hat, a
+// time not before `grenze` is rejected before the crossing test.
+__device__ __forceinline__ bool sk_kk_zeit(const SelbstDev& s, uint32_t i0, uint32_t i1, uint32_t j0, uint32_t j1, bool hat,
+                                           double grenze, double* zeit, double* sw) {
+  const uint32_t idx[4] = {i1, j0, j1, i0};
+  if (s.w[idx[0]] <= 0.0 && s.w[idx[1]] <= 0.0 && s.w[idx[2]] <= 0.0 && s.w[idx[3]] <= 0.0) return false;
+  D3 a0[4], a1[4];
+  for (int k = 0; k < 4; ++k) { a0[k] = s.x_anfang[idx[k]]; a1[k] = s.x[idx[k]]; }
+  if (!sk_koplanar_zeit(a0, a1, SK_STOSS * 0.1, zeit)) return false;
+  if (hat && grenze <= *zeit) return false;
+  D3 q[4];
+  for (int k = 0; k < 4; ++k) q[k] = sk_lerp(a0[k], a1[k], *zeit);
+  return sk_kanten_kreuzen(q[3], q[0], q[1], q[2], sw);
+}
+// The impact record of the hit of edge i with edge j.
+__device__ __forceinline__ void sk_kk_satz(const SelbstDev& s, int i, uint32_t j, Stoss* out) {
+  const uint32_t i0 = s.kanten[2 * i], i1 = s.kanten[2 * i + 1];
+  const uint32_t j0 = s.kanten[2 * j], j1 = s.kanten[2 * j + 1];
+  double zeit, sw[2];
+  sk_kk_zeit(s, i0, i1, j0, j1, false, 0.0, &zeit, sw);
+  const uint32_t idx[4] = {i1, j0, j1, i0};
+  D3 a0[4], a1[4], q[4];
+  for (int k = 0; k < 4; ++k) {
+    a0[k] = s.x_anfang[idx[k]];
+    a1[k] = s.x[idx[k]];
+    q[k] = sk_lerp(a0[k], a1[k], zeit);
+  }
+  const D3 n = sk_einheit(cross(q[0] - q[3], q[2] - q[1]));
+  const D3 auf_i = (a1[3] - a0[3]) * (1.0 - sw[0]) + (a1[0] - a0[0]) * sw[0];
+  const D3 auf_j = (a1[1] - a0[1]) * (1.0 - sw[1]) + (a1[2] - a0[2]) * sw[1];
+  const double seite = dot(auf_i - auf_j, n) <= 0.0 ? 1.0 : -1.0;
+  Stoss best;
+  best.art = 1;
+  best.element = (uint32_t)i;
+  best.element2 = j;
+  best.punkte[0] = i0; best.punkte[1] = i1; best.punkte[2] = j0; best.punkte[3] = j1;
+  best.lage[0] = sw[0];
+  best.lage[1] = sw[1];
+  best.zeit = zeit;
+  best.seite = seite;
+  best.normale = n * seite;
+  *out = best;
+}
+
+// Earliest partner edge j > i of each edge i; slot nt + i of the impact array.
+// One wave per edge, as in k_sk_stoss_pd; the partners j <= i, which the
+// reference skips first, are left out of the walk.
+__global__ void __launch_bounds__(256) k_sk_stoss_kk(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer) {
   const SelbstDev& s = *sp;
   const double zelle = *zelle_p;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.ne; i += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int wellen = gridDim.x * (blockDim.x >> 6);
+  for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < s.ne; i += wellen) {
     const uint32_t i0 = s.kanten[2 * i], i1 = s.kanten[2 * i + 1];
-    Stoss best;
-    best.art = SK_KEIN;
-    s.zahl[s.nt + i] = 0u;
+    bool hat = false;
+    double best_zeit = 0.0;
+    uint32_t best_j = 0;
     if (i0 != i1) {
       const Quader qi = s.quader[i];
-      sk_je_eintrag(s, qi, zelle, eimer, [&](uint32_t j, const int64_t* c) {
-        if (j <= (uint32_t)i) return;
-        const Quader qj = s.quader[j];
-        if (!sk_ueberlappt(qi, qj) || !sk_erste_zelle(qi, qj, zelle, c)) return;
-        const uint32_t j0 = s.kanten[2 * j], j1 = s.kanten[2 * j + 1];
-        if (sk_partner(s, j0, i0) || sk_partner(s, j0, i1) || sk_partner(s, j1, i0) || sk_partner(s, j1, i1)) return;
-        const uint32_t idx[4] = {i1, j0, j1, i0};
-        if (s.w[idx[0]] <= 0.0 && s.w[idx[1]] <= 0.0 && s.w[idx[2]] <= 0.0 && s.w[idx[3]] <= 0.0) return;
-        D3 a0[4], a1[4];
-        for (int k = 0; k < 4; ++k) { a0[k] = s.x_anfang[idx[k]]; a1[k] = s.x[idx[k]]; }
-        double zeit;
-        if (!sk_koplanar_zeit(a0, a1, SK_STOSS * 0.1, &zeit)) return;
-        if (best.art != SK_KEIN && best.zeit <= zeit) return;
-        D3 q[4];
-        for (int k = 0; k < 4; ++k) q[k] = sk_lerp(a0[k], a1[k], zeit);
-        double sw[2];
-        if (!sk_kanten_kreuzen(q[3], q[0], q[1], q[2], sw)) return;
-        const D3 n = sk_einheit(cross(q[0] - q[3], q[2] - q[1]));
-        const D3 auf_i = (a1[3] - a0[3]) * (1.0 - sw[0]) + (a1[0] - a0[0]) * sw[0];
-        const D3 auf_j = (a1[1] - a0[1]) * (1.0 - sw[1]) + (a1[2] - a0[2]) * sw[1];
-        const double seite = dot(auf_i - auf_j, n) <= 0.0 ? 1.0 : -1.0;
-        best.art = 1;
-        best.element = (uint32_t)i;
-        best.element2 = j;
-        best.punkte[0] = i0; best.punkte[1] = i1; best.punkte[2] = j0; best.punkte[3] = j1;
-        best.lage[0] = sw[0];
-        best.lage[1] = sw[1];
-        best.zeit = zeit;
-        best.seite = seite;
-        best.normale = n * seite;
+      sk_welle_je_eintrag(s, qi, zelle, eimer, (int64_t)i, [&](bool aktiv, uint32_t j, const int64_t* c) {
+        bool ok = false;
+        double zeit = 0.0;
+        if (aktiv) {
+          const Quader qj = s.quader[j];
+          if (sk_ueberlappt(qi, qj) && sk_erste_zelle(qi, qj, zelle, c)) {
+            const uint32_t j0 = s.kanten[2 * j], j1 = s.kanten[2 * j + 1];
+            if (!(sk_partner(s, j0, i0) || sk_partner(s, j0, i1) || sk_partner(s, j1, i0) || sk_partner(s, j1, i1))) {
+              double sw[2];
+              ok = sk_kk_zeit(s, i0, i1, j0, j1, hat, best_zeit, &zeit, sw);
+            }
+          }
+        }
+        sk_welle_falten(ok, zeit, j, &hat, &best_zeit, &best_j);
       });
     }
-    if (best.art != SK_KEIN) {
-      s.stoesse[s.nt + i] = best;
-      s.zahl[s.nt + i] = 1u;
+    if (lane == 0) {
+      if (hat) sk_kk_satz(s, i, best_j, &s.stoesse[s.nt + i]);
+      s.zahl[s.nt + i] = hat ? 1u : 0u;
     }
   }
 }

@@ -659,6 +659,11 @@ static unsigned sk_gitter(long count) {
   long b = (count + BUILD_THREADS - 1) / BUILD_THREADS;
   return (unsigned)std::max(1L, std::min(b, 4096L));
 }
+// blocks of BUILD_THREADS for kernels that give each item a whole wave
+static unsigned sk_gitter_wellen(long count) {
+  long b = (count * 64 + BUILD_THREADS - 1) / BUILD_THREADS;
+  return (unsigned)std::max(1L, std::min(b, 4096L));
+}
 
 // Exclusive scan of werte[0..anzahl) in place, total into werte[anzahl]
 // (and into kopie[0..anzahl] and *gesamt_ziel when given), over
@@ -697,15 +702,15 @@ static int launch_selbst_bau(gc_ctx* c) {
   // point-triangle (:400-467)
   hipLaunchKernelGGL(k_sk_quader_punkte, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.n, c->sk_zellen, c->sk_eimer_p);
-  hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 0);
+  hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter_wellen(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 0);
   sk_scan(c, k.zahl, k.nt, nullptr, k.zaehler + SK_Z_NAEHE_PD);
-  hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 1);
+  hipLaunchKernelGGL(k_sk_pd, dim3(sk_gitter_wellen(k.nt)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_p, 1);
   // edge-edge (:506-592)
   hipLaunchKernelGGL(k_sk_quader_kanten, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.ne, c->sk_zellen, c->sk_eimer_k);
-  hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 0);
+  hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter_wellen(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 0);
   sk_scan(c, k.zahl, k.ne, nullptr, k.zaehler + SK_Z_NAEHE_KK);
-  hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 1);
+  hipLaunchKernelGGL(k_sk_kk, dim3(sk_gitter_wellen(k.ne)), dim3(BUILD_THREADS), 0, st, sp, c->sk_eimer_k, 1);
   hipLaunchKernelGGL(k_sk_naehe_summe, dim3(1), dim3(64), 0, st, sp);
   // springs and the damping of the approach (:713-787)
   hipLaunchKernelGGL(k_sk_federn, dim3(sk_gitter(k.kapazitaet_naehe)), dim3(BUILD_THREADS), 0, st, sp);
@@ -714,10 +719,10 @@ static int launch_selbst_bau(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_quader_dreiecke, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp);
   hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 1);
   sk_hash(c, k.nt, c->sk_zellen + 1, c->sk_eimer_d);
-  hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
+  hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter_wellen(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
                      c->sk_eimer_d, 0);
   sk_scan(c, k.zahl, k.ne, nullptr, k.zaehler + SK_Z_SCHNITTE);
-  hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
+  hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter_wellen(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
                      c->sk_eimer_d, 1);
   hipLaunchKernelGGL(k_sk_schnitte_summe, dim3(1), dim3(64), 0, st, sp);
   hipLaunchKernelGGL(k_sk_gebiete, dim3(1), dim3(64), 0, st, sp);
@@ -752,11 +757,11 @@ static int launch_selbst_stoss(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 2);
   hipLaunchKernelGGL(k_sk_quader_weg_punkte, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.n, c->sk_zellen + 2, c->sk_eimer_p);
-  hipLaunchKernelGGL(k_sk_stoss_pd, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
+  hipLaunchKernelGGL(k_sk_stoss_pd, dim3(sk_gitter_wellen(k.nt)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
                      c->sk_eimer_p);
   hipLaunchKernelGGL(k_sk_quader_weg_kanten, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.ne, c->sk_zellen + 2, c->sk_eimer_k);
-  hipLaunchKernelGGL(k_sk_stoss_kk, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
+  hipLaunchKernelGGL(k_sk_stoss_kk, dim3(sk_gitter_wellen(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
                      c->sk_eimer_k);
   hipLaunchKernelGGL(k_sk_stoss_filter, dim3(sk_gitter(k.nt + k.ne)), dim3(BUILD_THREADS), 0, st, sp);
   hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, k.st_off, k.n + 1);
