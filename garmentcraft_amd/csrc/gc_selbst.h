@@ -340,17 +340,26 @@ template <typename F>
 __device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, F f) {
   int64_t lo[3], hi[3];
   if (!sk_zellen(s, q, zelle, lo, hi)) return;
-  const int64_t nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-  const int64_t nc = nx * ny * nz;
-  for (int64_t c = 0; c < nc; ++c) {
-    const int64_t i = lo[0] + c / (ny * nz), j = lo[1] + (c / nz) % ny, k = lo[2] + c % nz;
-    const uint32_t h = sk_streue(i, j, k, eimer);
+  // at most SK_MAX_ZELLEN cells: the offsets are ints, stepped k fastest
+  const int nc = (int)((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
+  const int ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
+  int i = 0, j = 0, k = 0;
+  for (int c = 0; c < nc; ++c) {
+    const uint32_t h = sk_streue(lo[0] + i, lo[1] + j, lo[2] + k, eimer);
     bool neu = true;
-    for (int64_t d = 0; d < c && neu; ++d) {
-      const int64_t i2 = lo[0] + d / (ny * nz), j2 = lo[1] + (d / nz) % ny, k2 = lo[2] + d % nz;
-      neu = sk_streue(i2, j2, k2, eimer) != h;
+    int i2 = 0, j2 = 0, k2 = 0;
+    for (int d = 0; d < c && neu; ++d) {
+      neu = sk_streue(lo[0] + i2, lo[1] + j2, lo[2] + k2, eimer) != h;
+      if (++k2 == nz) {
+        k2 = 0;
+        if (++j2 == ny) { j2 = 0; ++i2; }
+      }
     }
     if (neu) f(h);
+    if (++k == nz) {
+      k = 0;
+      if (++j == ny) { j = 0; ++i; }
+    }
   }
 }
 
@@ -981,18 +990,28 @@ __global__ void k_sk_quader_dreiecke(const SelbstDev* __restrict__ sp) {
 }
 
 // Longest edge at positions xs (one block; `aus` gets max(l, untergrenze)).
-__global__ void __launch_bounds__(1024) k_sk_laengste(const SelbstDev* __restrict__ sp, const V3<double>* xs, double untergrenze, double* aus) {
+// Longest edge, at least `untergrenze`: *aus is set to untergrenze first
+// (k_sk_laengste_start), then every block folds its maximum in with an atomic
+// max on the bits. The values are non-negative and never NaN (fmax drops a NaN
+// length, as the serial fold does), and for such doubles the order of the bit
+// patterns is the order of the values, so the result does not depend on the
+// order of the blocks.
+__global__ void k_sk_laengste_start(double untergrenze, double* aus) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *aus = untergrenze;
+}
+__global__ void __launch_bounds__(256) k_sk_laengste(const SelbstDev* __restrict__ sp, const V3<double>* xs, double* aus) {
   const SelbstDev& s = *sp;
-  __shared__ double m[1024];
+  __shared__ double m[256];
   double l = 0.0;
-  for (int e = threadIdx.x; e < s.ne; e += blockDim.x) l = fmax(l, norm(xs[s.kanten[2 * e]] - xs[s.kanten[2 * e + 1]]));
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < s.ne; e += gridDim.x * blockDim.x)
+    l = fmax(l, norm(xs[s.kanten[2 * e]] - xs[s.kanten[2 * e + 1]]));
   m[threadIdx.x] = l;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) m[threadIdx.x] = fmax(m[threadIdx.x], m[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *aus = fmax(m[0], untergrenze);
+  if (threadIdx.x == 0) atomicMax((unsigned long long*)aus, (unsigned long long)__double_as_longlong(m[0]));
 }
 
 // One edge's cuts through triangles, walked by a whole wave as in

@@ -717,7 +717,8 @@ static int launch_selbst_bau(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_daempfung, dim3(1), dim3(1024), 0, st, sp);
   // cuts, regions, push springs (:858-1059)
   hipLaunchKernelGGL(k_sk_quader_dreiecke, dim3(sk_gitter(k.nt)), dim3(BUILD_THREADS), 0, st, sp);
-  hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 1);
+  hipLaunchKernelGGL(k_sk_laengste_start, dim3(1), dim3(64), 0, st, 1e-3, c->sk_zellen + 1);
+  hipLaunchKernelGGL(k_sk_laengste, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const V3<double>*)k.x, c->sk_zellen + 1);
   sk_hash(c, k.nt, c->sk_zellen + 1, c->sk_eimer_d);
   hipLaunchKernelGGL(k_sk_schnitte, dim3(sk_gitter_wellen(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 1),
                      c->sk_eimer_d, 0);
@@ -754,7 +755,8 @@ static int launch_selbst_stoss(gc_ctx* c) {
   const SelbstDev& k = c->sk;
   const SysDev<T>* syp = dsys<T>(c);
   hipLaunchKernelGGL(k_sk_null, dim3(1), dim3(64), 0, st, k.zaehler + SK_Z_STOSS_ZAHL, 1);
-  hipLaunchKernelGGL(k_sk_laengste, dim3(1), dim3(1024), 0, st, sp, (const V3<double>*)k.x, 1e-3, c->sk_zellen + 2);
+  hipLaunchKernelGGL(k_sk_laengste_start, dim3(1), dim3(64), 0, st, 1e-3, c->sk_zellen + 2);
+  hipLaunchKernelGGL(k_sk_laengste, dim3(sk_gitter(k.ne)), dim3(BUILD_THREADS), 0, st, sp, (const V3<double>*)k.x, c->sk_zellen + 2);
   hipLaunchKernelGGL(k_sk_quader_weg_punkte, dim3(sk_gitter(k.n)), dim3(BUILD_THREADS), 0, st, sp);
   sk_hash(c, k.n, c->sk_zellen + 2, c->sk_eimer_p);
   hipLaunchKernelGGL(k_sk_stoss_pd, dim3(sk_gitter_wellen(k.nt)), dim3(BUILD_THREADS), 0, st, sp, (const double*)(c->sk_zellen + 2),
