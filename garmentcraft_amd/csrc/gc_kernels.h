@@ -13,6 +13,23 @@
 // CSR entry: constraint index << 2 | slot. Kante: slot 0 = first point (+), 1 = second (-).
 #define GC_KAT 4  // categories per point in the offset table: kanten, biege, membran, (end)
 
+// Operator records of one tile entry, in tile order (see SysDev::kachel_kante).
+template <typename T>
+struct KanteK {
+  V3<T> n;
+  T q, kk;
+};
+template <typename T>
+struct BiegeK {
+  V3<T> g[4];
+  T kt;
+};
+template <typename T>
+struct MembranK {
+  Membran<T> el;
+  Hessen<T> h;
+};
+
 template <typename T>
 struct SysDev {
   int32_t n, nk, nb, nm;
@@ -90,6 +107,13 @@ struct SysDev {
   const uint32_t* kachel_v_off;   // point-list offsets (n_kacheln + 1)
   const uint32_t* kachel_v;       // points touched by the tile's constraints, global index
   uint32_t kachel_slots_max;      // LDS slot area (entries) of the largest tile
+  // the records of every tile entry at its entry index, copied from the
+  // per-constraint arrays after each build (k_kachel_saetze), so stage B
+  // loads them directly instead of through the constraint index; an entry
+  // uses the array of its category only
+  KanteK<T>* kachel_kante;
+  BiegeK<T>* kachel_biege;
+  MembranK<T>* kachel_membran;
 
   // springs of cloth self-collision, rebuilt every step (counts on the
   // device; empty lists when it is off): per point its spring pairs
@@ -584,6 +608,34 @@ __global__ void k_reset(GridState* gs, Slot* slots) {
 // release pass precedes.
 __global__ void k_barriere_leeren(Slot* slots) { barriere_leeren(slots); }
 
+// Copies every tile entry's operator record from the per-constraint arrays to
+// its entry index (after each build; the records are constant during a solve).
+template <typename T>
+__global__ void k_kachel_saetze(const SysDev<T>* __restrict__ sp) {
+  const SysDev<T>& s = *sp;
+  const uint32_t ne = s.kachel_el_off[s.n_kacheln];
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+    const uint32_t code = s.kachel_el[e], c = code & 0x3FFFFFFFu, kat = code >> 30;
+    if (kat == 0) {
+      KanteK<T> r;
+      r.n = s.k_n[c];
+      r.q = s.k_q[c];
+      r.kk = s.k_kk[c];
+      s.kachel_kante[e] = r;
+    } else if (kat == 1) {
+      BiegeK<T> r;
+      for (int l = 0; l < 4; ++l) r.g[l] = s.b_grad[4 * (size_t)c + l];
+      r.kt = s.b_kt[c];
+      s.kachel_biege[e] = r;
+    } else {
+      MembranK<T> r;
+      r.el = s.m_el[c];
+      r.h = s.m_h[c];
+      s.kachel_membran[e] = r;
+    }
+  }
+}
+
 // Warm start (implizit.rs:906): dv of the previous step, h*g on the first.
 template <typename T>
 __global__ void k_warm(const SysDev<T>* __restrict__ sp, int hat_vorher) {
@@ -662,30 +714,27 @@ __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T
   const uint32_t e1 = s.kachel_kat[2 * t], e2 = s.kachel_kat[2 * t + 1];
   const uint32_t bd = blockDim.x;
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += bd) {
-    const uint32_t c = s.kachel_el[e] & 0x3FFFFFFFu;
     const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
     const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    const V3<T> n = s.k_n[c];
-    const T q = s.k_q[c], kk = s.k_kk[c];
+    const KanteK<T> rk = s.kachel_kante[e];
+    const V3<T> n = rk.n;
+    const T q = rk.q, kk = rk.kk;
     const V3<T> d = u[lw.x & 0xFFFFu] - u[lw.x >> 16];
     const V3<T> bd3 = (n * (dot(n, d) * (T(1) - q)) + d * q) * kk;
     if ((pw.x & 0xFFFFu) != 0xFFFFu) slot[pw.x & 0xFFFFu] = bd3;
     if ((pw.x >> 16) != 0xFFFFu) slot[pw.x >> 16] = bd3 * T(-1);
   }
   for (uint32_t e = e1 + threadIdx.x; e < e2; e += bd) {
-    const uint32_t c = s.kachel_el[e] & 0x3FFFFFFFu;
     const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
     const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    V3<T> g[4];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) g[l] = s.b_grad[4 * (size_t)c + l];
-    biege_beitrag(g, s.b_kt[c], lw, pw, slot, u);
+    const BiegeK<T> rb = s.kachel_biege[e];
+    biege_beitrag(rb.g, rb.kt, lw, pw, slot, u);
   }
   for (uint32_t e = e2 + threadIdx.x; e < e3; e += bd) {
-    const uint32_t c = s.kachel_el[e] & 0x3FFFFFFFu;
     const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
     const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    membran_beitrag(s.m_el[c], s.m_h[c], lw, pw, slot, u);
+    const MembranK<T> rm = s.kachel_membran[e];
+    membran_beitrag(rm.el, rm.h, lw, pw, slot, u);
   }
 }
 

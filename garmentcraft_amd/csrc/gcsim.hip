@@ -68,6 +68,7 @@ struct gc_ctx {
   size_t pcg_lds = 0;       // dynamic LDS of the PCG kernel (largest tile)
   size_t lds_budget = 0;    // bytes of LDS a tile may fill
   int n_kacheln = 0;
+  long kachel_eintraege = 0;  // entries of all tiles' element lists
   // timing
   std::vector<hipEvent_t> ev;
   double ms_schritt = 0.0, ms_pcg = 0.0;
@@ -373,7 +374,14 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   uint16_t* d_klv = dalloc<uint16_t>(c, k_lv.size());
   uint32_t* d_voff = dalloc<uint32_t>(c, v_off.size());
   uint32_t* d_kv = dalloc<uint32_t>(c, k_v.size());
-  if (!d_kz || !d_koff || !d_kkat || !d_kel || !d_kpos || !d_klv || !d_voff || !d_kv) return fehler(c, "hipMalloc failed (tiles)");
+  const size_t n_ein = std::max<size_t>(1, k_el.size());
+  s.kachel_kante = dalloc<KanteK<T>>(c, n_ein);
+  s.kachel_biege = dalloc<BiegeK<T>>(c, n_ein);
+  s.kachel_membran = dalloc<MembranK<T>>(c, n_ein);
+  c->kachel_eintraege = (long)k_el.size();
+  if (!d_kz || !d_koff || !d_kkat || !d_kel || !d_kpos || !d_klv || !d_voff || !d_kv || !s.kachel_kante || !s.kachel_biege ||
+      !s.kachel_membran)
+    return fehler(c, "hipMalloc failed (tiles)");
   if (hoch(c, d_kz, kz.data(), kz.size()) || hoch(c, d_koff, k_off.data(), k_off.size()) ||
       hoch(c, d_kkat, k_kat.data(), k_kat.size()) ||
       hoch(c, d_kel, k_el.data(), k_el.size()) || hoch(c, d_kpos, k_pos.data(), k_pos.size()) ||
@@ -794,6 +802,7 @@ static int launch_bau(gc_ctx* c) {
   if (s.nm) hipLaunchKernelGGL(k_bau_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, sp);
   hipLaunchKernelGGL(k_fest<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp, c->gs);
   hipLaunchKernelGGL(k_sammeln<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sp);
+  hipLaunchKernelGGL(k_kachel_saetze<T>, dim3(gitter(c->kachel_eintraege)), dim3(BUILD_THREADS), 0, st, sp);
   HIPCK(c, hipGetLastError());
   return 0;
 }
