@@ -30,6 +30,14 @@ struct MembranK {
   Hessen<T> h;
 };
 
+// Data of one entry of a point's spring list (see SysDev::fe_satz).
+template <typename T>
+struct FederK {
+  uint32_t pq, pr;  // the points on sides q and r of the entry's pair
+  T om;             // -g_q g_r
+  T B[9];           // the spring's block
+};
+
 template <typename T>
 struct SysDev {
   int32_t n, nk, nb, nm;
@@ -130,6 +138,7 @@ struct SysDev {
   const T* fe_block;            // 9 per spring, bulk type
   V3<T>* fe_t;                  // per entry of fe_ent: its term K (p_q - p_r) (-g_q g_r)
   uint32_t* fe_tq;              // per entry: the point on side q of its pair
+  FederK<T>* fe_satz;           // per entry: its points, weight product and block (k_feder_saetze)
 
   // bodies
   int32_t n_koerper;
@@ -346,14 +355,30 @@ template <typename T, typename PF>
 __device__ __forceinline__ void feder_terme(const SysDev<T>& s, int lo, int hi, PF p) {
   const uint32_t e0 = s.fe_off[lo], e1 = s.fe_off[hi];
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const uint32_t code = s.fe_ent[e], f = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
-    const uint32_t pq = s.fe_punkte[5 * f + q], prr = s.fe_punkte[5 * f + r];
-    const T om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
-    const T* B = s.fe_block + 9 * f;
-    const V3<T> d = p(pq) - p(prr);
+    const FederK<T> k = s.fe_satz[e];
+    const T* B = k.B;
+    const V3<T> d = p(k.pq) - p(k.pr);
     s.fe_t[e] = V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
-                      B[6] * d.x + B[7] * d.y + B[8] * d.z} * om;
-    s.fe_tq[e] = pq;
+                      B[6] * d.x + B[7] * d.y + B[8] * d.z} * k.om;
+    s.fe_tq[e] = k.pq;
+  }
+}
+
+// The data of every spring list entry for feder_terme, after each rebuild of
+// the lists and blocks: its pair's two points, -g_q g_r and the block, so the
+// PCG loads them directly instead of through the spring index.
+template <typename T>
+__global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
+  const SysDev<T>& s = *sp;
+  const uint32_t ne = s.fe_off[s.n];
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+    const uint32_t code = s.fe_ent[e], f = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
+    FederK<T> o;
+    o.pq = s.fe_punkte[5 * f + q];
+    o.pr = s.fe_punkte[5 * f + r];
+    o.om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
+    for (int a = 0; a < 9; ++a) o.B[a] = s.fe_block[9 * (size_t)f + a];
+    s.fe_satz[e] = o;
   }
 }
 

@@ -95,6 +95,7 @@ struct gc_ctx {
   void* sk_block_t = nullptr;                // spring blocks in the bulk type
   void* sk_fe_t = nullptr;                   // per spring entry: its term in the PCG (bulk type)
   uint32_t* sk_fe_tq = nullptr;              // per spring entry: its point on side q
+  void* sk_fe_satz = nullptr;                // per spring entry: its points, weights and block (bulk type)
   // graph of one substep
   hipGraphExec_t graph = nullptr;
   // adjoint of the step (allocated on first use): FP64 buffers, the adjoint
@@ -643,6 +644,7 @@ static int selbst_vorbereiten(gc_ctx* c) {
   c->sk_block_t = dalloc<T>(c, 9 * (size_t)k.kapazitaet_federn); chk(c->sk_block_t);
   c->sk_fe_t = dalloc<V3<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_t);
   c->sk_fe_tq = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_tq);
+  c->sk_fe_satz = dalloc<FederK<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_satz);
   c->sk_dev = dalloc<SelbstDev>(c, 1); chk(c->sk_dev);
   if (!ok) return fehler(c, "hipMalloc/hipMemcpy failed (self-collision buffers)");
   HIPCK(c, hipMemcpyAsync(c->sk_dev, &k, sizeof(SelbstDev), hipMemcpyHostToDevice, c->stream));
@@ -658,6 +660,7 @@ static int selbst_vorbereiten(gc_ctx* c) {
   sy.fe_block = (const T*)c->sk_block_t;
   sy.fe_t = (V3<T>*)c->sk_fe_t;
   sy.fe_tq = c->sk_fe_tq;
+  sy.fe_satz = (FederK<T>*)c->sk_fe_satz;
   c->sk_bereit = true;
   c->sk_dicke = dicke;
   return 0;
@@ -750,6 +753,7 @@ static int launch_selbst_bau(gc_ctx* c) {
   hipLaunchKernelGGL(k_sk_feder_leeren, dim3(sk_gitter(k.n + 1)), dim3(BUILD_THREADS), 0, st, sp);
   hipLaunchKernelGGL(k_sk_feder_bloecke<T>, dim3(sk_gitter(9L * k.kapazitaet_federn)), dim3(BUILD_THREADS), 0, st, sp,
                      (T*)c->sk_block_t);
+  hipLaunchKernelGGL(k_feder_saetze<T>, dim3(sk_gitter(k.kapazitaet_fe_ent)), dim3(BUILD_THREADS), 0, st, dsys<T>(c));
   hipLaunchKernelGGL(k_sk_statistik, dim3(1), dim3(64), 0, st, sp, c->gs, 0);
   HIPCK(c, hipGetLastError());
   return 0;
