@@ -386,7 +386,21 @@ __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
 // feder_zeile.
 template <typename T>
 __device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc) {
-  for (uint32_t e = s.fe_off[i]; e < s.fe_off[i + 1]; ++e) {
+  uint32_t e = s.fe_off[i];
+  const uint32_t b = s.fe_off[i + 1];
+  // four terms are loaded before their sums; the sums keep list order
+  for (; e + 4 <= b; e += 4) {
+    V3<T> t[4];
+    uint32_t q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      t[k] = s.fe_t[e + k];
+      q[k] = s.fe_tq[e + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc = q[k] == i ? acc + t[k] : acc - t[k];
+  }
+  for (; e < b; ++e) {
     const V3<T> t = s.fe_t[e];
     acc = s.fe_tq[e] == i ? acc + t : acc - t;
   }
@@ -768,7 +782,16 @@ template <typename T>
 __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_t basis, const V3<T>* slot) {
   const uint32_t a = s.off[GC_KAT * i] - basis, b = s.off[GC_KAT * i + GC_KAT] - basis;
   V3<T> hp = zero3<T>();
-  for (uint32_t q = a; q < b; ++q) hp = hp + slot[q];
+  uint32_t q = a;
+  // four slots are loaded before their sums; the sums keep CSR order
+  for (; q + 4 <= b; q += 4) {
+    const V3<T> v0 = slot[q], v1 = slot[q + 1], v2 = slot[q + 2], v3 = slot[q + 3];
+    hp = hp + v0;
+    hp = hp + v1;
+    hp = hp + v2;
+    hp = hp + v3;
+  }
+  for (; q < b; ++q) hp = hp + slot[q];
   return hp;
 }
 
