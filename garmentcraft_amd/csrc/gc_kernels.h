@@ -275,8 +275,21 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
     double acc[NV];
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
     if (!fehler)
-      for (unsigned b = lane; b < nb; b += 64)
-        for (int k = 0; k < NV; ++k) acc[k] += __hip_atomic_load(&buf[b].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (unsigned b0 = lane; b0 < nb; b0 += 256) {
+        // four partials per lane are loaded before their sums, which keep the
+        // order b0, b0 + 64, ...
+        double x[4][NV];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            x[u][k] = b0 + 64u * u < nb ? __hip_atomic_load(&buf[b0 + 64u * u].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (b0 + 64u * u < nb)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
+      }
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
