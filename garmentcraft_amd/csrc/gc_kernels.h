@@ -8,6 +8,7 @@
 // loops add into that point (implizit.rs:185-221, 232-279, 725-827). The FP64
 // build therefore reproduces the reference's per-point sums.
 #pragma once
+#include <type_traits>
 #include "gc_math.h"
 
 // CSR entry: constraint index << 2 | slot. Kante: slot 0 = first point (+), 1 = second (-).
@@ -29,6 +30,63 @@ struct MembranK {
   Membran<T> el;
   Hessen<T> h;
 };
+
+// Global-memory view of a pointer the compiler only knows as generic (every
+// pointer read from SysDev): loads and stores through it are global_*
+// instead of flat_* instructions, so they do not count against the LDS wait
+// counter that the stages' LDS reads wait on.
+#define GC_GM __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const GC_GM T* gm(const T* p) {
+  return (const GC_GM T*)p;
+}
+template <typename T>
+__device__ __forceinline__ GC_GM T* gm(T* p) {
+  return (GC_GM T*)p;
+}
+// One element read or written through a global view. Class types (vectors,
+// records) have no copy through a global lvalue, so they go word by word; the
+// compiler merges the words into wide loads and stores.
+template <typename T>
+__device__ __forceinline__ T lg(const GC_GM T* p) {
+  if constexpr (std::is_class<T>::value) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole words");
+    const GC_GM uint32_t* q = (const GC_GM uint32_t*)p;
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) w[k] = q[k];
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+  } else {
+    return *p;
+  }
+}
+template <typename T>
+__device__ __forceinline__ T lg(const T* p) {
+  return lg(gm(p));
+}
+template <typename T>
+__device__ __forceinline__ void sg(GC_GM T* p, const T& v) {
+  if constexpr (std::is_class<T>::value) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole words");
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+    GC_GM uint32_t* q = (GC_GM uint32_t*)p;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) q[k] = w[k];
+  } else {
+    *p = v;
+  }
+}
+// m3_mal with the matrix read through a global view
+template <typename T>
+__device__ __forceinline__ V3<T> m3_mal_g(const GC_GM T* m, V3<T> v) {
+  T a[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) a[q] = m[q];
+  return m3_mal(a, v);
+}
 
 // Data of one entry of a point's spring list (see SysDev::fe_satz).
 template <typename T>
@@ -366,14 +424,14 @@ __device__ __forceinline__ V3<T> feder_zeile(const SysDev<T>& s, uint32_t i, V3<
 // feder_zeile; the block's __syncthreads() publishes them to feder_summe.
 template <typename T, typename PF>
 __device__ __forceinline__ void feder_terme(const SysDev<T>& s, int lo, int hi, PF p) {
-  const uint32_t e0 = s.fe_off[lo], e1 = s.fe_off[hi];
+  const uint32_t e0 = gm(s.fe_off)[lo], e1 = gm(s.fe_off)[hi];
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const FederK<T> k = s.fe_satz[e];
+    const FederK<T> k = lg(gm(s.fe_satz) + (e));
     const T* B = k.B;
     const V3<T> d = p(k.pq) - p(k.pr);
-    s.fe_t[e] = V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
-                      B[6] * d.x + B[7] * d.y + B[8] * d.z} * k.om;
-    s.fe_tq[e] = k.pq;
+    sg(gm(s.fe_t) + (e), V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
+                      B[6] * d.x + B[7] * d.y + B[8] * d.z} * k.om);
+    gm(s.fe_tq)[e] = k.pq;
   }
 }
 
@@ -383,7 +441,7 @@ __device__ __forceinline__ void feder_terme(const SysDev<T>& s, int lo, int hi, 
 template <typename T>
 __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
   const SysDev<T>& s = *sp;
-  const uint32_t ne = s.fe_off[s.n];
+  const uint32_t ne = gm(s.fe_off)[s.n];
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
     const uint32_t code = s.fe_ent[e], f = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
     FederK<T> o;
@@ -391,7 +449,7 @@ __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
     o.pr = s.fe_punkte[5 * f + r];
     o.om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
     for (int a = 0; a < 9; ++a) o.B[a] = s.fe_block[9 * (size_t)f + a];
-    s.fe_satz[e] = o;
+    sg(gm(s.fe_satz) + (e), o);
   }
 }
 
@@ -399,23 +457,23 @@ __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
 // feder_zeile.
 template <typename T>
 __device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc) {
-  uint32_t e = s.fe_off[i];
-  const uint32_t b = s.fe_off[i + 1];
+  uint32_t e = gm(s.fe_off)[i];
+  const uint32_t b = gm(s.fe_off)[i + 1];
   // four terms are loaded before their sums; the sums keep list order
   for (; e + 4 <= b; e += 4) {
     V3<T> t[4];
     uint32_t q[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      t[k] = s.fe_t[e + k];
-      q[k] = s.fe_tq[e + k];
+      t[k] = lg(gm(s.fe_t) + (e + k));
+      q[k] = gm(s.fe_tq)[e + k];
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc = q[k] == i ? acc + t[k] : acc - t[k];
   }
   for (; e < b; ++e) {
-    const V3<T> t = s.fe_t[e];
-    acc = s.fe_tq[e] == i ? acc + t : acc - t;
+    const V3<T> t = lg(gm(s.fe_t) + (e));
+    acc = gm(s.fe_tq)[e] == i ? acc + t : acc - t;
   }
   return acc;
 }
@@ -721,11 +779,15 @@ __device__ __forceinline__ V3<T> filter_fl(uint8_t fl, V3<T> n, V3<T> v) {
 // L2 from one iteration to the next. Every read of such data across blocks
 // goes through this function; a block's own rows are read with plain loads.
 template <typename T>
-__device__ __forceinline__ V3<T> lade_k(const V3<T>* a, uint32_t j) {
-  const T* q = (const T*)(a + j);
+__device__ __forceinline__ V3<T> lade_k(const GC_GM V3<T>* a, uint32_t j) {
+  const GC_GM T* q = (const GC_GM T*)(a + j);
   return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
           __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
           __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+}
+template <typename T>
+__device__ __forceinline__ V3<T> lade_k(const V3<T>* a, uint32_t j) {
+  return lade_k(gm(a), j);
 }
 
 // One constraint's contributions to the tile's own points, each into its LDS
@@ -762,13 +824,13 @@ __device__ __forceinline__ void biege_beitrag(const V3<T> (&g)[4], T kt, uint2 l
 // slower: 7.7 -> 10.0 us per iteration for this stage on the drape.)
 template <typename T>
 __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T>* slot, const V3<T>* u) {
-  const uint32_t e0 = s.kachel_el_off[t], e3 = s.kachel_el_off[t + 1];
-  const uint32_t e1 = s.kachel_kat[2 * t], e2 = s.kachel_kat[2 * t + 1];
+  const uint32_t e0 = gm(s.kachel_el_off)[t], e3 = gm(s.kachel_el_off)[t + 1];
+  const uint32_t e1 = gm(s.kachel_kat)[2 * t], e2 = gm(s.kachel_kat)[2 * t + 1];
   const uint32_t bd = blockDim.x;
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += bd) {
-    const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
-    const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    const KanteK<T> rk = s.kachel_kante[e];
+    const uint2 lw = lg((const GC_GM uint2*)(s.kachel_lv + 4 * (size_t)e));
+    const uint2 pw = lg((const GC_GM uint2*)(s.kachel_pos + 4 * (size_t)e));
+    const KanteK<T> rk = lg(gm(s.kachel_kante) + (e));
     const V3<T> n = rk.n;
     const T q = rk.q, kk = rk.kk;
     const V3<T> d = u[lw.x & 0xFFFFu] - u[lw.x >> 16];
@@ -777,15 +839,15 @@ __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T
     if ((pw.x >> 16) != 0xFFFFu) slot[pw.x >> 16] = bd3 * T(-1);
   }
   for (uint32_t e = e1 + threadIdx.x; e < e2; e += bd) {
-    const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
-    const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    const BiegeK<T> rb = s.kachel_biege[e];
+    const uint2 lw = lg((const GC_GM uint2*)(s.kachel_lv + 4 * (size_t)e));
+    const uint2 pw = lg((const GC_GM uint2*)(s.kachel_pos + 4 * (size_t)e));
+    const BiegeK<T> rb = lg(gm(s.kachel_biege) + (e));
     biege_beitrag(rb.g, rb.kt, lw, pw, slot, u);
   }
   for (uint32_t e = e2 + threadIdx.x; e < e3; e += bd) {
-    const uint2 lw = *(const uint2*)(s.kachel_lv + 4 * (size_t)e);
-    const uint2 pw = *(const uint2*)(s.kachel_pos + 4 * (size_t)e);
-    const MembranK<T> rm = s.kachel_membran[e];
+    const uint2 lw = lg((const GC_GM uint2*)(s.kachel_lv + 4 * (size_t)e));
+    const uint2 pw = lg((const GC_GM uint2*)(s.kachel_pos + 4 * (size_t)e));
+    const MembranK<T> rm = lg(gm(s.kachel_membran) + (e));
     membran_beitrag(rm.el, rm.h, lw, pw, slot, u);
   }
 }
@@ -793,7 +855,7 @@ __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T
 // H*u of point i from its LDS slots, in CSR order.
 template <typename T>
 __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_t basis, const V3<T>* slot) {
-  const uint32_t a = s.off[GC_KAT * i] - basis, b = s.off[GC_KAT * i + GC_KAT] - basis;
+  const uint32_t a = gm(s.off)[GC_KAT * i] - basis, b = gm(s.off)[GC_KAT * i + GC_KAT] - basis;
   V3<T> hp = zero3<T>();
   uint32_t q = a;
   // four slots are loaded before their sums; the sums keep CSR order
@@ -814,11 +876,11 @@ __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_
 // K = true for a point another block owns: its z and p are read with
 // coherent loads (see lade_k).
 template <bool K, typename T>
-__device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const V3<T>* po, T tb, bool erste) {
-  const uint8_t fl = s.fflag[j];
-  V3<T> q = K ? lade_k(s.z, j) : s.z[j];
-  if (!erste) q = q + (K ? lade_k(po, j) : po[j]) * tb;
-  return filter_fl(fl, s.fnorm[j], q);
+__device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const GC_GM V3<T>* po, T tb, bool erste) {
+  const uint8_t fl = gm(s.fflag)[j];
+  V3<T> q = K ? lade_k(s.z, j) : lg(s.z + j);
+  if (!erste) q = q + (K ? lade_k(po, j) : lg(po + j)) * tb;
+  return filter_fl(fl, lg(gm(s.fnorm) + (j)), q);
 }
 
 // ---------------------------------------------------------------------------
@@ -841,7 +903,7 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
   unsigned epoch = 0;
-  auto zeilen = [&](int t, int* lo, int* hi) { *lo = s.kachel_zeilen[2 * t]; *hi = s.kachel_zeilen[2 * t + 1]; };
+  auto zeilen = [&](int t, int* lo, int* hi) { *lo = gm(s.kachel_zeilen)[2 * t]; *hi = gm(s.kachel_zeilen)[2 * t + 1]; };
   const T th2 = T(s.h2);
 
   // start: held along the normal get the target normal velocity (implizit.rs:375-384)
@@ -849,14 +911,14 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     int lo, hi;
     zeilen(t, &lo, &hi);
     for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const uint8_t fl = s.fflag[i];
+      const uint8_t fl = gm(s.fflag)[i];
       if (fl == 2) {
-        s.dv[i] = zero3<T>();
+        sg(gm(s.dv) + (i), zero3<T>());
       } else if (fl != 0) {  // 1; 3 only in the adjoint's solve (see k_freigabe)
-        const V3<double> n = s.fnorm64[i];
-        const double soll = s.soll_direkt ? s.soll_direkt[i] : s.vn_soll[i] - dot(s.v[i], n);
-        const V3<T> dvf = filter_fl(fl, s.fnorm[i], s.dv[i]);
-        s.dv[i] = dvf + s.fnorm[i] * T(soll);
+        const V3<double> n = lg(gm(s.fnorm64) + (i));
+        const double soll = s.soll_direkt ? gm(s.soll_direkt)[i] : gm(s.vn_soll)[i] - dot(lg(gm(s.v) + (i)), n);
+        const V3<T> dvf = filter_fl(fl, lg(gm(s.fnorm) + (i)), lg(gm(s.dv) + (i)));
+        sg(gm(s.dv) + (i), dvf + lg(gm(s.fnorm) + (i)) * T(soll));
       }
     }
   }
@@ -882,28 +944,28 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     for (int t = rang; t < s.n_kacheln; t += nb) {
       int lo, hi;
       zeilen(t, &lo, &hi);
-      const uint32_t basis = s.off[GC_KAT * lo];
-      const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = lade_k(s.dv, s.kachel_v[v]);
+      const uint32_t basis = gm(s.off)[GC_KAT * lo];
+      const uint32_t v0 = gm(s.kachel_v_off)[t], v1 = gm(s.kachel_v_off)[t + 1];
+      for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = lade_k(s.dv, gm(s.kachel_v)[v]);
       __syncthreads();
       kachel_beitraege(s, t, slot, ul);
       __syncthreads();
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return lade_k(s.dv, j); });
-        const V3<T> ap = hp * th2 + s.dv[i] * s.mdiag[i];
-        const uint8_t fl = s.fflag[i];
-        const V3<T> n = s.fnorm[i];
-        const V3<T> bi = s.b[i];
+        const V3<T> ap = hp * th2 + lg(gm(s.dv) + (i)) * gm(s.mdiag)[i];
+        const uint8_t fl = gm(s.fflag)[i];
+        const V3<T> n = lg(gm(s.fnorm) + (i));
+        const V3<T> bi = lg(gm(s.b) + (i));
         const V3<T> ri = filter_fl(fl, n, bi - ap);
-        const T* P = s.pinv + 9 * (size_t)i;
+        const GC_GM T* P = gm(s.pinv) + 9 * (size_t)i;
         if (fl != 2) {
           const V3<T> fb = filter_fl(fl, n, bi);
-          part[0] += (double)dot(fb, m3_mal(P, fb));
+          part[0] += (double)dot(fb, m3_mal_g(P, fb));
         }
-        const V3<T> zi = m3_mal(P, ri);
+        const V3<T> zi = m3_mal_g(P, ri);
         part[1] += (double)dot(ri, filter_fl(fl, n, zi));
-        s.r[i] = ri;
-        s.z[i] = zi;
+        sg(gm(s.r) + (i), ri);
+        sg(gm(s.z) + (i), zi);
       }
       __syncthreads();
     }
@@ -920,8 +982,8 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     int it_min = it;
     bool neu_starten = false;
     while (delta > ziel && it < s.cg_max) {
-      V3<T>* pn = s.p[cur];
-      const V3<T>* po = s.p[cur ^ 1];
+      GC_GM V3<T>* pn = gm(s.p[cur]);
+      const GC_GM V3<T>* po = gm(s.p[cur ^ 1]);
       const T tb = T(beta);
       const bool erste = it == it_start;
       // ap = S(A p), ca = p . ap (implizit.rs:401-405)
@@ -929,21 +991,21 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       for (int t = rang; t < s.n_kacheln; t += nb) {
         int lo, hi;
         zeilen(t, &lo, &hi);
-        const uint32_t basis = s.off[GC_KAT * lo];
-        const uint32_t v0 = s.kachel_v_off[t], v1 = s.kachel_v_off[t + 1];
-        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, s.kachel_v[v], po, tb, erste);
+        const uint32_t basis = gm(s.off)[GC_KAT * lo];
+        const uint32_t v0 = gm(s.kachel_v_off)[t], v1 = gm(s.kachel_v_off)[t + 1];
+        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, gm(s.kachel_v)[v], po, tb, erste);
         __syncthreads();
         kachel_beitraege(s, t, slot, ul);
         if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); });
         __syncthreads();
         for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-          const uint8_t fl = s.fflag[i];
+          const uint8_t fl = gm(s.fflag)[i];
           const V3<T> pi = richtung<false>(s, (uint32_t)i, po, tb, erste);
           const V3<T> hk = kachel_zeile(s, i, basis, slot);
           const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk) : hk;
-          const V3<T> ap = filter_fl(fl, s.fnorm[i], hp * th2 + pi * s.mdiag[i]);
-          pn[i] = pi;
-          s.ap[i] = ap;
+          const V3<T> ap = filter_fl(fl, lg(gm(s.fnorm) + (i)), hp * th2 + pi * gm(s.mdiag)[i]);
+          sg(pn + i, pi);
+          sg(gm(s.ap) + (i), ap);
           ca[0] += (double)dot(pi, ap);
         }
         __syncthreads();
@@ -958,11 +1020,11 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         int lo, hi;
         zeilen(t, &lo, &hi);
         for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-          s.dv[i] = s.dv[i] + pn[i] * ta;
-          const V3<T> ri = s.r[i] - s.ap[i] * ta;
-          s.r[i] = ri;
-          const V3<T> zi = m3_mal(s.pinv + 9 * (size_t)i, ri);
-          s.z[i] = zi;
+          sg(gm(s.dv) + (i), lg(gm(s.dv) + (i)) + lg(pn + i) * ta);
+          const V3<T> ri = lg(gm(s.r) + (i)) - lg(gm(s.ap) + (i)) * ta;
+          sg(gm(s.r) + (i), ri);
+          const V3<T> zi = m3_mal_g(gm(s.pinv) + 9 * (size_t)i, ri);
+          sg(gm(s.z) + (i), zi);
           neu[0] += (double)dot(ri, zi);
         }
       }
