@@ -92,7 +92,7 @@ __device__ __forceinline__ V3<T> m3_mal_g(const GC_GM T* m, V3<T> v) {
 template <typename T>
 struct FederK {
   uint32_t pq, pr;  // the points on sides q and r of the entry's pair
-  T om;             // -g_q g_r
+  T om;             // -g_q g_r, negated when the entry's row is not p_q
   T B[9];           // the spring's block
 };
 
@@ -173,6 +173,7 @@ struct SysDev {
   const uint32_t* kachel_v_off;   // point-list offsets (n_kacheln + 1)
   const uint32_t* kachel_v;       // points touched by the tile's constraints, global index
   uint32_t kachel_slots_max;      // LDS slot area (entries) of the largest tile
+  uint32_t kachel_v_max;          // LDS staging area (points) of the largest tile
   // the records of every tile entry at its entry index, copied from the
   // per-constraint arrays after each build (k_kachel_saetze), so stage B
   // loads them directly instead of through the constraint index; an entry
@@ -195,8 +196,8 @@ struct SysDev {
   const V3<double>* fe_kraft;
   const T* fe_block;            // 9 per spring, bulk type
   V3<T>* fe_t;                  // per entry of fe_ent: its term K (p_q - p_r) (-g_q g_r)
-  uint32_t* fe_tq;              // per entry: the point on side q of its pair
-  FederK<T>* fe_satz;           // per entry: its points, weight product and block (k_feder_saetze)
+  FederK<T>* fe_satz;           // per entry: its points, signed weight product and block (k_feder_saetze)
+  uint32_t fe_lds_kap;          // spring terms of a tile kept in LDS (after the staging); the rest in fe_t
 
   // bodies
   int32_t n_koerper;
@@ -421,60 +422,67 @@ __device__ __forceinline__ V3<T> feder_zeile(const SysDev<T>& s, uint32_t i, V3<
 // computed by the whole block, one entry per thread: each entry gathers two
 // directions and a block of its own, so a row with many springs no longer
 // walks them in sequence while the grid waits on it. Same expression as
-// feder_zeile; the block's __syncthreads() publishes them to feder_summe.
+// feder_zeile, with the row's sign in the weight product (c * -om is exactly
+// -(c * om)). Entry e - e0 < fe_lds_kap goes to lds, the rest to fe_t; the
+// block's __syncthreads() publishes them to feder_summe.
 template <typename T, typename PF>
-__device__ __forceinline__ void feder_terme(const SysDev<T>& s, int lo, int hi, PF p) {
+__device__ __forceinline__ void feder_terme(const SysDev<T>& s, int lo, int hi, PF p, V3<T>* lds) {
   const uint32_t e0 = gm(s.fe_off)[lo], e1 = gm(s.fe_off)[hi];
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const FederK<T> k = lg(gm(s.fe_satz) + (e));
     const T* B = k.B;
     const V3<T> d = p(k.pq) - p(k.pr);
-    sg(gm(s.fe_t) + (e), V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
-                      B[6] * d.x + B[7] * d.y + B[8] * d.z} * k.om);
-    gm(s.fe_tq)[e] = k.pq;
+    const V3<T> t = V3<T>{B[0] * d.x + B[1] * d.y + B[2] * d.z, B[3] * d.x + B[4] * d.y + B[5] * d.z,
+                          B[6] * d.x + B[7] * d.y + B[8] * d.z} * k.om;
+    if (e - e0 < s.fe_lds_kap) lds[e - e0] = t;
+    else sg(gm(s.fe_t) + (e), t);
   }
 }
 
 // The data of every spring list entry for feder_terme, after each rebuild of
-// the lists and blocks: its pair's two points, -g_q g_r and the block, so the
-// PCG loads them directly instead of through the spring index.
+// the lists and blocks: its pair's two points, the block and -g_q g_r,
+// negated when the list the entry sits in is not p_q's (the row adds the term
+// then, feder_zeile subtracts it), so a row just adds its terms. The row of
+// entry e is found by bisection over the list offsets.
 template <typename T>
 __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
   const SysDev<T>& s = *sp;
   const uint32_t ne = gm(s.fe_off)[s.n];
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
     const uint32_t code = s.fe_ent[e], f = code >> 5, pr = code & 31u, q = pr / 5u, r = pr % 5u;
-    FederK<T> o;
-    o.pq = s.fe_punkte[5 * f + q];
-    o.pr = s.fe_punkte[5 * f + r];
-    o.om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
-    for (int a = 0; a < 9; ++a) o.B[a] = s.fe_block[9 * (size_t)f + a];
-    sg(gm(s.fe_satz) + (e), o);
+    // the row: the last i with fe_off[i] <= e (empty rows share their offset)
+    uint32_t u = 0, o = (uint32_t)s.n;
+    while (o - u > 1) {
+      const uint32_t m = (u + o) >> 1;
+      if (s.fe_off[m] <= e) u = m;
+      else o = m;
+    }
+    FederK<T> k;
+    k.pq = s.fe_punkte[5 * f + q];
+    k.pr = s.fe_punkte[5 * f + r];
+    const T om = T(-s.fe_gew[5 * f + q] * s.fe_gew[5 * f + r]);
+    k.om = k.pq == u ? om : -om;
+    for (int a = 0; a < 9; ++a) k.B[a] = s.fe_block[9 * (size_t)f + a];
+    sg(gm(s.fe_satz) + (e), k);
   }
 }
 
-// Row i's springs from the terms of feder_terme, added in the order of
-// feder_zeile.
+// Row i's springs from the terms of feder_terme (in LDS from entry e0 of the
+// tile on, up to fe_lds_kap of them), added in the order of feder_zeile; four
+// terms are loaded before their sums.
 template <typename T>
-__device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc) {
+__device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc, const V3<T>* lds, uint32_t e0) {
   uint32_t e = gm(s.fe_off)[i];
   const uint32_t b = gm(s.fe_off)[i + 1];
-  // four terms are loaded before their sums; the sums keep list order
+  auto term = [&](uint32_t q) { return q - e0 < s.fe_lds_kap ? lds[q - e0] : lg(gm(s.fe_t) + (q)); };
   for (; e + 4 <= b; e += 4) {
     V3<T> t[4];
-    uint32_t q[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      t[k] = lg(gm(s.fe_t) + (e + k));
-      q[k] = gm(s.fe_tq)[e + k];
-    }
+    for (int k = 0; k < 4; ++k) t[k] = term(e + k);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc = q[k] == i ? acc + t[k] : acc - t[k];
+    for (int k = 0; k < 4; ++k) acc = acc + t[k];
   }
-  for (; e < b; ++e) {
-    const V3<T> t = lg(gm(s.fe_t) + (e));
-    acc = gm(s.fe_tq)[e] == i ? acc + t : acc - t;
-  }
+  for (; e < b; ++e) acc = acc + term(e);
   return acc;
 }
 
@@ -899,6 +907,7 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
                                            unsigned char* lds_roh, double* sh) {
   V3<T>* slot = (V3<T>*)lds_roh;
   V3<T>* ul = slot + s.kachel_slots_max;
+  V3<T>* fterm = ul + s.kachel_v_max;
   if (durchgang == 1) {
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
@@ -991,18 +1000,19 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       for (int t = rang; t < s.n_kacheln; t += nb) {
         int lo, hi;
         zeilen(t, &lo, &hi);
+        const uint32_t fe0 = s.selbst ? gm(s.fe_off)[lo] : 0u;
         const uint32_t basis = gm(s.off)[GC_KAT * lo];
         const uint32_t v0 = gm(s.kachel_v_off)[t], v1 = gm(s.kachel_v_off)[t + 1];
         for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, gm(s.kachel_v)[v], po, tb, erste);
         __syncthreads();
         kachel_beitraege(s, t, slot, ul);
-        if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); });
+        if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); }, fterm);
         __syncthreads();
         for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
           const uint8_t fl = gm(s.fflag)[i];
           const V3<T> pi = richtung<false>(s, (uint32_t)i, po, tb, erste);
           const V3<T> hk = kachel_zeile(s, i, basis, slot);
-          const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk) : hk;
+          const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk, fterm, fe0) : hk;
           const V3<T> ap = filter_fl(fl, lg(gm(s.fnorm) + (i)), hp * th2 + pi * gm(s.mdiag)[i]);
           sg(pn + i, pi);
           sg(gm(s.ap) + (i), ap);

@@ -65,7 +65,10 @@ struct gc_ctx {
   bool topo = false, hat_param = false, built = false;
   bool hat_vorher = false;
   int pcg_blocks = 0, pcg_threads = 1024, pcg_max_blocks = 0;
-  size_t pcg_lds = 0;       // dynamic LDS of the PCG kernel (largest tile)
+  size_t pcg_lds = 0;       // dynamic LDS of the PCG kernel (largest tile, spring terms)
+  size_t pcg_lds_kachel = 0;  // its part for the largest tile (slots and staging)
+  uint32_t fe_kap_allein = 0;  // spring terms in LDS when the system runs alone
+  size_t lds_cu = 65536;    // LDS of one CU
   size_t lds_budget = 0;    // bytes of LDS a tile may fill
   int n_kacheln = 0;
   long kachel_eintraege = 0;  // entries of all tiles' element lists
@@ -94,7 +97,6 @@ struct gc_ctx {
   double* sk_naht_ruhe_dev = nullptr;
   void* sk_block_t = nullptr;                // spring blocks in the bulk type
   void* sk_fe_t = nullptr;                   // per spring entry: its term in the PCG (bulk type)
-  uint32_t* sk_fe_tq = nullptr;              // per spring entry: its point on side q
   void* sk_fe_satz = nullptr;                // per spring entry: its points, weights and block (bulk type)
   // graph of one substep
   hipGraphExec_t graph = nullptr;
@@ -399,9 +401,18 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   s.kachel_v_off = d_voff;
   s.kachel_v = d_kv;
   s.kachel_slots_max = (uint32_t)max_slots;
+  s.kachel_v_max = (uint32_t)max_v;
   const size_t groesste = max_slots + max_v;
+  // the CU's LDS left after the largest tile's slots and staging keeps a
+  // tile's self-collision spring terms (up to 3072; more go to global memory)
+  const size_t lds_tile = groesste * sizeof(V3<T>), lds_rest = 2048;
+  size_t fe_kap = lds_tile + lds_rest < c->lds_cu ? (c->lds_cu - lds_tile - lds_rest) / sizeof(V3<T>) : 0;
+  fe_kap = std::min<size_t>(fe_kap, 3072);
+  s.fe_lds_kap = (uint32_t)fe_kap;
+  c->fe_kap_allein = (uint32_t)fe_kap;
+  c->pcg_lds_kachel = std::max<size_t>(16, lds_tile);
   // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
-  c->pcg_lds = std::max<size_t>(16, groesste * sizeof(V3<T>));
+  c->pcg_lds = std::max<size_t>(16, (groesste + fe_kap) * sizeof(V3<T>));
   (void)hipFuncSetAttribute((const void*)k_pcg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->pcg_lds);
   int occ = 0;
   HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, c->pcg_lds));
@@ -643,7 +654,6 @@ static int selbst_vorbereiten(gc_ctx* c) {
   c->sk_scan_teil = dalloc<uint32_t>(c, c->sk_scan_max / SK_SCAN_CHUNK + 1); chk(c->sk_scan_teil);
   c->sk_block_t = dalloc<T>(c, 9 * (size_t)k.kapazitaet_federn); chk(c->sk_block_t);
   c->sk_fe_t = dalloc<V3<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_t);
-  c->sk_fe_tq = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_tq);
   c->sk_fe_satz = dalloc<FederK<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_satz);
   c->sk_dev = dalloc<SelbstDev>(c, 1); chk(c->sk_dev);
   if (!ok) return fehler(c, "hipMalloc/hipMemcpy failed (self-collision buffers)");
@@ -659,7 +669,6 @@ static int selbst_vorbereiten(gc_ctx* c) {
   sy.fe_kraft = k.fe_kraft;
   sy.fe_block = (const T*)c->sk_block_t;
   sy.fe_t = (V3<T>*)c->sk_fe_t;
-  sy.fe_tq = c->sk_fe_tq;
   sy.fe_satz = (FederK<T>*)c->sk_fe_satz;
   c->sk_bereit = true;
   c->sk_dicke = dicke;
@@ -1082,10 +1091,29 @@ static int tabelle_laden(gc_batch* b) {
   return 0;
 }
 
+// LDS of a batched launch: the largest tile of its systems. Each system keeps
+// as many spring terms in LDS as fit beside its own tile in that size (never
+// more than when it runs alone), so a batch gets the blocks per CU of its tiles.
+// The counts reach the device with the next upload of each system.
+template <typename T>
+static size_t batch_lds(gc_batch* b) {
+  size_t lds = 16;
+  for (gc_ctx* c : b->ctx) lds = std::max(lds, c->pcg_lds_kachel);
+  for (gc_ctx* c : b->ctx) {
+    const size_t frei = (lds - c->pcg_lds_kachel) / sizeof(V3<T>);
+    sys<T>(c).fe_lds_kap = (uint32_t)std::min<size_t>(frei, c->fe_kap_allein);
+  }
+  return lds;
+}
+
 template <typename T>
 static int batch_form(gc_batch* b, int threads, int gesamt) {
-  size_t lds = 16;
-  for (gc_ctx* c : b->ctx) lds = std::max(lds, c->pcg_lds);
+  const size_t lds = batch_lds<T>(b);
+  // the systems' spring counts must be on the device before any launch of
+  // this shape (the autotuner launches right after)
+  BHIPCK(b, hipStreamSynchronize(b->stream));
+  for (gc_ctx* c : b->ctx)
+    if (sys_hoch<T>(c)) return bfehler(b, c->err);
   BHIPCK(b, hipFuncSetAttribute((const void*)k_pcg_stapel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int occ = 0;
   BHIPCK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg_stapel<T>, threads, lds));
@@ -1347,8 +1375,7 @@ static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* 
         b->gewicht = gw ? nach_arbeit : nach_kacheln;
         // blocks available at this thread count
         int occ = 0;
-        size_t lds = 16;
-        for (gc_ctx* c : b->ctx) lds = std::max(lds, c->pcg_lds);
+        const size_t lds = batch_lds<T>(b);
         if (hipFuncSetAttribute((const void*)k_pcg_stapel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg_stapel<T>, threads_kand[ti], lds) != hipSuccess ||
             occ < 1)
@@ -1740,6 +1767,7 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
   gc_ctx* c = new gc_ctx();
   c->opt = o;
   c->cus = p.multiProcessorCount;
+  c->lds_cu = p.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)p.maxSharedMemoryPerMultiProcessor : 65536;
   c->lds_budget = (size_t)(o.lds_kb > 0 ? o.lds_kb : 64) * 1024;
   if (c->lds_budget + 1024 > (size_t)p.sharedMemPerBlock) {
     melde("lds_kb exceeds the LDS of a workgroup");
