@@ -219,7 +219,8 @@ def stapel_lauf(args) -> int:
     def kontexte():
         gs = []
         for s, _ in satz:
-            gs.append(GeraetSzene(s, param, praezision=args.praezision, koerper_quelle=gs[0] if gs else None))
+            gs.append(GeraetSzene(s, param, praezision=args.praezision, koerper_quelle=gs[0] if gs else None,
+                                  lds_kb=args.lds_kb))
         return gs
 
     A = kontexte()
@@ -456,7 +457,7 @@ def gross_lauf(args) -> int:
     print(f"[bench] scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     selbst = not args.ohne_selbstkollision
     p = ImplizitParameter(schritte_je_bild=1, cg_max=20000, cg_toleranz=1e-7, naht_k=500.0, selbstkollision=selbst)
-    g = {pr: GeraetSzene(s, p, praezision=pr) for pr in ("fp32", "fp64")}
+    g = {pr: GeraetSzene(s, p, praezision=pr, bloecke=args.bloecke, lds_kb=args.lds_kb) for pr in ("fp32", "fp64")}
     ms = {"fp32": 0.0, "fp64": 0.0}
     ms_pcg = {"fp32": 0.0, "fp64": 0.0}
     fehler_max, fehler_mittel = [], []

@@ -35,9 +35,10 @@ enum { GC_FP64 = 0, GC_FP32 = 1 };
 typedef struct gc_optionen {
   int32_t praezision;     /* GC_FP64 or GC_FP32 */
   int32_t geraet;         /* HIP device ordinal */
-  int32_t bloecke;        /* persistent PCG blocks; 0 = 128 (or one per tile if fewer) */
+  int32_t bloecke;        /* persistent PCG blocks; 0 = one per tile, up to the
+                           * co-resident maximum (one 1024-thread block per CU) */
   int32_t threads;        /* threads per block; 0 = choose */
-  int32_t lds_kb;         /* LDS a PCG tile may fill (KiB); 0 = 64 */
+  int32_t lds_kb;         /* LDS a PCG tile may fill (KiB); 0 = 80 */
 } gc_optionen;
 
 /* Topology of the system, as `System` carries it when the backend builds the
