@@ -64,8 +64,23 @@ def test_system_mit_kontakt_wie_orakel(kleiner_drape, praezision):
     # rounding floor, where the last decade can take a few dozen more iterations
     spiel = la["iterationen"] // 20 if praezision == "fp64" else la["iterationen"] // 8
     assert abs(la["iterationen"] - lb.iterationen) <= max(3, spiel)
-    # both solutions meet the same relative tolerance; their difference is bounded by it
-    assert rel(dva, dvb) <= (1e-5 if praezision == "fp64" else 1e-4)
+    # both solutions meet the same relative residual (rtol 1e-6); how far apart
+    # two such solutions lie is set by the conditioning and by the order of the
+    # FP64 sums (the device groups its partials by tile and block), not by the
+    # precision of the bulk
+    assert rel(dva, dvb) <= 1e-4
+    if praezision == "fp64":
+        # at rtol 1e-10 both reach the filtered system's solution
+        p2 = dict(P, cg_toleranz=1e-10)
+        o.set_param(O.param(**p2))
+        g.set_param(ImplizitParameter(**p2))
+        o.sys_bauen(H)
+        g.bauen(H)
+        dva2, la2 = o.sys_loese(dv0)
+        dvb2, lb2 = g.loese(dv0)
+        print(praezision, "rtol 1e-10: oracle", la2, "device", lb2, "difference", rel(dva2, dvb2))
+        assert not lb2.nicht_konvergiert
+        assert rel(dva2, dvb2) <= 1e-8
     g.close()
 
 
