@@ -830,10 +830,31 @@ __device__ __forceinline__ void biege_beitrag(const V3<T> (&g)[4], T kt, uint2 l
 // boundaries), so each loop runs one kind of constraint without a branch on
 // the kind. (Two constraints per thread and pass, loads first, was measured
 // slower: 7.7 -> 10.0 us per iteration for this stage on the drape.)
+// The per-tile constants of a solve (row range, CSR base, point-list and
+// element-list bounds, first spring entry).
+struct KachelK {
+  int lo, hi;
+  uint32_t basis, v0, v1, e0, e1, e2, e3, fe0;
+};
 template <typename T>
-__device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, int t, V3<T>* slot, const V3<T>* u) {
-  const uint32_t e0 = gm(s.kachel_el_off)[t], e3 = gm(s.kachel_el_off)[t + 1];
-  const uint32_t e1 = gm(s.kachel_kat)[2 * t], e2 = gm(s.kachel_kat)[2 * t + 1];
+__device__ __forceinline__ KachelK kachel_konst(const SysDev<T>& s, int t) {
+  KachelK k;
+  k.lo = gm(s.kachel_zeilen)[2 * t];
+  k.hi = gm(s.kachel_zeilen)[2 * t + 1];
+  k.basis = gm(s.off)[GC_KAT * k.lo];
+  k.v0 = gm(s.kachel_v_off)[t];
+  k.v1 = gm(s.kachel_v_off)[t + 1];
+  k.e0 = gm(s.kachel_el_off)[t];
+  k.e3 = gm(s.kachel_el_off)[t + 1];
+  k.e1 = gm(s.kachel_kat)[2 * t];
+  k.e2 = gm(s.kachel_kat)[2 * t + 1];
+  k.fe0 = s.selbst ? gm(s.fe_off)[k.lo] : 0u;
+  return k;
+}
+
+template <typename T>
+__device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, const KachelK& kk, V3<T>* slot, const V3<T>* u) {
+  const uint32_t e0 = kk.e0, e1 = kk.e1, e2 = kk.e2, e3 = kk.e3;
   const uint32_t bd = blockDim.x;
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += bd) {
     const uint2 lw = lg((const GC_GM uint2*)(s.kachel_lv + 4 * (size_t)e));
@@ -883,6 +904,14 @@ __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_
 // reads the point evaluate this same expression, so they agree to the bit.
 // K = true for a point another block owns: its z and p are read with
 // coherent loads (see lade_k).
+// richtung with the point's flag and normal already at hand (staged in LDS)
+template <typename T>
+__device__ __forceinline__ V3<T> richtung_fn(const SysDev<T>& s, uint32_t j, uint8_t fl, V3<T> n, const GC_GM V3<T>* po, T tb,
+                                              bool erste) {
+  V3<T> q = lade_k(s.z, j);
+  if (!erste) q = q + lade_k(po, j) * tb;
+  return filter_fl(fl, n, q);
+}
 template <bool K, typename T>
 __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const GC_GM V3<T>* po, T tb, bool erste) {
   const uint8_t fl = gm(s.fflag)[j];
@@ -908,6 +937,25 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
   V3<T>* slot = (V3<T>*)lds_roh;
   V3<T>* ul = slot + s.kachel_slots_max;
   V3<T>* fterm = ul + s.kachel_v_max;
+  // a block that owns one tile (the default shape) keeps the tile's constants
+  // in registers and its point list with flags and normals in LDS for the
+  // whole solve (flags and normals change only between solves)
+  V3<T>* kn = fterm + s.fe_lds_kap;
+  uint32_t* kj = (uint32_t*)(kn + s.kachel_v_max);
+  uint8_t* kf = (uint8_t*)(kj + s.kachel_v_max);
+  const bool eine = rang < s.n_kacheln && rang + nb >= s.n_kacheln;
+  KachelK k1{};
+  if (eine) {
+    k1 = kachel_konst(s, rang);
+    for (uint32_t v = k1.v0 + threadIdx.x; v < k1.v1; v += blockDim.x) {
+      const uint32_t j = gm(s.kachel_v)[v];
+      kj[v - k1.v0] = j;
+      kf[v - k1.v0] = gm(s.fflag)[j];
+      kn[v - k1.v0] = lg(gm(s.fnorm) + (j));
+    }
+    __syncthreads();
+  }
+  auto konst = [&](int t) { return eine ? k1 : kachel_konst(s, t); };
   if (durchgang == 1) {
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
@@ -951,13 +999,12 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z) (implizit.rs:385-391)
     double part[2] = {0.0, 0.0};
     for (int t = rang; t < s.n_kacheln; t += nb) {
-      int lo, hi;
-      zeilen(t, &lo, &hi);
-      const uint32_t basis = gm(s.off)[GC_KAT * lo];
-      const uint32_t v0 = gm(s.kachel_v_off)[t], v1 = gm(s.kachel_v_off)[t + 1];
+      const KachelK kk = konst(t);
+      const int lo = kk.lo, hi = kk.hi;
+      const uint32_t basis = kk.basis, v0 = kk.v0, v1 = kk.v1;
       for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = lade_k(s.dv, gm(s.kachel_v)[v]);
       __syncthreads();
-      kachel_beitraege(s, t, slot, ul);
+      kachel_beitraege(s, kk, slot, ul);
       __syncthreads();
       for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const V3<T> hp = feder_zeile(s, (uint32_t)i, kachel_zeile(s, i, basis, slot), [&](uint32_t j) { return lade_k(s.dv, j); });
@@ -998,14 +1045,17 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       // ap = S(A p), ca = p . ap (implizit.rs:401-405)
       double ca[1] = {0.0};
       for (int t = rang; t < s.n_kacheln; t += nb) {
-        int lo, hi;
-        zeilen(t, &lo, &hi);
-        const uint32_t fe0 = s.selbst ? gm(s.fe_off)[lo] : 0u;
-        const uint32_t basis = gm(s.off)[GC_KAT * lo];
-        const uint32_t v0 = gm(s.kachel_v_off)[t], v1 = gm(s.kachel_v_off)[t + 1];
-        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, gm(s.kachel_v)[v], po, tb, erste);
+        const KachelK kk = konst(t);
+        const int lo = kk.lo, hi = kk.hi;
+        const uint32_t fe0 = kk.fe0, basis = kk.basis, v0 = kk.v0, v1 = kk.v1;
+        if (eine) {
+          for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x)
+            ul[v - v0] = richtung_fn(s, kj[v - v0], kf[v - v0], kn[v - v0], po, tb, erste);
+        } else {
+          for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, gm(s.kachel_v)[v], po, tb, erste);
+        }
         __syncthreads();
-        kachel_beitraege(s, t, slot, ul);
+        kachel_beitraege(s, kk, slot, ul);
         if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); }, fterm);
         __syncthreads();
         for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
@@ -1028,7 +1078,12 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       const T ta = T(alpha);
       for (int t = rang; t < s.n_kacheln; t += nb) {
         int lo, hi;
-        zeilen(t, &lo, &hi);
+        if (eine) {
+          lo = k1.lo;
+          hi = k1.hi;
+        } else {
+          zeilen(t, &lo, &hi);
+        }
         for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
           sg(gm(s.dv) + (i), lg(gm(s.dv) + (i)) + lg(pn + i) * ta);
           const V3<T> ri = lg(gm(s.r) + (i)) - lg(gm(s.ap) + (i)) * ta;

@@ -405,14 +405,17 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   const size_t groesste = max_slots + max_v;
   // the CU's LDS left after the largest tile's slots and staging keeps a
   // tile's self-collision spring terms (up to 3072; more go to global memory)
-  const size_t lds_tile = groesste * sizeof(V3<T>), lds_rest = 2048;
+  // the tile's part: slots and staging, plus the point list with flags and
+  // normals that a one-tile block keeps for the whole solve
+  const size_t lds_punkte = (max_v * (sizeof(V3<T>) + sizeof(uint32_t) + 1) + 15) / 16 * 16;
+  const size_t lds_tile = groesste * sizeof(V3<T>) + lds_punkte, lds_rest = 2048;
   size_t fe_kap = lds_tile + lds_rest < c->lds_cu ? (c->lds_cu - lds_tile - lds_rest) / sizeof(V3<T>) : 0;
   fe_kap = std::min<size_t>(fe_kap, 3072);
   s.fe_lds_kap = (uint32_t)fe_kap;
   c->fe_kap_allein = (uint32_t)fe_kap;
   c->pcg_lds_kachel = std::max<size_t>(16, lds_tile);
   // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
-  c->pcg_lds = std::max<size_t>(16, (groesste + fe_kap) * sizeof(V3<T>));
+  c->pcg_lds = std::max<size_t>(16, lds_tile + fe_kap * sizeof(V3<T>));
   (void)hipFuncSetAttribute((const void*)k_pcg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->pcg_lds);
   int occ = 0;
   HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, c->pcg_lds));
