@@ -471,9 +471,7 @@ __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
 // tile on, up to fe_lds_kap of them), added in the order of feder_zeile; four
 // terms are loaded before their sums.
 template <typename T>
-__device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc, const V3<T>* lds, uint32_t e0) {
-  uint32_t e = gm(s.fe_off)[i];
-  const uint32_t b = gm(s.fe_off)[i + 1];
+__device__ __forceinline__ V3<T> feder_summe_ab(const SysDev<T>& s, uint32_t e, uint32_t b, V3<T> acc, const V3<T>* lds, uint32_t e0) {
   auto term = [&](uint32_t q) { return q - e0 < s.fe_lds_kap ? lds[q - e0] : lg(gm(s.fe_t) + (q)); };
   for (; e + 4 <= b; e += 4) {
     V3<T> t[4];
@@ -484,6 +482,10 @@ __device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<
   }
   for (; e < b; ++e) acc = acc + term(e);
   return acc;
+}
+template <typename T>
+__device__ __forceinline__ V3<T> feder_summe(const SysDev<T>& s, uint32_t i, V3<T> acc, const V3<T>* lds, uint32_t e0) {
+  return feder_summe_ab(s, gm(s.fe_off)[i], gm(s.fe_off)[i + 1], acc, lds, e0);
 }
 
 // A*p at row i (implizit.rs:224-229).
@@ -883,8 +885,7 @@ __device__ __forceinline__ void kachel_beitraege(const SysDev<T>& s, const Kache
 
 // H*u of point i from its LDS slots, in CSR order.
 template <typename T>
-__device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_t basis, const V3<T>* slot) {
-  const uint32_t a = gm(s.off)[GC_KAT * i] - basis, b = gm(s.off)[GC_KAT * i + GC_KAT] - basis;
+__device__ __forceinline__ V3<T> kachel_zeile_ab(uint32_t a, uint32_t b, const V3<T>* slot) {
   V3<T> hp = zero3<T>();
   uint32_t q = a;
   // four slots are loaded before their sums; the sums keep CSR order
@@ -897,6 +898,10 @@ __device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_
   }
   for (; q < b; ++q) hp = hp + slot[q];
   return hp;
+}
+template <typename T>
+__device__ __forceinline__ V3<T> kachel_zeile(const SysDev<T>& s, int i, uint32_t basis, const V3<T>* slot) {
+  return kachel_zeile_ab(gm(s.off)[GC_KAT * i] - basis, gm(s.off)[GC_KAT * i + GC_KAT] - basis, slot);
 }
 
 // The search direction of point j, p = S(z + beta p_old) (implizit.rs:420-422);
@@ -931,7 +936,9 @@ __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const 
 // ---------------------------------------------------------------------------
 // The solve of one system by the nb blocks of rank 0..nb-1 that share gs and
 // slots; lds_roh is the block's dynamic LDS.
-template <typename T>
+// ZR: rows may be held in registers (below); off for the batched kernel,
+// whose blocks carry more state and would spill.
+template <typename T, bool ZR>
 __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Slot* slots, int durchgang, int rang, int nb,
                                            unsigned char* lds_roh, double* sh) {
   V3<T>* slot = (V3<T>*)lds_roh;
@@ -956,6 +963,28 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     __syncthreads();
   }
   auto konst = [&](int t) { return eine ? k1 : kachel_konst(s, t); };
+  // FP32 one-tile blocks with at most one row per thread also hold that row's
+  // solve constants (slot and spring ranges, diagonal, flag, normal) in
+  // registers and load its z and previous direction at the start of each
+  // operator pass, beside the staging loads, so stage C reads only LDS.
+  const bool zeile_reg = ZR && sizeof(T) == 4 && eine && k1.hi - k1.lo <= (int)blockDim.x;
+  const int zr = eine ? k1.lo + (int)threadIdx.x : -1;
+  const bool hat_zeile = zeile_reg && zr < k1.hi;
+  uint32_t r_a = 0, r_b = 0, r_fa = 0, r_fb = 0;
+  T r_md = T(0);
+  uint8_t r_fl = 0;
+  V3<T> r_n = zero3<T>();
+  if (hat_zeile) {
+    r_a = gm(s.off)[GC_KAT * zr] - k1.basis;
+    r_b = gm(s.off)[GC_KAT * zr + GC_KAT] - k1.basis;
+    if (s.selbst) {
+      r_fa = gm(s.fe_off)[zr];
+      r_fb = gm(s.fe_off)[zr + 1];
+    }
+    r_md = gm(s.mdiag)[zr];
+    r_fl = gm(s.fflag)[zr];
+    r_n = lg(gm(s.fnorm) + (zr));
+  }
   if (durchgang == 1) {
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
@@ -1048,6 +1077,11 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         const KachelK kk = konst(t);
         const int lo = kk.lo, hi = kk.hi;
         const uint32_t fe0 = kk.fe0, basis = kk.basis, v0 = kk.v0, v1 = kk.v1;
+        V3<T> vor_z = zero3<T>(), vor_p = zero3<T>();
+        if (hat_zeile) {
+          vor_z = lg(gm(s.z) + (zr));
+          if (!erste) vor_p = lg(po + zr);
+        }
         if (eine) {
           for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x)
             ul[v - v0] = richtung_fn(s, kj[v - v0], kf[v - v0], kn[v - v0], po, tb, erste);
@@ -1058,15 +1092,29 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         kachel_beitraege(s, kk, slot, ul);
         if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); }, fterm);
         __syncthreads();
-        for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-          const uint8_t fl = gm(s.fflag)[i];
-          const V3<T> pi = richtung<false>(s, (uint32_t)i, po, tb, erste);
-          const V3<T> hk = kachel_zeile(s, i, basis, slot);
-          const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk, fterm, fe0) : hk;
-          const V3<T> ap = filter_fl(fl, lg(gm(s.fnorm) + (i)), hp * th2 + pi * gm(s.mdiag)[i]);
-          sg(pn + i, pi);
-          sg(gm(s.ap) + (i), ap);
-          ca[0] += (double)dot(pi, ap);
+        if (zeile_reg) {
+          if (hat_zeile) {
+            V3<T> q = vor_z;
+            if (!erste) q = q + vor_p * tb;
+            const V3<T> pi = filter_fl(r_fl, r_n, q);
+            const V3<T> hk = kachel_zeile_ab(r_a, r_b, slot);
+            const V3<T> hp = s.selbst ? feder_summe_ab(s, r_fa, r_fb, hk, fterm, fe0) : hk;
+            const V3<T> ap = filter_fl(r_fl, r_n, hp * th2 + pi * r_md);
+            sg(pn + zr, pi);
+            sg(gm(s.ap) + (zr), ap);
+            ca[0] += (double)dot(pi, ap);
+          }
+        } else {
+          for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+            const uint8_t fl = gm(s.fflag)[i];
+            const V3<T> pi = richtung<false>(s, (uint32_t)i, po, tb, erste);
+            const V3<T> hk = kachel_zeile(s, i, basis, slot);
+            const V3<T> hp = s.selbst ? feder_summe(s, (uint32_t)i, hk, fterm, fe0) : hk;
+            const V3<T> ap = filter_fl(fl, lg(gm(s.fnorm) + (i)), hp * th2 + pi * gm(s.mdiag)[i]);
+            sg(pn + i, pi);
+            sg(gm(s.ap) + (i), ap);
+            ca[0] += (double)dot(pi, ap);
+          }
         }
         __syncthreads();
       }
@@ -1135,7 +1183,7 @@ template <typename T>
 __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
   extern __shared__ __align__(16) unsigned char lds_roh[];
   __shared__ double sh[16 * 2 + 2];
-  pcg_loesen<T>(*sp, gs, slots, durchgang, (int)blockIdx.x, (int)gridDim.x, lds_roh, sh);
+  pcg_loesen<T, true>(*sp, gs, slots, durchgang, (int)blockIdx.x, (int)gridDim.x, lds_roh, sh);
 }
 
 // Batched solves: block b works for system tab[b].sys as rank tab[b].rang of
@@ -1154,7 +1202,7 @@ __global__ void __launch_bounds__(1024) k_pcg_stapel(const StapelBlock<T>* __res
   extern __shared__ __align__(16) unsigned char lds_roh[];
   __shared__ double sh[16 * 2 + 2];
   const StapelBlock<T> e = tab[blockIdx.x];
-  pcg_loesen<T>(*e.sys, e.gs, e.slots, durchgang, e.rang, e.nb, lds_roh, sh);
+  pcg_loesen<T, false>(*e.sys, e.gs, e.slots, durchgang, e.rang, e.nb, lds_roh, sh);
 }
 
 // Release pass (implizit.rs:916-942): normal force from the residual
