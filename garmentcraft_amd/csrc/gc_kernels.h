@@ -282,7 +282,11 @@ __device__ __forceinline__ void barriere_leeren(Slot* slots) {
 // wrote in this launch (partials, counters, dv / z / p of points it does not
 // own) is read with agent-scope loads, which the program order after the
 // observed flag puts after every writer's write-back.
-template <int NV>
+// VOLL = false: the caller's stores before this all-reduce are not read by
+// other blocks until after the next full one, so only the block's partial
+// (an agent-coherent store) has to be complete before the arrival; the L2
+// write-back of the release is left to that next all-reduce.
+template <int NV, bool VOLL = true>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
                                                unsigned rang, unsigned nb) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
@@ -310,7 +314,8 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
       const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE, g = rang / GC_GRUPPE;
       const unsigned gsz = nb - g * GC_GRUPPE < GC_GRUPPE ? nb - g * GC_GRUPPE : GC_GRUPPE;
       unsigned* flagge = barr_zaehler(slots, GC_BARR_GRUPPEN + 1 + g);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if constexpr (VOLL) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      else __builtin_amdgcn_s_waitcnt(0x0F70);  // s_waitcnt vmcnt(0): the partial's store is complete
       const unsigned alt = __hip_atomic_fetch_add(barr_zaehler(slots, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (alt == gsz * epoch - 1u) {  // last of its group
         const unsigned alt2 =
@@ -1118,7 +1123,9 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         }
         __syncthreads();
       }
-      if (!grid_allreduce<1>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+      // p and A p are read by other blocks only in the next operator pass,
+      // after the r.z all-reduce, whose release writes them back
+      if (!grid_allreduce<1, false>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
       if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
       const double alpha = delta / ca[0];
       // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
