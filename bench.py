@@ -137,7 +137,7 @@ def main() -> int:
             # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
             # drape, scaled by the counters of a plain copy of known size
             # (profiles/k_pcg_drape40k_pmc.md): bytes per PCG iteration
-            roofline["traffic"] = 5124684
+            roofline["traffic"] = 5137583
             roofline["traffic_unit"] = "B per PCG iteration (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB x 1024)"
 
     cpu = None
