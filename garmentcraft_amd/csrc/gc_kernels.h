@@ -174,6 +174,7 @@ struct SysDev {
   const uint32_t* kachel_v;       // points touched by the tile's constraints, global index
   uint32_t kachel_slots_max;      // LDS slot area (entries) of the largest tile
   uint32_t kachel_v_max;          // LDS staging area (points) of the largest tile
+  uint32_t punkte_lds;            // 1: a one-tile block keeps its point list, flags and normals in LDS
   // the records of every tile entry at its entry index, copied from the
   // per-constraint arrays after each build (k_kachel_saetze), so stage B
   // loads them directly instead of through the constraint index; an entry
@@ -956,9 +957,10 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
   uint32_t* kj = (uint32_t*)(kn + s.kachel_v_max);
   uint8_t* kf = (uint8_t*)(kj + s.kachel_v_max);
   const bool eine = rang < s.n_kacheln && rang + nb >= s.n_kacheln;
+  const bool eine_lds = eine && s.punkte_lds;
   KachelK k1{};
-  if (eine) {
-    k1 = kachel_konst(s, rang);
+  if (eine) k1 = kachel_konst(s, rang);
+  if (eine_lds) {
     for (uint32_t v = k1.v0 + threadIdx.x; v < k1.v1; v += blockDim.x) {
       const uint32_t j = gm(s.kachel_v)[v];
       kj[v - k1.v0] = j;
@@ -1087,7 +1089,7 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
           vor_z = lg(gm(s.z) + (zr));
           if (!erste) vor_p = lg(po + zr);
         }
-        if (eine) {
+        if (eine_lds) {
           for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x)
             ul[v - v0] = richtung_fn(s, kj[v - v0], kf[v - v0], kn[v - v0], po, tb, erste);
         } else {

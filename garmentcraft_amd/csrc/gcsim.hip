@@ -412,8 +412,10 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   size_t fe_kap = lds_tile + lds_rest < c->lds_cu ? (c->lds_cu - lds_tile - lds_rest) / sizeof(V3<T>) : 0;
   fe_kap = std::min<size_t>(fe_kap, 3072);
   s.fe_lds_kap = (uint32_t)fe_kap;
+  s.punkte_lds = 1;
   c->fe_kap_allein = (uint32_t)fe_kap;
-  c->pcg_lds_kachel = std::max<size_t>(16, lds_tile);
+  // a batch keeps no point lists in LDS (so two 512-thread blocks still fit a CU)
+  c->pcg_lds_kachel = std::max<size_t>(16, groesste * sizeof(V3<T>));
   // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
   c->pcg_lds = std::max<size_t>(16, lds_tile + fe_kap * sizeof(V3<T>));
   (void)hipFuncSetAttribute((const void*)k_pcg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->pcg_lds);
@@ -1097,10 +1099,11 @@ static int tabelle_laden(gc_batch* b) {
   return 0;
 }
 
-// LDS of a batched launch: the largest tile of its systems. Each system keeps
-// as many spring terms in LDS as fit beside its own tile in that size (never
-// more than when it runs alone), so a batch gets the blocks per CU of its tiles.
-// The counts reach the device with the next upload of each system.
+// LDS of a batched launch: the largest tile of its systems (slots and
+// staging; no point lists). Each system keeps as many spring terms in LDS as
+// fit beside its own tile in that size (never more than when it runs alone),
+// so a batch gets the blocks per CU of its tiles. The settings reach the
+// device with the next upload of each system.
 template <typename T>
 static size_t batch_lds(gc_batch* b) {
   size_t lds = 16;
@@ -1108,6 +1111,7 @@ static size_t batch_lds(gc_batch* b) {
   for (gc_ctx* c : b->ctx) {
     const size_t frei = (lds - c->pcg_lds_kachel) / sizeof(V3<T>);
     sys<T>(c).fe_lds_kap = (uint32_t)std::min<size_t>(frei, c->fe_kap_allein);
+    sys<T>(c).punkte_lds = 0;
   }
   return lds;
 }
