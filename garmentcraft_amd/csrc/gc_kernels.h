@@ -947,6 +947,11 @@ __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const 
 template <typename T, bool ZR>
 __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Slot* slots, int durchgang, int rang, int nb,
                                            unsigned char* lds_roh, double* sh) {
+  // the second solve runs only if the release pass freed a point (the same
+  // answer for every block of the launch)
+  if (durchgang == 1) {
+    if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  }
   V3<T>* slot = (V3<T>*)lds_roh;
   V3<T>* ul = slot + s.kachel_slots_max;
   V3<T>* fterm = ul + s.kachel_v_max;
@@ -991,9 +996,6 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     r_md = gm(s.mdiag)[zr];
     r_fl = gm(s.fflag)[zr];
     r_n = lg(gm(s.fnorm) + (zr));
-  }
-  if (durchgang == 1) {
-    if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
   unsigned epoch = 0;
   auto zeilen = [&](int t, int* lo, int* hi) { *lo = gm(s.kachel_zeilen)[2 * t]; *hi = gm(s.kachel_zeilen)[2 * t + 1]; };
