@@ -130,6 +130,8 @@ struct SelbstDev {
   uint32_t* fk_off;                 // n + 1
   uint32_t* fk_ent;                 // force entries: spring << 3 | q
   uint32_t kapazitaet_fk_ent;
+  uint32_t* fe_roh;                 // fe_ent and fk_ent as filled (unordered per point)
+  uint32_t* fk_roh;
   uint32_t* tausch;                 // scratch of the region merge (2 per pair)
   uint32_t kapazitaet_tausch;
   uint32_t* uf;                     // scratch of the region merge (3 per cut)
@@ -1337,16 +1339,17 @@ __global__ void k_sk_feder_fuellen(const SelbstDev* __restrict__ sp, uint32_t* f
     const uint32_t* p = s.fe_punkte + 5 * f;
     const double* g = s.fe_gew + 5 * f;
     for (int q = 0; q < 5; ++q) {
-      if (g[q] != 0.0) s.fk_ent[atomicAdd(&fk_fuell[p[q]], 1u)] = (f << 3) | (uint32_t)q;
+      if (g[q] != 0.0) s.fk_roh[atomicAdd(&fk_fuell[p[q]], 1u)] = (f << 3) | (uint32_t)q;
       for (int r = 0; r < 5; ++r)
         if (g[q] > 0.0 && g[r] < 0.0) {
           const uint32_t code = (f << 5) | (uint32_t)(5 * q + r);
-          s.fe_ent[atomicAdd(&fe_fuell[p[q]], 1u)] = code;
-          s.fe_ent[atomicAdd(&fe_fuell[p[r]], 1u)] = code;
+          s.fe_roh[atomicAdd(&fe_fuell[p[q]], 1u)] = code;
+          s.fe_roh[atomicAdd(&fe_fuell[p[r]], 1u)] = code;
         }
     }
   }
 }
+// Insertion sort of a[lo, hi) (short lists: the impacts of one point).
 __device__ __forceinline__ void sk_sortiere(uint32_t* a, uint32_t lo, uint32_t hi) {
   for (uint32_t i = lo + 1; i < hi; ++i) {
     const uint32_t v = a[i];
@@ -1358,12 +1361,31 @@ __device__ __forceinline__ void sk_sortiere(uint32_t* a, uint32_t lo, uint32_t h
     a[j] = v;
   }
 }
+
+// Every entry of a per-point list to its place in ascending order: its
+// rank among the entries of its list (the smaller ones, and the equal ones
+// filled before it). Parallel over entries, so a point with a long list costs
+// its length per entry, not its length squared in one thread. Equal entries
+// are equal codes, so the lists are those of a sequential sort.
+__device__ __forceinline__ void sk_rang_platz(const uint32_t* roh, uint32_t* ent, const uint32_t* off, int n, uint32_t p) {
+  uint32_t u = 0, o = (uint32_t)n;  // the list: the last i with off[i] <= p
+  while (o - u > 1) {
+    const uint32_t m = (u + o) >> 1;
+    if (off[m] <= p) u = m;
+    else o = m;
+  }
+  const uint32_t a = off[u], b = off[u + 1], v = roh[p];
+  uint32_t rang = 0;
+  for (uint32_t q = a; q < b; ++q) rang += (roh[q] < v || (roh[q] == v && q < p)) ? 1u : 0u;
+  ent[a + rang] = v;
+}
 __global__ void k_sk_feder_sortieren(const SelbstDev* __restrict__ sp) {
   const SelbstDev& s = *sp;
   if (s.zaehler[SK_Z_FEDER_AUS]) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
-    sk_sortiere(s.fe_ent, s.fe_off[i], s.fe_off[i + 1]);
-    sk_sortiere(s.fk_ent, s.fk_off[i], s.fk_off[i + 1]);
+  const uint32_t ne = s.fe_off[s.n], nk = s.fk_off[s.n];
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < ne + nk; p += gridDim.x * blockDim.x) {
+    if (p < ne) sk_rang_platz(s.fe_roh, s.fe_ent, s.fe_off, s.n, p);
+    else sk_rang_platz(s.fk_roh, s.fk_ent, s.fk_off, s.n, p - ne);
   }
 }
 // Operator records of the springs in the bulk type.

@@ -650,6 +650,8 @@ static int selbst_vorbereiten(gc_ctx* c) {
   k.fe_ent = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(k.fe_ent);
   k.fk_off = dalloc<uint32_t>(c, (size_t)n + 1); chk(k.fk_off);
   k.fk_ent = dalloc<uint32_t>(c, k.kapazitaet_fk_ent); chk(k.fk_ent);
+  k.fe_roh = dalloc<uint32_t>(c, k.kapazitaet_fe_ent); chk(k.fe_roh);
+  k.fk_roh = dalloc<uint32_t>(c, k.kapazitaet_fk_ent); chk(k.fk_roh);
   k.st_off = dalloc<uint32_t>(c, (size_t)n + 1); chk(k.st_off);
   k.st_ent = dalloc<uint32_t>(c, k.kapazitaet_st_ent); chk(k.st_ent);
   k.st_dx = dalloc<V3<double>>(c, 5 * ((size_t)nt + ne)); chk(k.st_dx);
