@@ -39,3 +39,52 @@ def test_keine_gpu_meldet_sich_laut():
     h = L.gc_create(ctypes.byref(opt), err, 256)
     assert not h
     assert err.value
+
+
+# header struct -> its ctypes mirror in garmentcraft_amd/_lib.py
+STRUKTUREN = {"gc_optionen": "Optionen", "gc_topologie": "Topologie", "gc_param": "Param", "gc_cg_lauf": "CgLauf_c",
+              "gc_statistik": "Statistik_c", "gc_adjoint_stat": "AdjointStat_c"}
+C_TYPEN = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "double": ctypes.c_double, "int64_t": ctypes.c_int64,
+           "uint64_t": ctypes.c_uint64}
+
+
+def strukturen():
+    """Field names, element types and array lengths of every typedef struct in the header, in order."""
+    text = open(os.path.join(WURZEL, "include", "gcsim.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    aus = {}
+    for name, koerper in re.findall(r"typedef struct (gc_\w+) \{(.*?)\} \1;", text, flags=re.S):
+        felder = []
+        for decl in koerper.split(";"):
+            decl = " ".join(decl.split())
+            if not decl:
+                continue
+            m = re.match(r"(?:const )?(\w+)\s*(\*?)\s*(\w+)(?:\[(\d+)\])?$", decl)
+            assert m, (name, decl)
+            typ, zeiger, feld, laenge = m.groups()
+            for f in feld.split(","):
+                felder.append((f, typ, bool(zeiger), int(laenge) if laenge else 0))
+        aus[name] = felder
+    return aus
+
+
+def test_strukturen_wie_im_header():
+    """Every struct the header hands across the boundary has a ctypes mirror
+    with the same fields in the same order, the same scalar types, pointers
+    where the header has pointers and the same array lengths, so the two
+    layouts agree (an added or moved field fails here, without a GPU)."""
+    kopf = strukturen()
+    assert set(STRUKTUREN) <= set(kopf), sorted(set(STRUKTUREN) - set(kopf))
+    for c_name, py_name in STRUKTUREN.items():
+        py = getattr(_lib, py_name)
+        felder = kopf[c_name]
+        assert [f for f, *_ in felder] == [f for f, _ in py._fields_], c_name
+        for (f, typ, zeiger, laenge), (_, ct) in zip(felder, py._fields_):
+            if zeiger:
+                assert issubclass(ct, ctypes._Pointer) or ct in (ctypes.c_void_p,), (c_name, f, ct)
+                continue
+            erwartet = C_TYPEN[typ]
+            if laenge:
+                assert issubclass(ct, ctypes.Array) and ct._length_ == laenge and ct._type_ == erwartet, (c_name, f, ct)
+            else:
+                assert ct == erwartet, (c_name, f, ct)
