@@ -62,7 +62,7 @@ class Statistik_c(C.Structure):
         ("barriere_spins_max", C.c_int64),
         ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
         ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64), ("durchstoesse", C.c_int64),
-        ("cg_rundungsgrenze", C.c_int64), ("cg_neustarts", C.c_int64),
+        ("cg_rundungsgrenze", C.c_int64), ("cg_neustarts", C.c_int64), ("selbst_listen_faktor", C.c_int64),
     ]
 
 

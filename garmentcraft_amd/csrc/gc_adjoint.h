@@ -172,7 +172,7 @@ __global__ void k_adj_update(const SysDev<T>* __restrict__ sp, AdjDev a, V3<T>* 
       continue;
     }
     const uint8_t fl = s.fflag[i];
-    const V3<double> x0 = a.x0[i], v0 = a.v0[i], dv = cast3<double>(s.dv[i]);
+    const V3<double> x0 = a.x0[i], v0 = a.v0[i], dv = dv_loesung(s, (uint32_t)i);
     const double fn = s.fn_kraft[i], mu = s.reib[i];
     double g[11];
     int eintritte = 0;
@@ -223,7 +223,7 @@ __global__ void k_adj_kanten(const SysDev<T>* __restrict__ sp, AdjDev a, const V
     const double k0 = s.k_k[c], l0in = s.k_ruhe[c];
     const bool naht = s.k_naht[c] != 0;
     const V3<double> mi = cast3<double>(mu[i]), mj = cast3<double>(mu[j]);
-    const V3<double> ud = (a.v0[i] + cast3<double>(s.dv[i])) - (a.v0[j] + cast3<double>(s.dv[j]));
+    const V3<double> ud = (a.v0[i] + dv_loesung(s, (uint32_t)i)) - (a.v0[j] + dv_loesung(s, (uint32_t)j));
     double g[7];
     for (int r = 0; r < 7; ++r) {
       const V3<Dual> xi = saat(a.x0[i], r, 0), xj = saat(a.x0[j], r, 3);
@@ -266,7 +266,7 @@ __global__ void k_adj_biegung(const SysDev<T>* __restrict__ sp, AdjDev a, const 
     for (int m = 0; m < 4; ++m) {
       xs[m] = a.x0[bp[m]];
       ms[m] = cast3<double>(mu[bp[m]]);
-      us[m] = a.v0[bp[m]] + cast3<double>(s.dv[bp[m]]);
+      us[m] = a.v0[bp[m]] + dv_loesung(s, bp[m]);
     }
     const double k0 = s.b_k[c], ruhe = s.b_ruhe[c];
     double g[13];
@@ -308,7 +308,7 @@ __global__ void k_adj_membran(const SysDev<T>* __restrict__ sp, AdjDev a, const 
     for (int k = 0; k < 3; ++k) {
       xs[k] = a.x0[mp[k]];
       ms[k] = cast3<double>(mu[mp[k]]);
-      us[k] = a.v0[mp[k]] + cast3<double>(s.dv[mp[k]]);
+      us[k] = a.v0[mp[k]] + dv_loesung(s, mp[k]);
     }
     const Membran<double> el = s.m_el64[c];
     double g[12];
@@ -362,7 +362,7 @@ __global__ void k_adj_sammeln(const SysDev<T>* __restrict__ sp, AdjDev a, const 
     gv = gv - mui * (s.h * s.c * s.masse[i]) - cast3<double>(hm) * s.h2;
     if (s.fflag[i] == 1 || s.fflag[i] == 3) {  // held in the last solve
       const double lam = s.h * s.fn_kraft[i], nu = a.nu[i];
-      const V3<double> w = mui * lam - cast3<double>(s.dv[i]) * nu;
+      const V3<double> w = mui * lam - dv_loesung(s, (uint32_t)i) * nu;
       double g[6];
       for (int r = 0; r < 6; ++r) {
         const V3<Dual> X = saat(a.x0[i], r, 0), V = saat(a.v0[i], r, 3);

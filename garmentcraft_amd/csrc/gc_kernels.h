@@ -158,6 +158,14 @@ struct SysDev {
   V3<T>* p[2];
   V3<T>* ap;
   V3<double>* dv_vorher;
+  // FP32 bulk only (null in FP64): the solve's solution in FP64, and the
+  // device address of an FP64 copy of this system (SysDev<double>: the same
+  // topology with FP64 operator records, right-hand side and mass diagonal).
+  // The FP32 PCG then solves for corrections of dv64 whose residual is taken
+  // in FP64 from that copy (iterative refinement), so its result converges to
+  // the FP64 system's solution rather than to the FP32-rounded one's.
+  V3<double>* dv64;
+  const void* schatten;
   // normal target of held points for the solve; null: vn_soll - v.n of the
   // step (implizit.rs:375-384); set: the adjoint solve's own targets
   const double* soll_direkt;
@@ -207,11 +215,17 @@ struct SysDev {
 
 // Shared device state of the grid-wide reductions and the step's counters.
 struct GridState {
-  unsigned int zaehler;      // barrier arrivals (monotonic within a launch, reset before each)
-  unsigned int fehler;       // spin limit hit
+  unsigned int zaehler;      // epoch of the last grid barrier, carried from launch to launch within a
+                             // refined FP32 solve (k_verf, k_pcg rounds, k_pcg64: same grid, so the
+                             // barrier counters stay monotonic); 0 whenever the counters are cleared
+  unsigned int fehler;       // 1: spin limit hit; 2: an all-reduce's partials were not all of its barrier
   unsigned int frei[2];      // a point was released in pass k
   unsigned int n_fest;       // points held along a normal
-  unsigned int pad0[3];
+  // FP32 bulk with the FP64 copy (k_verf rounds): the solve has converged or
+  // ended; it is handed over to FP64 iterations (k_pcg64)
+  unsigned int fertig;
+  unsigned int uebergabe;
+  unsigned int pad0;
   // statistics of the last step (device side)
   long long cg_iterationen, loesungen, nicht_konvergiert, kontakte, haftend, freigegeben;
   double rest_max;
@@ -225,11 +239,13 @@ struct GridState {
   int nicht_konv;
   double rest;
   double delta, ziel, bb;
+  double delta_best;         // k_verf: FP64 residual at the FP64 solution dv64
 };
 
 #define GC_MAX_BLOECKE 1024
 struct alignas(64) Slot {
   double v[4];
+  unsigned epoche;  // the barrier whose partial v holds (the all-reduce's check)
 };
 
 // Arrival counters of the grid barrier, after the two partial buffers of a
@@ -290,6 +306,7 @@ __device__ __forceinline__ void barriere_leeren(Slot* slots) {
 template <int NV, bool VOLL = true>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
                                                unsigned rang, unsigned nb) {
+  static_assert(NV <= 4, "a slot holds four values");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k)
@@ -312,6 +329,9 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
         for (int w = 0; w < nw; ++w) a += sh[w * NV + k];
         __hip_atomic_store(&buf[rang].v[k], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      // check: the barrier's epoch beside the partial; a partial left from an
+      // earlier barrier or launch (stale counters) fails the all-reduce
+      __hip_atomic_store(&buf[rang].epoche, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE, g = rang / GC_GRUPPE;
       const unsigned gsz = nb - g * GC_GRUPPE < GC_GRUPPE ? nb - g * GC_GRUPPE : GC_GRUPPE;
       unsigned* flagge = barr_zaehler(slots, GC_BARR_GRUPPEN + 1 + g);
@@ -339,28 +359,37 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
     // l, l + 64, ..., then the wave's butterfly
     double acc[NV];
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    bool gleich = true;  // every partial read is of this barrier
     if (!fehler)
       for (unsigned b0 = lane; b0 < nb; b0 += 256) {
         // four partials per lane are loaded before their sums, which keep the
         // order b0, b0 + 64, ...
         double x[4][NV];
+        unsigned ep[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
+          const bool da = b0 + 64u * u < nb;
 #pragma unroll
           for (int k = 0; k < NV; ++k)
-            x[u][k] = b0 + 64u * u < nb ? __hip_atomic_load(&buf[b0 + 64u * u].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+            x[u][k] = da ? __hip_atomic_load(&buf[b0 + 64u * u].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+          ep[u] = da ? __hip_atomic_load(&buf[b0 + 64u * u].epoche, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : epoch;
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
+          gleich = gleich && ep[u] == epoch;
           if (b0 + 64u * u < nb)
 #pragma unroll
             for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
+        }
       }
 #pragma unroll
     for (int k = 0; k < NV; ++k)
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (!fehler && __ballot(!gleich) != 0ull) fehler = 2;
     if (lane == 0) {
       for (int k = 0; k < NV; ++k) aus[k] = acc[k];
+      if (fehler == 2) __hip_atomic_store(&gs->fehler, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_fehler = fehler;
     }
   }
@@ -732,7 +761,10 @@ __global__ void k_reset(GridState* gs, Slot* slots) {
 
 // Clears a system's barrier counters before a PCG launch that no k_reset or
 // release pass precedes.
-__global__ void k_barriere_leeren(Slot* slots) { barriere_leeren(slots); }
+__global__ void k_barriere_leeren(GridState* gs, Slot* slots) {
+  if (threadIdx.x == 0) gs->zaehler = 0;
+  barriere_leeren(slots);
+}
 
 // Copies every tile entry's operator record from the per-constraint arrays to
 // its entry index (after each build; the records are constant during a solve).
@@ -768,8 +800,17 @@ __global__ void k_warm(const SysDev<T>* __restrict__ sp, int hat_vorher) {
   const SysDev<T>& s = *sp;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     const V3<double> gh = V3<double>{s.g[0], s.g[1], s.g[2]} * s.h;
-    s.dv[i] = cast3<T>(hat_vorher ? s.dv_vorher[i] : gh);
+    const V3<double> w = hat_vorher ? s.dv_vorher[i] : gh;
+    s.dv[i] = cast3<T>(w);
+    if (s.dv64) s.dv64[i] = w;
   }
+}
+
+// The step's solution dv at point i in FP64: dv64 where the FP32 solve
+// refines in FP64, the bulk-type dv otherwise.
+template <typename T>
+__device__ __forceinline__ V3<double> dv_loesung(const SysDev<T>& s, uint32_t i) {
+  return s.dv64 ? s.dv64[i] : cast3<double>(s.dv[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -944,6 +985,262 @@ __device__ __forceinline__ V3<T> richtung(const SysDev<T>& s, uint32_t j, const 
 // slots; lds_roh is the block's dynamic LDS.
 // ZR: rows may be held in registers (below); off for the batched kernel,
 // whose blocks carry more state and would spill.
+// Iterative refinement of the FP32 solve (SysDev::schatten), over the rows of
+// the block's tiles.
+//
+// dv64 += dv, dv = 0: the correction of the last iterations folded in.
+template <typename T>
+__device__ __forceinline__ void falten_fp64(const SysDev<T>& s, int rang, int nb) {
+  for (int t = rang; t < s.n_kacheln; t += nb) {
+    const int lo = gm(s.kachel_zeilen)[2 * t], hi = gm(s.kachel_zeilen)[2 * t + 1];
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      s.dv64[i] = s.dv64[i] + cast3<double>(lg(gm(s.dv) + (i)));
+      sg(gm(s.dv) + (i), zero3<T>());
+    }
+  }
+}
+
+// dv = 0: the correction of the last iterations dropped (it did not lower
+// the FP64 residual).
+template <typename T>
+__device__ __forceinline__ void verwerfen_fp64(const SysDev<T>& s, int rang, int nb) {
+  for (int t = rang; t < s.n_kacheln; t += nb) {
+    const int lo = gm(s.kachel_zeilen)[2 * t], hi = gm(s.kachel_zeilen)[2 * t + 1];
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) sg(gm(s.dv) + (i), zero3<T>());
+  }
+}
+
+// r = S(b - A x) of the FP64 system at x = dv64 (+ dv where mit_dv);
+// part[0] += sum S(b) P S(b), part[1] += r . S(P r), in FP64 (P, the
+// bulk-type preconditioner, applied in FP64); r and z = P r go to the bulk
+// type for the iterations. dv64 and dv of other blocks' points are read with
+// agent-scope loads (after the grid barrier that follows their last writes).
+template <typename T>
+__device__ __forceinline__ void rest_fp64(const SysDev<T>& s, int rang, int nb, bool mit_dv, double* part) {
+  const SysDev<double>& sd = *(const SysDev<double>*)s.schatten;
+  for (int t = rang; t < s.n_kacheln; t += nb) {
+    const int lo = gm(s.kachel_zeilen)[2 * t], hi = gm(s.kachel_zeilen)[2 * t + 1];
+    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      const V3<double> ap = mal_a_zeile(sd, (uint32_t)i, [&](uint32_t j) {
+        const V3<double> x = lade_k(s.dv64, j);
+        return mit_dv ? x + cast3<double>(lade_k(s.dv, j)) : x;
+      });
+      const uint8_t fl = gm(s.fflag)[i];
+      const V3<double> n = lg(gm(s.fnorm64) + (i));
+      const V3<double> bi = sd.b[i];
+      const V3<double> ri = filter_fl(fl, n, bi - ap);
+      const GC_GM T* P = gm(s.pinv) + 9 * (size_t)i;
+      auto pmal = [&](V3<double> v) {
+        return V3<double>{(double)P[0] * v.x + (double)P[1] * v.y + (double)P[2] * v.z,
+                          (double)P[3] * v.x + (double)P[4] * v.y + (double)P[5] * v.z,
+                          (double)P[6] * v.x + (double)P[7] * v.y + (double)P[8] * v.z};
+      };
+      if (fl != 2) {
+        const V3<double> fb = filter_fl(fl, n, bi);
+        part[0] += dot(fb, pmal(fb));
+      }
+      part[1] += dot(ri, filter_fl(fl, n, pmal(ri)));
+      const V3<T> rt = cast3<T>(ri);
+      sg(gm(s.r) + (i), rt);
+      sg(gm(s.z) + (i), m3_mal_g(P, rt));
+    }
+  }
+}
+
+// The outcome of a refined FP32 solve (after its k_verf rounds and, if
+// handed over, its FP64 iterations), as k_pcg reports an unrefined one.
+template <typename T>
+__device__ __forceinline__ void loesung_zaehlen(const SysDev<T>& s, GridState* gs, bool uebergeben) {
+  const double delta = gs->delta, ziel = gs->ziel;
+  gs->nicht_konv = delta <= ziel ? 0 : 1;  // a NaN delta counts as not converged
+  gs->rest = sqrt(delta / gmax(gs->bb, s.boden));
+  gs->loesungen += 1;
+  if (!(delta <= ziel)) gs->nicht_konvergiert += 1;
+  if (uebergeben) gs->rundungsgrenze += 1;
+  gs->rest_max = gmax(gs->rest_max, gs->rest);
+}
+
+// k_verf: the FP64 side of a refined FP32 solve (SysDev::schatten), between
+// the rounds of FP32 iterations (k_pcg), one launch per step of the
+// sequence start, round, check, round, check, ..., k_pcg64:
+// - modus 0 (start): held points get their target normal velocity in the
+//   FP64 solution dv64 (implizit.rs:375-384), the correction dv is cleared;
+//   r = S(b - A dv64) of the FP64 system, bb, delta and the stopping target
+//   (implizit.rs:385-400), r and z = P r in the bulk type for the round.
+// - modus 1 (check): the residual at dv64 + dv. A correction that lowered
+//   the FP64 residual is folded into dv64; one that did not (a round that
+//   diverged in FP32) is dropped and the residual taken at dv64 again. The
+//   solve is done when that residual meets the stopping test or the
+//   iteration budget is spent; otherwise the next round starts from it, and
+//   after the last round (letzte) the solve is handed over to FP64
+//   iterations (k_pcg64).
+template <typename T>
+__device__ __forceinline__ void verf_schritt(const SysDev<T>& s, GridState* gs, Slot* slots, int durchgang, int modus, int letzte,
+                                             int rang, int nb, double* sh) {
+  if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  if (modus == 1 && __hip_atomic_load(&gs->fertig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  // every block reads the epoch the solve's last launch ended at before its
+  // first arrival, so rank 0 (which passes the last barrier only after every
+  // block arrived) can store this launch's last epoch for the next one
+  unsigned epoch = __hip_atomic_load(&gs->zaehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (modus == 0) {
+    for (int t = rang; t < s.n_kacheln; t += nb) {
+      const int lo = gm(s.kachel_zeilen)[2 * t], hi = gm(s.kachel_zeilen)[2 * t + 1];
+      for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint8_t fl = gm(s.fflag)[i];
+        sg(gm(s.dv) + (i), zero3<T>());
+        if (fl == 2) {
+          s.dv64[i] = zero3<double>();
+        } else if (fl != 0) {
+          const V3<double> n = lg(gm(s.fnorm64) + (i));
+          const double soll = s.soll_direkt ? gm(s.soll_direkt)[i] : gm(s.vn_soll)[i] - dot(lg(gm(s.v) + (i)), n);
+          s.dv64[i] = filter_fl(fl, n, s.dv64[i]) + n * soll;
+        }
+      }
+    }
+    double nix[1] = {0.0};
+    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    double part[2] = {0.0, 0.0};
+    rest_fp64(s, rang, nb, false, part);
+    if (!grid_allreduce<2>(part, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    if (rang == 0 && threadIdx.x == 0) {
+      const double bb = part[0], delta = part[1];
+      const double ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
+      gs->bb = bb;
+      gs->ziel = ziel;
+      gs->delta = delta;
+      gs->delta_best = delta;
+      gs->iterationen = 0;
+      gs->uebergabe = 0u;
+      gs->fertig = delta <= ziel ? 1u : 0u;
+      gs->zaehler = epoch;
+    }
+    return;
+  }
+  const double best = gs->delta_best;
+  double delta = 0.0, neu_best = best;
+  for (bool mit_dv = true;;) {
+    double part[2] = {0.0, 0.0};
+    rest_fp64(s, rang, nb, mit_dv, part);
+    if (!grid_allreduce<2>(part, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    delta = part[1];
+    if (!mit_dv) break;  // the residual at dv64 alone, after a dropped correction
+    if (delta <= best) {
+      // the correction lowered the FP64 residual: dv64 += dv (every block
+      // has read dv and dv64 before the all-reduce above)
+      falten_fp64(s, rang, nb);
+      neu_best = delta;
+      break;
+    }
+    verwerfen_fp64(s, rang, nb);  // dropped
+    mit_dv = false;
+  }
+  if (rang == 0 && threadIdx.x == 0) {
+    gs->zaehler = epoch;
+    gs->delta = delta;
+    gs->delta_best = neu_best;
+    if (delta <= gs->ziel || gs->iterationen >= s.cg_max) {
+      gs->fertig = 1u;
+    } else {
+      gs->neustarts += 1;
+      if (letzte) {
+        gs->uebergabe = 1u;
+        gs->fertig = 1u;
+      }
+    }
+  }
+}
+
+// The FP32 solve's last resort: when refinement makes no headway (the
+// system is too ill-conditioned for FP32 iterations to lower the FP64
+// residual), k_pcg hands the solve over (GridState::uebergabe) and
+// k_pcg64, launched after every FP32 PCG launch, goes on from dv64 as the
+// reference's modified PCG (implizit.rs:385-417) entirely in FP64, on the
+// FP64 copy of the system: its records through the per-point CSR lists, its
+// FP64 preconditioner, and FP64 vectors r, z, p, A p of its own. Three grid
+// barriers per iteration (the direction p is gathered by other blocks'
+// rows). A launch in which nothing was handed over returns at once; its own
+// kernel keeps this loop's registers out of k_pcg's.
+template <typename T>
+__device__ __forceinline__ void pcg64_loesen(const SysDev<T>& s, GridState* gs, Slot* slots, int durchgang, int rang, int nb,
+                                             double* sh) {
+  if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  if (__hip_atomic_load(&gs->uebergabe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+    if (rang == 0 && threadIdx.x == 0) loesung_zaehlen(s, gs, false);
+    return;
+  }
+  const SysDev<double>& sd = *(const SysDev<double>*)s.schatten;
+  V3<double>* const r = sd.r;
+  V3<double>* const z = sd.z;
+  V3<double>* const p = sd.p[0];
+  V3<double>* const ap = sd.ap;
+  const double ziel = gs->ziel;
+  int it = gs->iterationen;
+  unsigned epoch = __hip_atomic_load(&gs->zaehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  auto zeilen = [&](auto f) {
+    for (int t = rang; t < s.n_kacheln; t += nb) {
+      const int lo = gm(s.kachel_zeilen)[2 * t], hi = gm(s.kachel_zeilen)[2 * t + 1];
+      for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) f(i);
+    }
+  };
+  // r = S(b - A dv64), z = P r, delta = r . S(z)
+  double part[1] = {0.0};
+  zeilen([&](int i) {
+    const uint8_t fl = gm(s.fflag)[i];
+    const V3<double> n = lg(gm(s.fnorm64) + (i));
+    const V3<double> ri = filter_fl(fl, n, sd.b[i] - mal_a_zeile(sd, (uint32_t)i, [&](uint32_t j) { return lade_k(s.dv64, j); }));
+    const V3<double> zi = m3_mal(sd.pinv + 9 * (size_t)i, ri);
+    r[i] = ri;
+    z[i] = zi;
+    part[0] += dot(ri, filter_fl(fl, n, zi));
+  });
+  if (!grid_allreduce<1>(part, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+  double delta = part[0];
+  const int it0 = it;
+  double beta = 0.0;
+  for (bool erste = true; delta > ziel && it < s.cg_max; erste = false) {
+    // p = S(z + beta p)
+    zeilen([&](int i) {
+      const V3<double> q = erste ? z[i] : z[i] + p[i] * beta;
+      p[i] = filter_fl(gm(s.fflag)[i], lg(gm(s.fnorm64) + (i)), q);
+    });
+    double nix[1] = {0.0};
+    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    // ap = S(A p), ca = p . ap
+    double ca[1] = {0.0};
+    zeilen([&](int i) {
+      const V3<double> a = filter_fl(gm(s.fflag)[i], lg(gm(s.fnorm64) + (i)),
+                                     mal_a_zeile(sd, (uint32_t)i, [&](uint32_t j) { return lade_k(p, j); }));
+      ap[i] = a;
+      ca[0] += dot(p[i], a);
+    });
+    if (!grid_allreduce<1, false>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
+    const double alpha = delta / ca[0];
+    double neu[1] = {0.0};
+    zeilen([&](int i) {
+      s.dv64[i] = s.dv64[i] + p[i] * alpha;
+      const V3<double> ri = r[i] - ap[i] * alpha;
+      r[i] = ri;
+      const V3<double> zi = m3_mal(sd.pinv + 9 * (size_t)i, ri);
+      z[i] = zi;
+      neu[0] += dot(ri, zi);
+    });
+    if (!grid_allreduce<1>(neu, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+    beta = neu[0] / delta;
+    delta = neu[0];
+    ++it;
+  }
+  if (rang == 0 && threadIdx.x == 0) {
+    gs->zaehler = epoch;
+    gs->uebergabe = 0u;
+    gs->iterationen = it;
+    gs->delta = delta;
+    gs->cg_iterationen += it - it0;
+    loesung_zaehlen(s, gs, true);
+  }
+}
+
 template <typename T, bool ZR>
 __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Slot* slots, int durchgang, int rang, int nb,
                                            unsigned char* lds_roh, double* sh) {
@@ -952,6 +1249,8 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
   if (durchgang == 1) {
     if (__hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   }
+  // a refined FP32 solve that k_verf has finished takes no further round
+  if (sizeof(T) == 4 && s.schatten && __hip_atomic_load(&gs->fertig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   V3<T>* slot = (V3<T>*)lds_roh;
   V3<T>* ul = slot + s.kachel_slots_max;
   V3<T>* fterm = ul + s.kachel_v_max;
@@ -1001,39 +1300,55 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
   auto zeilen = [&](int t, int* lo, int* hi) { *lo = gm(s.kachel_zeilen)[2 * t]; *hi = gm(s.kachel_zeilen)[2 * t + 1]; };
   const T th2 = T(s.h2);
 
-  // start: held along the normal get the target normal velocity (implizit.rs:375-384)
-  for (int t = rang; t < s.n_kacheln; t += nb) {
-    int lo, hi;
-    zeilen(t, &lo, &hi);
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const uint8_t fl = gm(s.fflag)[i];
-      if (fl == 2) {
-        sg(gm(s.dv) + (i), zero3<T>());
-      } else if (fl != 0) {  // 1; 3 only in the adjoint's solve (see k_freigabe)
-        const V3<double> n = lg(gm(s.fnorm64) + (i));
-        const double soll = s.soll_direkt ? gm(s.soll_direkt)[i] : gm(s.vn_soll)[i] - dot(lg(gm(s.v) + (i)), n);
-        const V3<T> dvf = filter_fl(fl, lg(gm(s.fnorm) + (i)), lg(gm(s.dv) + (i)));
-        sg(gm(s.dv) + (i), dvf + lg(gm(s.fnorm) + (i)) * T(soll));
-      }
-    }
-  }
-  {
-    double nix[1] = {0.0};
-    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
-  }
-  // FP32 true-residual restarts. The FP32 product A p is exact only to a few
-  // ulps, so with a stiff system the recursively updated residual drifts from
-  // b - A dv: it stalls near what FP32 can reach and can then grow without
-  // bound. When delta stops improving (400 iterations without a new best) or
-  // grows 1e4x above its best, the solve starts again from the current dv
-  // with the true residual (residual replacement), at most 3 times; after
-  // that it ends as not converged and is counted at its rounding floor. The
-  // FP64 build runs the reference's loop unchanged (no restart).
+  // FP32 bulk with an FP64 copy of the system (verf): the launch is one round
+  // of FP32 iterations on the correction dv. k_verf, launched before and
+  // after every round, holds the FP64 solution dv64, takes the FP64
+  // residual, keeps or drops the round's correction and decides whether the
+  // solve has converged (SysDev::schatten); this kernel keeps no FP64 code.
+  const bool verf = sizeof(T) == 4 && s.schatten != nullptr;
   double bb = 0.0, delta = 0.0, ziel = 0.0;
   int it = 0;
   bool grenze = false;
   int neustarts = 0;
+  if (verf) {
+    bb = gs->bb;
+    ziel = gs->ziel;
+    delta = gs->delta;
+    it = gs->iterationen;
+    epoch = __hip_atomic_load(&gs->zaehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    // start: held along the normal get the target normal velocity (implizit.rs:375-384)
+    for (int t = rang; t < s.n_kacheln; t += nb) {
+      int lo, hi;
+      zeilen(t, &lo, &hi);
+      for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint8_t fl = gm(s.fflag)[i];
+        if (fl == 2) {
+          sg(gm(s.dv) + (i), zero3<T>());
+        } else if (fl != 0) {  // 1; 3 only in the adjoint's solve (see k_freigabe)
+          const V3<double> n = lg(gm(s.fnorm64) + (i));
+          const double soll = s.soll_direkt ? gm(s.soll_direkt)[i] : gm(s.vn_soll)[i] - dot(lg(gm(s.v) + (i)), n);
+          const V3<T> dvf = filter_fl(fl, lg(gm(s.fnorm) + (i)), lg(gm(s.dv) + (i)));
+          sg(gm(s.dv) + (i), dvf + lg(gm(s.fnorm) + (i)) * T(soll));
+        }
+      }
+    }
+    double nix[1] = {0.0};
+    if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+  }
+  const int it_anfang = it;
+  // FP32 restarts without the FP64 copy (the adjoint's FP32 solve). The FP32
+  // product A p is exact only to a few ulps, so with a stiff system the
+  // recursively updated residual drifts from b - A dv: it stalls near what
+  // FP32 can reach and can then grow without bound. When delta stops
+  // improving (400 iterations without a new best) or grows 1e4x above its
+  // best, the solve starts again from the current dv with the true FP32
+  // residual (residual replacement), at most 3 times; after that it ends as
+  // not converged and is counted at its rounding floor. With the FP64 copy
+  // such a round ends instead and k_verf restarts from the FP64 residual.
+  // The FP64 build runs the reference's loop unchanged.
   for (bool erster_start = true;; erster_start = false) {
+    if (!verf) {
     // r = S(b - A dv), bb = sum S(b) P S(b), z = P r, delta = r . S(z) (implizit.rs:385-391)
     double part[2] = {0.0, 0.0};
     for (int t = rang; t < s.n_kacheln; t += nb) {
@@ -1068,6 +1383,7 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     if (erster_start) {
       bb = part[0];
       ziel = gmax(s.cg_tol2 * gmax(bb, delta), s.boden);
+    }
     }
     double beta = 0.0;
     int cur = 0;
@@ -1167,12 +1483,22 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         }
       }
     }
+    if (verf) break;  // the round ends; k_verf checks it in FP64
     if (!neu_starten) break;
     if (neustarts == 3) {
       grenze = true;
       break;
     }
     ++neustarts;
+  }
+  if (verf) {  // the round's outcome for k_verf
+    if (rang == 0 && threadIdx.x == 0) {
+      gs->zaehler = epoch;
+      gs->iterationen = it;
+      gs->delta = delta;
+      gs->cg_iterationen += it - it_anfang;
+    }
+    return;
   }
   if (rang == 0 && threadIdx.x == 0) {
     gs->iterationen = it;
@@ -1197,6 +1523,22 @@ __global__ void __launch_bounds__(1024) k_pcg(const SysDev<T>* __restrict__ sp, 
   pcg_loesen<T, true>(*sp, gs, slots, durchgang, (int)blockIdx.x, (int)gridDim.x, lds_roh, sh);
 }
 
+// The FP64 steps of a refined FP32 solve (verf_schritt), with the PCG's
+// launch shape and no dynamic LDS.
+__global__ void __launch_bounds__(1024) k_verf(const SysDev<float>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang,
+                                               int modus, int letzte) {
+  __shared__ double sh[16 * 2 + 2];
+  verf_schritt(*sp, gs, slots, durchgang, modus, letzte, (int)blockIdx.x, (int)gridDim.x, sh);
+}
+
+// The FP64 iterations of a handed-over FP32 solve and the statistics of
+// every refined one (pcg64_loesen); the last launch of the sequence.
+__global__ void __launch_bounds__(1024) k_pcg64(const SysDev<float>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
+  __shared__ double sh[16 * 2 + 2];
+  if (!sp->schatten) return;
+  pcg64_loesen(*sp, gs, slots, durchgang, (int)blockIdx.x, (int)gridDim.x, sh);
+}
+
 // Batched solves: block b works for system tab[b].sys as rank tab[b].rang of
 // its tab[b].nb blocks; every system runs its own iterations, reductions and
 // stopping test, all in one persistent launch (all blocks co-resident).
@@ -1216,22 +1558,49 @@ __global__ void __launch_bounds__(1024) k_pcg_stapel(const StapelBlock<T>* __res
   pcg_loesen<T, false>(*e.sys, e.gs, e.slots, durchgang, e.rang, e.nb, lds_roh, sh);
 }
 
+// k_verf for a batch: each system's blocks take its FP64 step.
+__global__ void __launch_bounds__(1024) k_verf_stapel(const StapelBlock<float>* __restrict__ tab, int durchgang, int modus,
+                                                      int letzte) {
+  __shared__ double sh[16 * 2 + 2];
+  const StapelBlock<float> e = tab[blockIdx.x];
+  if (!e.sys->schatten) return;
+  verf_schritt(*e.sys, e.gs, e.slots, durchgang, modus, letzte, e.rang, e.nb, sh);
+}
+
+// k_pcg64 for a batch: each system's blocks finish its handed-over solve.
+__global__ void __launch_bounds__(1024) k_pcg64_stapel(const StapelBlock<float>* __restrict__ tab, int durchgang) {
+  __shared__ double sh[16 * 2 + 2];
+  const StapelBlock<float> e = tab[blockIdx.x];
+  if (!e.sys->schatten) return;
+  pcg64_loesen(*e.sys, e.gs, e.slots, durchgang, e.rang, e.nb, sh);
+}
+
 // Release pass (implizit.rs:916-942): normal force from the residual
 // A*dv - b; a held point that pulls is released.
 template <typename T>
 __global__ void k_freigabe(const SysDev<T>* __restrict__ sp, GridState* gs, Slot* slots, int durchgang) {
   const SysDev<T>& s = *sp;
-  if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
-  if (__hip_atomic_load(&gs->n_fest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
-  if (blockIdx.x == 0) {  // the next PCG launch starts with clear barrier counters
+  // the second PCG launch starts with clear barrier counters. Block 0 clears
+  // them before any early return: the held count it reads below can already
+  // be 0 because other blocks released every held point, and the second solve
+  // still runs then (frei[0] > 0)
+  if (durchgang == 0 && blockIdx.x == 0) {
     if (threadIdx.x == 0) gs->zaehler = 0;
     barriere_leeren(slots);
   }
+  if (durchgang == 1 && __hip_atomic_load(&gs->frei[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  if (__hip_atomic_load(&gs->n_fest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
   unsigned frei = 0, weg = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     if (s.fflag[i] != 1) continue;
-    const V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return s.dv[j]; });
-    const V3<double> rest = cast3<double>(ap) - cast3<double>(s.b[i]);
+    V3<double> rest;
+    if (s.schatten) {  // FP32 bulk: the residual of the FP64 system at the FP64 solution
+      const SysDev<double>& sd = *(const SysDev<double>*)s.schatten;
+      rest = mal_a_zeile(sd, (uint32_t)i, [&](uint32_t j) { return s.dv64[j]; }) - sd.b[i];
+    } else {
+      const V3<T> ap = mal_a_zeile(s, (uint32_t)i, [&](uint32_t j) { return s.dv[j]; });
+      rest = cast3<double>(ap) - cast3<double>(s.b[i]);
+    }
     const double fn = dot(rest, s.fnorm64[i]) / s.h;
     if (fn < 0.0) {
       // released; after the second solve (no third follows) the point keeps
@@ -1263,7 +1632,7 @@ __global__ void k_update(const SysDev<T>* __restrict__ sp, GridState* gs) {
   const SysDev<T>& s = *sp;
   unsigned kont = 0, haft = 0, stoss = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
-    const V3<double> dvi = cast3<double>(s.dv[i]);
+    const V3<double> dvi = dv_loesung(s, (uint32_t)i);
     s.dv_vorher[i] = dvi;
     if (s.w[i] <= 0.0) continue;
     V3<double> v = s.v[i] + dvi;

@@ -29,7 +29,11 @@
 #define SK_STOSS_KANTE 1e-4 // :1094
 // Cells one entry or query may cover. A box over more cells only comes from a
 // collapsed state; the step then reports an error instead of searching on.
-#define SK_MAX_ZELLEN 4096
+// An entry of the hash whose box covers more cells than this has its buckets
+// found by one block (k_sk_gross) instead of by one thread, whose pairwise
+// dedup of the cells' buckets costs the square of their count. The reference
+// accepts boxes of up to 64 cells per axis (sk_bereich_ok).
+#define SK_KLEIN_ZELLEN 512
 
 struct Quader {
   double lo[3], hi[3];
@@ -110,6 +114,9 @@ struct SelbstDev {
   uint32_t* eintraege;              // kapazitaet_eintraege, bucket contents in entry order
   uint32_t* eintraege_roh;          // the same after the atomic fill, unordered
   uint32_t kapazitaet_eintraege;
+  uint32_t* gross;                  // entries of the current hash with more than SK_KLEIN_ZELLEN cells
+  uint32_t kapazitaet_gross;
+  uint32_t* eimer_bits;             // eimer_max bits: the buckets one such entry has reached
   uint32_t* zahl;                   // max(n, ne, nt) + 1: per-query counts, scanned
   // lists
   Naehe* naehe;
@@ -147,7 +154,7 @@ struct SelbstDev {
 
 enum {
   SK_Z_NAEHE_PD = 0, SK_Z_NAEHE_KK, SK_Z_NAEHE, SK_Z_SCHNITTE, SK_Z_GEBIETE, SK_Z_FEDERN, SK_Z_STOESSE_PD, SK_Z_STOESSE,
-  SK_Z_STOSS_ZAHL, SK_Z_UEBERLAUF, SK_Z_EINTRAEGE, SK_Z_FEDER_AUS, SK_Z_ANZAHL
+  SK_Z_STOSS_ZAHL, SK_Z_UEBERLAUF, SK_Z_EINTRAEGE, SK_Z_FEDER_AUS, SK_Z_GROSS, SK_Z_ANZAHL
 };
 
 // ---------------------------------------------------------------------------
@@ -322,15 +329,13 @@ __device__ __forceinline__ bool sk_partner(const SelbstDev& s, uint32_t a, uint3
 // ---------------------------------------------------------------------------
 // hash (:256-344): bucket count, per-entry dedup of buckets, sorted buckets
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool sk_zellen(const SelbstDev& s, const Quader& q, double zelle, int64_t* lo, int64_t* hi) {
+// The cell range of a box the hash accepts (at most 65 cells per axis, so
+// at most 65^3 cells: the counts fit an int).
+__device__ __forceinline__ bool sk_zellen(const Quader& q, double zelle, int64_t* lo, int64_t* hi) {
   if (!sk_bereich_ok(q, zelle)) return false;
   for (int a = 0; a < 3; ++a) {
     lo[a] = sk_zelle(q.lo[a], zelle);
     hi[a] = sk_zelle(q.hi[a], zelle);
-  }
-  if ((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) > SK_MAX_ZELLEN) {
-    atomicAdd(&s.zaehler[SK_Z_UEBERLAUF], 1u);
-    return false;
   }
   return true;
 }
@@ -338,12 +343,23 @@ __device__ __forceinline__ bool sk_zellen(const SelbstDev& s, const Quader& q, d
 // Calls f(bucket) once per distinct bucket of the entry's cells (the reference
 // sorts and dedups the bucket list; the set is the same, its order does not
 // matter because each bucket only receives the entry index).
+// An entry with more than SK_KLEIN_ZELLEN cells is left to k_sk_gross; the
+// counting pass (zaehlen) puts it on the list of such entries.
 template <typename F>
-__device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, F f) {
+__device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, uint32_t e,
+                                            bool zaehlen, F f) {
   int64_t lo[3], hi[3];
-  if (!sk_zellen(s, q, zelle, lo, hi)) return;
-  // at most SK_MAX_ZELLEN cells: the offsets are ints, stepped k fastest
+  if (!sk_zellen(q, zelle, lo, hi)) return;
   const int nc = (int)((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
+  if (nc > SK_KLEIN_ZELLEN) {
+    if (zaehlen) {
+      const uint32_t pos = atomicAdd(&s.zaehler[SK_Z_GROSS], 1u);
+      if (pos < s.kapazitaet_gross) s.gross[pos] = e;
+      else atomicAdd(&s.zaehler[SK_Z_UEBERLAUF], 1u);
+    }
+    return;
+  }
+  // the offsets are ints, stepped k fastest
   const int ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
   int i = 0, j = 0, k = 0;
   for (int c = 0; c < nc; ++c) {
@@ -369,7 +385,7 @@ __global__ void k_sk_eimer_zaehlen(const SelbstDev* __restrict__ sp, int anzahl,
   const SelbstDev& s = *sp;
   const double zelle = *zelle_p;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < anzahl; e += gridDim.x * blockDim.x)
-    sk_je_eimer(s, s.quader[e], zelle, eimer, [&](uint32_t h) { atomicAdd(&s.eimer_fuell[h], 1u); });
+    sk_je_eimer(s, s.quader[e], zelle, eimer, (uint32_t)e, true, [&](uint32_t h) { atomicAdd(&s.eimer_fuell[h], 1u); });
 }
 
 __global__ void k_sk_eimer_fuellen(const SelbstDev* __restrict__ sp, int anzahl, const double* zelle_p, uint32_t eimer) {
@@ -377,10 +393,43 @@ __global__ void k_sk_eimer_fuellen(const SelbstDev* __restrict__ sp, int anzahl,
   const double zelle = *zelle_p;
   if (s.zaehler[SK_Z_EINTRAEGE] > s.kapazitaet_eintraege) return;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < anzahl; e += gridDim.x * blockDim.x)
-    sk_je_eimer(s, s.quader[e], zelle, eimer, [&](uint32_t h) {
+    sk_je_eimer(s, s.quader[e], zelle, eimer, (uint32_t)e, false, [&](uint32_t h) {
       const uint32_t pos = atomicAdd(&s.eimer_fuell[h], 1u);
       s.eintraege_roh[pos] = (uint32_t)e;
     });
+}
+
+// The entries with more than SK_KLEIN_ZELLEN cells, one after another, by one
+// block: the bucket of every cell sets its bit in a bitmap over the buckets,
+// and the cell that sets it first counts the entry in that bucket (fuellen =
+// 0) or puts it there (fuellen = 1). Each bucket receives the entry once, as
+// with the reference's sorted and deduplicated bucket list; the order inside
+// a bucket is fixed afterwards by k_sk_eimer_sortieren.
+__global__ void __launch_bounds__(1024) k_sk_gross(const SelbstDev* __restrict__ sp, const double* zelle_p, uint32_t eimer,
+                                                   int fuellen) {
+  const SelbstDev& s = *sp;
+  const double zelle = *zelle_p;
+  if (fuellen && s.zaehler[SK_Z_EINTRAEGE] > s.kapazitaet_eintraege) return;
+  const uint32_t ng = min(s.zaehler[SK_Z_GROSS], s.kapazitaet_gross);
+  const uint32_t woerter = (eimer + 31u) / 32u;
+  for (uint32_t g = 0; g < ng; ++g) {
+    const uint32_t e = s.gross[g];
+    int64_t lo[3], hi[3];
+    (void)sk_zellen(s.quader[e], zelle, lo, hi);  // accepted by the counting pass
+    const int ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
+    const int nc = (int)(hi[0] - lo[0] + 1) * ny * nz;
+    for (uint32_t w = threadIdx.x; w < woerter; w += blockDim.x)
+      __hip_atomic_store(&s.eimer_bits[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const uint32_t h = sk_streue(lo[0] + c / (ny * nz), lo[1] + (c / nz) % ny, lo[2] + c % nz, eimer);
+      const uint32_t bit = 1u << (h & 31u);
+      if (atomicOr(&s.eimer_bits[h >> 5], bit) & bit) continue;
+      const uint32_t pos = atomicAdd(&s.eimer_fuell[h], 1u);
+      if (fuellen) s.eintraege_roh[pos] = e;
+    }
+    __syncthreads();
+  }
 }
 
 // Entries of a bucket in index order, as the reference's sequential fill
@@ -434,10 +483,6 @@ __device__ __forceinline__ void sk_welle_je_eintrag(const SelbstDev& s, const Qu
   // each extent is at most 65 cells (sk_bereich_ok), so the counts fit an int
   const int nx = (int)(hi[0] - lo[0] + 1), ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
   const int nc = nx * ny * nz;
-  if (nc > SK_MAX_ZELLEN) {
-    if (lane == 0) atomicAdd(&s.zaehler[SK_Z_UEBERLAUF], 1u);
-    return;
-  }
   const int nyz = ny * nz;
   for (int c0 = 0; c0 < nc; c0 += 64) {
     const int c = c0 + lane;

@@ -60,6 +60,12 @@ struct gc_ctx {
   // pointer) and the bytes of its last upload
   void* sys_dev = nullptr;
   std::vector<unsigned char> sys_alt;
+  // FP32 bulk: the FP64 copy of the system that the PCG's refinement and the
+  // release pass take their residual from (its records, right-hand side and
+  // mass diagonal are its own; topology, state and forces are shared)
+  SysDev<double> schatten{};
+  void* schatten_dev = nullptr;
+  std::vector<unsigned char> schatten_alt;
   GridState* gs = nullptr;
   Slot* slots = nullptr;
   bool topo = false, hat_param = false, built = false;
@@ -85,6 +91,11 @@ struct gc_ctx {
   std::vector<double> sk_naht_ruhe;
   bool sk_bereit = false;
   double sk_dicke = -1.0;
+  // capacity factor of the self-collision lists: a step whose lists overflow
+  // is taken again from its start with the capacities doubled (the reference
+  // keeps them in growable vectors)
+  int sk_mal = 1;
+  std::vector<void*> sk_allocs;              // buffers of the self-collision tables (freed when they grow)
   SelbstDev sk{};
   SelbstDev* sk_dev = nullptr;
   uint32_t sk_eimer_p = 0, sk_eimer_k = 0, sk_eimer_d = 0, sk_eimer_max = 0;
@@ -100,6 +111,10 @@ struct gc_ctx {
   void* sk_fe_satz = nullptr;                // per spring entry: its points, weights and block (bulk type)
   // graph of one substep
   hipGraphExec_t graph = nullptr;
+  // generation of the launch sequence: grows whenever a setting that the
+  // host turns into launches changes (self-collision on or off, impact
+  // iterations, a body moved), so that captured graphs are captured anew
+  uint64_t folge = 0;
   // adjoint of the step (allocated on first use): FP64 buffers, the adjoint
   // right-hand side and solution in the bulk type, a copy of the system
   // description whose b, dv and normal targets are the adjoint's, and a grid
@@ -117,9 +132,14 @@ struct gc_ctx {
     V3<double>* x = nullptr;
     V3<double>* v = nullptr;
     V3<double>* dvv = nullptr;
+    uint32_t* folge = nullptr;   // self-collision push-spring sequence per triangle (selbstkollision.rs:1024-1026)
     bool hat_vorher = false;
   };
   std::map<int, Zustand> zustaende;
+  // the state at the start of the current gc_step / gc_step_graph call, with
+  // the step counters, for taking the call again after a list overflow
+  Zustand anfang;
+  GridState anfang_gs{};
 };
 
 #define HIPCK(ctx, call)                                                                      \
@@ -157,6 +177,16 @@ static int hoch(gc_ctx* c, X* dst, const X* src, size_t count) {
   return 0;
 }
 
+// A setting that the launch sequence depends on changed: the context's graph
+// is dropped, and a batch's graph is captured again at its next replay.
+static void folge_neu(gc_ctx* c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  ++c->folge;
+}
+
 static void frei_alles(gc_ctx* c) {
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
@@ -164,6 +194,8 @@ static void frei_alles(gc_ctx* c) {
   }
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  for (void* p : c->sk_allocs) (void)hipFree(p);
+  c->sk_allocs.clear();
   for (auto& k : c->koerper)
     if (k.werte_dev) (void)hipFree(k.werte_dev);
 }
@@ -180,8 +212,46 @@ static const SysDev<T>* dsys(gc_ctx* c) { return (const SysDev<T>*)c->sys_dev; }
 // inside a graph capture: it waits for the stream first, so no kernel still
 // reads the old copy. A captured graph reads the description at replay time,
 // so parameter and body updates reach it without a new capture.
+// The FP64 copy of an FP32 system: the scalars and the shared arrays of the
+// bulk system, its own FP64 records (allocated with the topology), uploaded
+// when it changed.
+static int schatten_hoch(gc_ctx* c) {
+  if (!c->schatten_dev) return 0;
+  const SysDev<float>& f = c->sf;
+  SysDev<double>& d = c->schatten;
+  d.n = f.n; d.nk = f.nk; d.nb = f.nb; d.nm = f.nm;
+  d.h = f.h; d.h2 = f.h2; d.c = f.c; d.naht_k = f.naht_k; d.kontakt_rand = f.kontakt_rand;
+  for (int q = 0; q < 3; ++q) d.g[q] = f.g[q];
+  d.cg_max = f.cg_max; d.cg_tol2 = f.cg_tol2; d.boden = f.boden;
+  d.aus_last = f.aus_last; d.durchstoss_pruefen = f.durchstoss_pruefen;
+  d.off = f.off; d.csr = f.csr;
+  d.k_ij = f.k_ij; d.k_k = f.k_k; d.k_ruhe = f.k_ruhe; d.k_naht = f.k_naht; d.k_f = f.k_f; d.k_zug = f.k_zug;
+  d.b_p = f.b_p; d.b_k = f.b_k; d.b_ruhe = f.b_ruhe; d.b_kf = f.b_kf;
+  d.m_p = f.m_p; d.m_el = f.m_el64; d.m_el64 = f.m_el64; d.m_f = f.m_f; d.m_z = f.m_z;
+  d.masse = f.masse; d.w = f.w; d.koll = f.koll; d.reib = f.reib;
+  d.x = f.x; d.v = f.v; d.f = f.f; d.last = f.last;
+  d.fflag = f.fflag; d.fnorm = f.fnorm64; d.fnorm64 = f.fnorm64;
+  d.vn_soll = f.vn_soll; d.vk = f.vk; d.fn_kraft = f.fn_kraft;
+  d.dv = f.dv64; d.dv_vorher = f.dv_vorher;
+  d.selbst = f.selbst;
+  d.fe_off = f.fe_off; d.fe_ent = f.fe_ent; d.fk_off = f.fk_off; d.fk_ent = f.fk_ent;
+  d.fe_punkte = f.fe_punkte; d.fe_gew = f.fe_gew; d.fe_kraft = f.fe_kraft;
+  d.fe_block = f.selbst ? c->sk.fe_block : nullptr;  // the FP64 blocks of the springs
+  d.n_koerper = f.n_koerper;
+  for (int b = 0; b < GC_MAX_KOERPER; ++b) d.koerper[b] = f.koerper[b];
+  const unsigned char* roh = (const unsigned char*)&d;
+  if (c->schatten_alt.size() == sizeof(SysDev<double>) && std::memcmp(c->schatten_alt.data(), roh, sizeof(SysDev<double>)) == 0)
+    return 0;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  HIPCK(c, hipMemcpyAsync(c->schatten_dev, roh, sizeof(SysDev<double>), hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->schatten_alt.assign(roh, roh + sizeof(SysDev<double>));
+  return 0;
+}
+
 template <typename T>
 static int sys_hoch(gc_ctx* c) {
+  if (sizeof(T) == 4 && schatten_hoch(c)) return 1;
   const SysDev<T>& s = sys<T>(c);
   const unsigned char* roh = (const unsigned char*)&s;
   if (c->sys_alt.size() == sizeof(SysDev<T>) && std::memcmp(c->sys_alt.data(), roh, sizeof(SysDev<T>)) == 0) return 0;
@@ -277,6 +347,26 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
       !s.x || !s.v || !s.mdiag || !s.f || !s.last || !s.b || !s.pinv || !s.fflag || !s.fnorm || !s.fnorm64 || !s.vn_soll ||
       !s.vk || !s.fn_kraft || !s.dv || !s.r || !s.z || !s.p[0] || !s.p[1] || !s.ap || !s.dv_vorher)
     return fehler(c, "hipMalloc failed (vectors)");
+  if (sizeof(T) == 4) {
+    // the FP64 copy's own arrays: operator records, right-hand side, mass
+    // diagonal, its (unused) preconditioner, and the FP64 solution
+    SysDev<double>& d = c->schatten;
+    std::memset((void*)&d, 0, sizeof(SysDev<double>));
+    c->schatten_alt.clear();
+    d.k_n = dalloc<V3<double>>(c, nk); d.k_q = dalloc<double>(c, nk); d.k_kk = dalloc<double>(c, nk);
+    d.b_kt = dalloc<double>(c, nb); d.b_grad = dalloc<V3<double>>(c, 4 * (size_t)nb);
+    d.m_h = dalloc<Hessen<double>>(c, nm);
+    d.mdiag = dalloc<double>(c, n); d.b = dalloc<V3<double>>(c, n); d.pinv = dalloc<double>(c, 9 * (size_t)n);
+    // the FP64 iterations a stalled FP32 solve hands over to
+    d.r = dalloc<V3<double>>(c, n); d.z = dalloc<V3<double>>(c, n); d.p[0] = dalloc<V3<double>>(c, n);
+    d.ap = dalloc<V3<double>>(c, n);
+    s.dv64 = dalloc<V3<double>>(c, n);
+    c->schatten_dev = dalloc<SysDev<double>>(c, 1);
+    if (!d.k_n || !d.k_q || !d.k_kk || !d.b_kt || !d.b_grad || !d.m_h || !d.mdiag || !d.b || !d.pinv || !d.r || !d.z ||
+        !d.p[0] || !d.ap || !s.dv64 || !c->schatten_dev)
+      return fehler(c, "hipMalloc failed (FP64 copy of the system)");
+    s.schatten = c->schatten_dev;
+  }
   // Tiles: consecutive rows whose CSR entries plus the points their
   // constraints touch fit the LDS budget. Per tile: the constraints touching
   // its rows, each with the tile-local index of its points and the LDS slot of
@@ -418,7 +508,16 @@ static int topologie_laden(gc_ctx* c, const std::vector<uint32_t>& off, const st
   c->pcg_lds_kachel = std::max<size_t>(16, groesste * sizeof(V3<T>));
   // PCG launch shape: co-resident blocks only (the kernel waits on all of them)
   c->pcg_lds = std::max<size_t>(16, lds_tile + fe_kap * sizeof(V3<T>));
-  (void)hipFuncSetAttribute((const void*)k_pcg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->pcg_lds);
+  {
+    // the kernel's static LDS plus the tile's dynamic LDS must fit one CU;
+    // the occupancy query below then confirms one block per CU
+    hipFuncAttributes fa{};
+    HIPCK(c, hipFuncGetAttributes(&fa, (const void*)k_pcg<T>));
+    if (fa.sharedSizeBytes + c->pcg_lds > c->lds_cu)
+      return fehler(c, "PCG tile LDS (" + std::to_string(c->pcg_lds) + " B) plus the kernel's static LDS (" +
+                           std::to_string(fa.sharedSizeBytes) + " B) exceed the CU's LDS");
+  }
+  HIPCK(c, hipFuncSetAttribute((const void*)k_pcg<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->pcg_lds));
   int occ = 0;
   HIPCK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_pcg<T>, c->pcg_threads, c->pcg_lds));
   if (occ < 1) return fehler(c, "PCG kernel does not fit on a CU with this LDS budget");
@@ -589,14 +688,19 @@ static int selbst_vorbereiten(gc_ctx* c) {
   c->sk_eimer_k = (uint32_t)std::max<size_t>(2 * (size_t)ne, 64);
   c->sk_eimer_d = (uint32_t)std::max<size_t>(2 * (size_t)nt, 64);
   c->sk_eimer_max = std::max(c->sk_eimer_p, std::max(c->sk_eimer_k, c->sk_eimer_d));
-  k.kapazitaet_eintraege = (uint32_t)(32 * mx);
-  k.kapazitaet_naehe = (uint32_t)(4 * ((size_t)n + ne));
-  k.kapazitaet_schnitte = (uint32_t)(2 * (size_t)ne + 1024);
+  const size_t mal = (size_t)c->sk_mal;
+  if (mal * (32 * mx + 12 * (4 * ((size_t)n + ne) + 2 * (size_t)ne + 1024)) >= (size_t)UINT32_MAX)
+    return fehler(c, "self-collision lists would exceed 32-bit indexing");
+  k.kapazitaet_eintraege = (uint32_t)(mal * 32 * mx);
+  k.kapazitaet_naehe = (uint32_t)(mal * 4 * ((size_t)n + ne));
+  k.kapazitaet_schnitte = (uint32_t)(mal * (2 * (size_t)ne + 1024));
   k.kapazitaet_tausch = 6 * k.kapazitaet_schnitte;
   k.kapazitaet_federn = k.kapazitaet_naehe + k.kapazitaet_schnitte;
   k.kapazitaet_fe_ent = 12 * k.kapazitaet_federn;
   k.kapazitaet_fk_ent = 5 * k.kapazitaet_federn;
-  k.kapazitaet_st_ent = 4 * (uint32_t)(nt + ne);
+  k.kapazitaet_st_ent = (uint32_t)(mal * 4 * ((size_t)nt + ne));
+  // every buffer allocated below belongs to the tables and is freed when they grow
+  const size_t a0 = c->allocs.size();
   bool ok = true;
   auto chk = [&](const void* q) { ok = ok && q != nullptr; };
   k.dreiecke = hoch_neu(c, c->sk_dreiecke_in); chk(k.dreiecke);
@@ -634,6 +738,9 @@ static int selbst_vorbereiten(gc_ctx* c) {
   k.eimer_fuell = dalloc<uint32_t>(c, c->sk_eimer_max + 1); chk(k.eimer_fuell);
   k.eintraege = dalloc<uint32_t>(c, k.kapazitaet_eintraege); chk(k.eintraege);
   k.eintraege_roh = dalloc<uint32_t>(c, k.kapazitaet_eintraege); chk(k.eintraege_roh);
+  k.kapazitaet_gross = (uint32_t)(1024 * mal);
+  k.gross = dalloc<uint32_t>(c, k.kapazitaet_gross); chk(k.gross);
+  k.eimer_bits = dalloc<uint32_t>(c, c->sk_eimer_max / 32 + 1); chk(k.eimer_bits);
   k.zahl = dalloc<uint32_t>(c, (size_t)n + ne + nt + 1); chk(k.zahl);
   k.naehe = dalloc<Naehe>(c, k.kapazitaet_naehe); chk(k.naehe);
   k.schnitte = dalloc<Schnitt>(c, k.kapazitaet_schnitte); chk(k.schnitte);
@@ -666,6 +773,8 @@ static int selbst_vorbereiten(gc_ctx* c) {
   c->sk_fe_t = dalloc<V3<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_t);
   c->sk_fe_satz = dalloc<FederK<T>>(c, k.kapazitaet_fe_ent); chk(c->sk_fe_satz);
   c->sk_dev = dalloc<SelbstDev>(c, 1); chk(c->sk_dev);
+  c->sk_allocs.insert(c->sk_allocs.end(), c->allocs.begin() + (long)a0, c->allocs.end());
+  c->allocs.resize(a0);
   if (!ok) return fehler(c, "hipMalloc/hipMemcpy failed (self-collision buffers)");
   HIPCK(c, hipMemcpyAsync(c->sk_dev, &k, sizeof(SelbstDev), hipMemcpyHostToDevice, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
@@ -711,9 +820,12 @@ static void sk_hash(gc_ctx* c, int anzahl, const double* zelle_p, uint32_t eimer
   hipStream_t st = c->stream;
   SelbstDev* sp = c->sk_dev;
   hipLaunchKernelGGL(k_sk_null, dim3(sk_gitter(eimer + 1)), dim3(BUILD_THREADS), 0, st, c->sk.eimer_fuell, (int)eimer + 1);
+  hipLaunchKernelGGL(k_sk_null, dim3(1), dim3(64), 0, st, c->sk.zaehler + SK_Z_GROSS, 1);
   hipLaunchKernelGGL(k_sk_eimer_zaehlen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
+  hipLaunchKernelGGL(k_sk_gross, dim3(1), dim3(1024), 0, st, sp, zelle_p, eimer, 0);
   sk_scan(c, c->sk.eimer_fuell, (int)eimer, c->sk.eimer_anfang, c->sk.zaehler + SK_Z_EINTRAEGE);
   hipLaunchKernelGGL(k_sk_eimer_fuellen, dim3(sk_gitter(anzahl)), dim3(BUILD_THREADS), 0, st, sp, anzahl, zelle_p, eimer);
+  hipLaunchKernelGGL(k_sk_gross, dim3(1), dim3(1024), 0, st, sp, zelle_p, eimer, 1);
   hipLaunchKernelGGL(k_sk_eimer_sortieren, dim3(sk_gitter(c->sk.kapazitaet_eintraege)), dim3(BUILD_THREADS), 0, st, sp, eimer);
 }
 
@@ -820,6 +932,16 @@ static int launch_bau(gc_ctx* c) {
   hipStream_t st = c->stream;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, st, c->gs, c->slots);
   if (s.selbst && launch_selbst_bau<T>(c)) return 1;
+  // FP32 bulk: the FP64 copy's records, right-hand side and mass diagonal
+  // first (the shared FP64 outputs, forces and loads, are the same in both
+  // builds, and the bulk build below writes them last)
+  const SysDev<double>* sdp = (const SysDev<double>*)c->schatten_dev;
+  if (sizeof(T) == 4 && sdp) {
+    if (s.nk) hipLaunchKernelGGL(k_bau_kanten<double>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sdp);
+    if (s.nb) hipLaunchKernelGGL(k_bau_biegung<double>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, sdp);
+    if (s.nm) hipLaunchKernelGGL(k_bau_membran<double>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, sdp);
+    hipLaunchKernelGGL(k_sammeln<double>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, st, sdp);
+  }
   if (s.nk) hipLaunchKernelGGL(k_bau_kanten<T>, dim3(gitter(s.nk)), dim3(BUILD_THREADS), 0, st, sp);
   if (s.nb) hipLaunchKernelGGL(k_bau_biegung<T>, dim3(gitter(s.nb)), dim3(BUILD_THREADS), 0, st, sp);
   if (s.nm) hipLaunchKernelGGL(k_bau_membran<T>, dim3(gitter(s.nm)), dim3(BUILD_THREADS), 0, st, sp);
@@ -830,10 +952,33 @@ static int launch_bau(gc_ctx* c) {
   return 0;
 }
 
+// Rounds of FP32 iterations of a refined solve: its FP64 start, then up to
+// GC_RUNDEN rounds each checked in FP64, then the FP64 iterations of a solve
+// handed over and the solve's statistics. A launch after the solve is done
+// returns at once, so the sequence is fixed (and capturable). The launches
+// share one grid and carry the barrier epoch from one to the next
+// (GridState::zaehler), so the counters, cleared before the first, are not
+// cleared between them.
+#define GC_RUNDEN 4
+static void launch_verfeinert(hipStream_t st, int bloecke, int threads, size_t lds, const SysDev<float>* sp, GridState* gs,
+                              Slot* slots, int durchgang) {
+  hipLaunchKernelGGL(k_verf, dim3(bloecke), dim3(threads), 0, st, sp, gs, slots, durchgang, 0, 0);
+  for (int r = 0; r < GC_RUNDEN; ++r) {
+    hipLaunchKernelGGL(k_pcg<float>, dim3(bloecke), dim3(threads), lds, st, sp, gs, slots, durchgang);
+    hipLaunchKernelGGL(k_verf, dim3(bloecke), dim3(threads), 0, st, sp, gs, slots, durchgang, 1, r == GC_RUNDEN - 1 ? 1 : 0);
+  }
+  hipLaunchKernelGGL(k_pcg64, dim3(bloecke), dim3(threads), 0, st, sp, gs, slots, durchgang);
+}
+
 template <typename T>
 static int launch_pcg(gc_ctx* c, int durchgang) {
-  hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, c->stream, dsys<T>(c), c->gs, c->slots,
-                     durchgang);
+  if (sizeof(T) == 4 && c->schatten_dev) {
+    launch_verfeinert(c->stream, c->pcg_blocks, c->pcg_threads, c->pcg_lds, (const SysDev<float>*)dsys<T>(c), c->gs, c->slots,
+                      durchgang);
+  } else {
+    hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, c->stream, dsys<T>(c), c->gs,
+                       c->slots, durchgang);
+  }
   HIPCK(c, hipGetLastError());
   return 0;
 }
@@ -885,14 +1030,105 @@ static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
   return launch_nach_pcg<T>(c);
 }
 
-static int pruefe_gitter(gc_ctx* c) {
+static V3<double>* zustand_x(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.x : c->sd.x; }
+static V3<double>* zustand_v(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.v : c->sd.v; }
+static V3<double>* zustand_dvv(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.dv_vorher : c->sd.dv_vorher; }
+
+// Saves x, v, the warm start and the self-collision push-spring sequence of
+// the context's state into z (device copies, allocated on first use).
+static int zustand_sichern_in(gc_ctx* c, gc_ctx::Zustand& z) {
+  const size_t n = (size_t)c->n;
+  if (!z.x) {
+    z.x = dalloc<V3<double>>(c, n);
+    z.v = dalloc<V3<double>>(c, n);
+    z.dvv = dalloc<V3<double>>(c, n);
+    if (!z.x || !z.v || !z.dvv) return fehler(c, "hipMalloc failed (saved state)");
+  }
+  const size_t bytes = n * sizeof(V3<double>);
+  HIPCK(c, hipMemcpyAsync(z.x, zustand_x(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(z.v, zustand_v(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(z.dvv, zustand_dvv(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (c->sk_bereit) {
+    if (!z.folge) z.folge = dalloc<uint32_t>(c, (size_t)c->sk.nt);
+    if (!z.folge) return fehler(c, "hipMalloc failed (saved state)");
+    HIPCK(c, hipMemcpyAsync(z.folge, c->sk.folge, (size_t)c->sk.nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  }
+  z.hat_vorher = c->hat_vorher;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int zustand_laden_aus(gc_ctx* c, const gc_ctx::Zustand& z) {
+  const size_t bytes = (size_t)c->n * sizeof(V3<double>);
+  HIPCK(c, hipMemcpyAsync(zustand_x(c), z.x, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(zustand_v(c), z.v, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCK(c, hipMemcpyAsync(zustand_dvv(c), z.dvv, bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (c->sk_bereit && z.folge)
+    HIPCK(c, hipMemcpyAsync(c->sk.folge, z.folge, (size_t)c->sk.nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  c->hat_vorher = z.hat_vorher;
+  c->built = false;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// The state and the step counters at the start of a stepping call.
+static int anfang_sichern(gc_ctx* c) {
+  if (zustand_sichern_in(c, c->anfang)) return 1;
+  HIPCK(c, hipMemcpy(&c->anfang_gs, c->gs, sizeof(GridState), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// After a self-collision list overflowed: free the tables so that the next
+// selbst_vorbereiten builds them with twice the capacities, and put the
+// state and the counters back to the start of the call (the push-spring
+// sequence is restored by anfang_zurueck once the tables exist again).
+static int selbst_wachsen(gc_ctx* c) {
+  if (c->sk_mal >= 16)
+    return fehler(c, "a self-collision list exceeded its capacity at 16 times the default sizes (state of the step is "
+                     "not the reference's)");
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  for (void* p : c->sk_allocs) (void)hipFree(p);
+  c->sk_allocs.clear();
+  c->sk_bereit = false;
+  c->sk_dev = nullptr;
+  c->sk_mal *= 2;
+  folge_neu(c);
+  return 0;
+}
+
+static int anfang_zurueck(gc_ctx* c) {
+  if (zustand_laden_aus(c, c->anfang)) return 1;
+  GridState g = c->anfang_gs;
+  g.fehler = 0;
+  g.selbst_ueberlauf = 0;
+  HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// The message of a grid all-reduce's error flag (GridState::fehler).
+static std::string barriere_meldung(unsigned code) {
+  if (code == 2)
+    return "grid all-reduce summed partials of another barrier (stale barrier counters); the solve's result is "
+           "not valid";
+  return "grid barrier wait exceeded its spin limit (blocks not co-resident?)";
+}
+
+// ueberlauf: where given, a self-collision list overflow is reported there
+// (and cleared) instead of as an error.
+static int pruefe_gitter(gc_ctx* c, bool* ueberlauf = nullptr) {
   GridState g;
   HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
   HIPCK(c, hipStreamSynchronize(c->stream));
   if (g.fehler) {
     unsigned null = 0;
     HIPCK(c, hipMemcpy(&c->gs->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
-    return fehler(c, "grid barrier wait exceeded its spin limit (blocks not co-resident?)");
+    return fehler(c, barriere_meldung(g.fehler));
+  }
+  if (g.selbst_ueberlauf && ueberlauf) {
+    unsigned null = 0;
+    HIPCK(c, hipMemcpy(&c->gs->selbst_ueberlauf, &null, sizeof(unsigned), hipMemcpyHostToDevice));
+    *ueberlauf = true;
+    return 0;
   }
   if (g.selbst_ueberlauf) {
     unsigned null = 0;
@@ -916,26 +1152,35 @@ static int schritte(gc_ctx* c, double h, int n) {
   sys<T>(c).h2 = h * h;
   if (selbst_vorbereiten<T>(c)) return 1;
   if (sys_hoch<T>(c)) return 1;
-  HIPCK(c, hipEventRecord(c->ev[0], c->stream));
-  for (int k = 0; k < n; ++k) {
-    if (launch_schritt<T>(c, c->hat_vorher ? 1 : 0, &c->ev[2 + 4 * k])) return 1;
-    c->hat_vorher = true;
+  // with self-collision on, a call whose lists overflow is taken again from
+  // its start with larger lists
+  const bool sichern = sys<T>(c).selbst != 0;
+  if (sichern && anfang_sichern(c)) return 1;
+  for (;;) {
+    HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int k = 0; k < n; ++k) {
+      if (launch_schritt<T>(c, c->hat_vorher ? 1 : 0, &c->ev[2 + 4 * k])) return 1;
+      c->hat_vorher = true;
+    }
+    HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCK(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->ms_schritt = ms;
+    double pcg = 0.0;
+    for (int k = 0; k < n; ++k) {
+      float m0 = 0.f, m1 = 0.f;
+      HIPCK(c, hipEventElapsedTime(&m0, c->ev[2 + 4 * k], c->ev[3 + 4 * k]));
+      HIPCK(c, hipEventElapsedTime(&m1, c->ev[4 + 4 * k], c->ev[5 + 4 * k]));
+      pcg += (double)m0 + (double)m1;
+    }
+    c->ms_pcg = pcg;
+    c->built = false;
+    bool ueberlauf = false;
+    if (pruefe_gitter(c, sichern ? &ueberlauf : nullptr)) return 1;
+    if (!ueberlauf) return 0;
+    if (selbst_wachsen(c) || selbst_vorbereiten<T>(c) || sys_hoch<T>(c) || anfang_zurueck(c)) return 1;
   }
-  HIPCK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCK(c, hipEventSynchronize(c->ev[1]));
-  float ms = 0.f;
-  HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  c->ms_schritt = ms;
-  double pcg = 0.0;
-  for (int k = 0; k < n; ++k) {
-    float m0 = 0.f, m1 = 0.f;
-    HIPCK(c, hipEventElapsedTime(&m0, c->ev[2 + 4 * k], c->ev[3 + 4 * k]));
-    HIPCK(c, hipEventElapsedTime(&m1, c->ev[4 + 4 * k], c->ev[5 + 4 * k]));
-    pcg += (double)m0 + (double)m1;
-  }
-  c->ms_pcg = pcg;
-  c->built = false;
-  return pruefe_gitter(c);
 }
 
 template <typename T>
@@ -952,17 +1197,8 @@ static int schritte_graph(gc_ctx* c, double h, int n) {
   sys<T>(c).h2 = h * h;
   if (selbst_vorbereiten<T>(c)) return 1;
   if (sys_hoch<T>(c)) return 1;
-  if (!c->graph) {
-    hipGraph_t g;
-    HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_schritt<T>(c, 1, nullptr);
-    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (rc) return 1;
-    if (e2 != hipSuccess) return fehler(c, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
-    hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e3 != hipSuccess) return fehler(c, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
-  }
+  const bool sichern = sys<T>(c).selbst != 0;
+  if (sichern && anfang_sichern(c)) return 1;
   if (c->ev.size() < 2) {
     while (c->ev.size() < 2) {
       hipEvent_t e;
@@ -970,16 +1206,34 @@ static int schritte_graph(gc_ctx* c, double h, int n) {
       c->ev.push_back(e);
     }
   }
-  HIPCK(c, hipEventRecord(c->ev[0], c->stream));
-  for (int k = done; k < n; ++k) HIPCK(c, hipGraphLaunch(c->graph, c->stream));
-  HIPCK(c, hipEventRecord(c->ev[1], c->stream));
-  HIPCK(c, hipEventSynchronize(c->ev[1]));
-  float ms = 0.f;
-  HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  c->ms_schritt = ms;
-  c->ms_pcg = std::nan("");
-  c->built = false;
-  return pruefe_gitter(c);
+  for (;;) {
+    if (!c->graph) {
+      hipGraph_t g;
+      HIPCK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = launch_schritt<T>(c, 1, nullptr);
+      hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+      if (rc) return 1;
+      if (e2 != hipSuccess) return fehler(c, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+      hipError_t e3 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e3 != hipSuccess) return fehler(c, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    HIPCK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int k = done; k < n; ++k) HIPCK(c, hipGraphLaunch(c->graph, c->stream));
+    HIPCK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCK(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIPCK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->ms_schritt = ms;
+    c->ms_pcg = std::nan("");
+    c->built = false;
+    bool ueberlauf = false;
+    if (pruefe_gitter(c, sichern ? &ueberlauf : nullptr)) return 1;
+    if (!ueberlauf) return 0;
+    // the tables grow (the graph is dropped with them) and the replays are
+    // taken again from the state after the eager first substep, if any
+    if (selbst_wachsen(c) || selbst_vorbereiten<T>(c) || sys_hoch<T>(c) || anfang_zurueck(c)) return 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -999,6 +1253,7 @@ struct gc_batch {
   void* tab = nullptr;             // StapelBlock<T>[sum of bloecke]
   int n_tab = 0;
   hipGraphExec_t graph = nullptr;
+  std::vector<uint64_t> folge;     // each system's launch-sequence generation at the graph's capture
   std::vector<hipEvent_t> ev;
   double ms_schritt = 0.0, ms_pcg = 0.0;
   // the systems' own kernels (build, release pass, update) run on several
@@ -1141,8 +1396,20 @@ static int batch_form(gc_batch* b, int threads, int gesamt) {
 
 template <typename T>
 static int batch_pcg(gc_batch* b, int durchgang) {
-  hipLaunchKernelGGL(k_pcg_stapel<T>, dim3(b->n_tab), dim3(b->threads), b->lds, b->stream, (const StapelBlock<T>*)b->tab,
-                     durchgang);
+  if (sizeof(T) == 4) {  // refined solves in rounds, as launch_verfeinert
+    const StapelBlock<float>* tab = (const StapelBlock<float>*)b->tab;
+    hipLaunchKernelGGL(k_verf_stapel, dim3(b->n_tab), dim3(b->threads), 0, b->stream, tab, durchgang, 0, 0);
+    for (int r = 0; r < GC_RUNDEN; ++r) {
+      hipLaunchKernelGGL(k_pcg_stapel<T>, dim3(b->n_tab), dim3(b->threads), b->lds, b->stream, (const StapelBlock<T>*)b->tab,
+                         durchgang);
+      hipLaunchKernelGGL(k_verf_stapel, dim3(b->n_tab), dim3(b->threads), 0, b->stream, tab, durchgang, 1,
+                         r == GC_RUNDEN - 1 ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_pcg64_stapel, dim3(b->n_tab), dim3(b->threads), 0, b->stream, tab, durchgang);
+  } else {
+    hipLaunchKernelGGL(k_pcg_stapel<T>, dim3(b->n_tab), dim3(b->threads), b->lds, b->stream, (const StapelBlock<T>*)b->tab,
+                       durchgang);
+  }
   BHIPCK(b, hipGetLastError());
   return 0;
 }
@@ -1214,13 +1481,40 @@ static int batch_ev(gc_batch* b, size_t anzahl) {
   return 0;
 }
 
+// nochmal: where given, the systems whose self-collision lists overflowed get
+// larger tables and *nochmal says that the call is to be taken again.
 template <typename T>
-static int batch_pruefen(gc_batch* b) {
+static int batch_pruefen(gc_batch* b, bool* nochmal = nullptr) {
   if (batch_sync(b)) return 1;
   for (gc_ctx* c : b->ctx) {
     c->built = false;
-    if (pruefe_gitter(c)) return bfehler(b, c->err);
+    bool ueberlauf = false;
+    if (pruefe_gitter(c, nochmal ? &ueberlauf : nullptr)) return bfehler(b, c->err);
+    if (ueberlauf) {
+      if (selbst_wachsen(c)) return bfehler(b, c->err);
+      *nochmal = true;
+    }
   }
+  return 0;
+}
+
+// With any system's self-collision on, every system's state and counters at
+// the start of a batch call are saved, so that the call can be taken again.
+template <typename T>
+static int batch_sichern(gc_batch* b, bool* sichern) {
+  *sichern = false;
+  for (gc_ctx* c : b->ctx) *sichern = *sichern || sys<T>(c).selbst != 0;
+  if (*sichern)
+    for (gc_ctx* c : b->ctx)
+      if (anfang_sichern(c)) return bfehler(b, c->err);
+  return 0;
+}
+
+template <typename T>
+static int batch_zurueck(gc_batch* b, double h) {
+  if (batch_vorbereiten<T>(b, h)) return 1;
+  for (gc_ctx* c : b->ctx)
+    if (anfang_zurueck(c)) return bfehler(b, c->err);
   return 0;
 }
 
@@ -1229,25 +1523,32 @@ static int batch_schritte(gc_batch* b, double h, int n) {
   StromTausch tausch(b);
   if (batch_vorbereiten<T>(b, h)) return 1;
   if (batch_ev(b, 4 * (size_t)n + 2)) return 1;
-  BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
-  for (int k = 0; k < n; ++k) {
-    if (batch_schritt<T>(b, &b->ev[2 + 4 * k])) return 1;
-    for (gc_ctx* c : b->ctx) c->hat_vorher = true;
+  bool sichern = false;
+  if (batch_sichern<T>(b, &sichern)) return 1;
+  for (;;) {
+    BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
+    for (int k = 0; k < n; ++k) {
+      if (batch_schritt<T>(b, &b->ev[2 + 4 * k])) return 1;
+      for (gc_ctx* c : b->ctx) c->hat_vorher = true;
+    }
+    BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
+    BHIPCK(b, hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    BHIPCK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->ms_schritt = ms;
+    double pcg = 0.0;
+    for (int k = 0; k < n; ++k) {
+      float m0 = 0.f, m1 = 0.f;
+      BHIPCK(b, hipEventElapsedTime(&m0, b->ev[2 + 4 * k], b->ev[3 + 4 * k]));
+      BHIPCK(b, hipEventElapsedTime(&m1, b->ev[4 + 4 * k], b->ev[5 + 4 * k]));
+      pcg += (double)m0 + (double)m1;
+    }
+    b->ms_pcg = pcg;
+    bool nochmal = false;
+    if (batch_pruefen<T>(b, sichern ? &nochmal : nullptr)) return 1;
+    if (!nochmal) return 0;
+    if (batch_zurueck<T>(b, h)) return 1;
   }
-  BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
-  BHIPCK(b, hipEventSynchronize(b->ev[1]));
-  float ms = 0.f;
-  BHIPCK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-  b->ms_schritt = ms;
-  double pcg = 0.0;
-  for (int k = 0; k < n; ++k) {
-    float m0 = 0.f, m1 = 0.f;
-    BHIPCK(b, hipEventElapsedTime(&m0, b->ev[2 + 4 * k], b->ev[3 + 4 * k]));
-    BHIPCK(b, hipEventElapsedTime(&m1, b->ev[4 + 4 * k], b->ev[5 + 4 * k]));
-    pcg += (double)m0 + (double)m1;
-  }
-  b->ms_pcg = pcg;
-  return batch_pruefen<T>(b);
 }
 
 template <typename T>
@@ -1263,27 +1564,44 @@ static int batch_schritte_graph(gc_batch* b, double h, int n) {
   }
   StromTausch tausch(b);
   if (batch_vorbereiten<T>(b, h)) return 1;
-  if (!b->graph) {
-    hipGraph_t g;
-    BHIPCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = batch_schritt<T>(b, nullptr);
-    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-    if (rc) return 1;
-    if (e2 != hipSuccess) return bfehler(b, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
-    hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e3 != hipSuccess) return bfehler(b, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
-  }
   if (batch_ev(b, 2)) return 1;
-  BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
-  for (int k = done; k < n; ++k) BHIPCK(b, hipGraphLaunch(b->graph, b->stream));
-  BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
-  BHIPCK(b, hipEventSynchronize(b->ev[1]));
-  float ms = 0.f;
-  BHIPCK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
-  b->ms_schritt = ms;
-  b->ms_pcg = std::nan("");
-  return batch_pruefen<T>(b);
+  bool sichern = false;
+  if (batch_sichern<T>(b, &sichern)) return 1;
+  for (;;) {
+    // a system's launch sequence changed since the capture (or its
+    // self-collision tables grew): capture again
+    bool gleich = b->folge.size() == b->ctx.size();
+    for (size_t q = 0; gleich && q < b->ctx.size(); ++q) gleich = b->folge[q] == b->ctx[q]->folge;
+    if (!gleich && b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+    if (!b->graph) {
+      b->folge.clear();
+      for (gc_ctx* c : b->ctx) b->folge.push_back(c->folge);
+      hipGraph_t g;
+      BHIPCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = batch_schritt<T>(b, nullptr);
+      hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if (rc) return 1;
+      if (e2 != hipSuccess) return bfehler(b, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+      hipError_t e3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e3 != hipSuccess) return bfehler(b, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
+    for (int k = done; k < n; ++k) BHIPCK(b, hipGraphLaunch(b->graph, b->stream));
+    BHIPCK(b, hipEventRecord(b->ev[1], b->stream));
+    BHIPCK(b, hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    BHIPCK(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    b->ms_schritt = ms;
+    b->ms_pcg = std::nan("");
+    bool nochmal = false;
+    if (batch_pruefen<T>(b, sichern ? &nochmal : nullptr)) return 1;
+    if (!nochmal) return 0;
+    if (batch_zurueck<T>(b, h)) return 1;
+  }
 }
 
 // Weights from the last measured iterations per solve of every system.
@@ -1314,7 +1632,7 @@ static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* 
     tmp.push_back(q);
     return q;
   };
-  struct Gesichert { void *x, *v, *dv; GridState alt, start; };
+  struct Gesichert { void *x, *v, *dv, *dv64; GridState alt, start; };
   std::vector<Gesichert> g(n);
   int rc = batch_sync(b);
   auto ende = [&]() {
@@ -1327,7 +1645,8 @@ static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* 
     g[k].x = neu(s.n * sizeof(V3<double>));
     g[k].v = neu(s.n * sizeof(V3<double>));
     g[k].dv = neu(s.n * sizeof(V3<T>));
-    if (!g[k].x || !g[k].v || !g[k].dv) { rc = bfehler(b, "hipMalloc failed (autotune)"); break; }
+    g[k].dv64 = s.dv64 ? neu(s.n * sizeof(V3<double>)) : nullptr;  // the FP64 warm start of a refined FP32 solve
+    if (!g[k].x || !g[k].v || !g[k].dv || (s.dv64 && !g[k].dv64)) { rc = bfehler(b, "hipMalloc failed (autotune)"); break; }
     if (hipMemcpyAsync(g[k].x, s.x, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
         hipMemcpyAsync(g[k].v, s.v, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
         hipMemcpyAsync(&g[k].alt, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, b->stream) != hipSuccess)
@@ -1341,6 +1660,8 @@ static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* 
   for (size_t k = 0; k < n && !rc; ++k) {
     const SysDev<T>& s = sys<T>(b->ctx[k]);
     if (hipMemcpyAsync(g[k].dv, s.dv, s.n * sizeof(V3<T>), hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+        (s.dv64 && hipMemcpyAsync(g[k].dv64, s.dv64, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream) !=
+                       hipSuccess) ||
         hipMemcpyAsync(&g[k].start, b->ctx[k]->gs, sizeof(GridState), hipMemcpyDeviceToHost, b->stream) != hipSuccess)
       rc = bfehler(b, "hipMemcpy failed (autotune)");
   }
@@ -1351,8 +1672,9 @@ static int batch_autotune(gc_batch* b, double h, int32_t* threads_out, int32_t* 
     for (size_t k = 0; k < n; ++k) {
       const SysDev<T>& s = sys<T>(b->ctx[k]);
       BHIPCK(b, hipMemcpyAsync(s.dv, g[k].dv, s.n * sizeof(V3<T>), hipMemcpyDeviceToDevice, b->stream));
+      if (s.dv64) BHIPCK(b, hipMemcpyAsync(s.dv64, g[k].dv64, s.n * sizeof(V3<double>), hipMemcpyDeviceToDevice, b->stream));
       BHIPCK(b, hipMemcpyAsync(b->ctx[k]->gs, &g[k].start, sizeof(GridState), hipMemcpyHostToDevice, b->stream));
-      hipLaunchKernelGGL(k_barriere_leeren, dim3(1), dim3(256), 0, b->stream, b->ctx[k]->slots);
+      hipLaunchKernelGGL(k_barriere_leeren, dim3(1), dim3(256), 0, b->stream, b->ctx[k]->gs, b->ctx[k]->slots);
     }
     BHIPCK(b, hipEventRecord(b->ev[0], b->stream));
     if (batch_pcg<T>(b, 0)) return 1;
@@ -1465,16 +1787,18 @@ static int loese(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
   if (!c->built) return fehler(c, "no system built (call gc_build first)");
   SysDev<T>& s = sys<T>(c);
   if (hoch_vec<T>(c, s.dv, dv)) return 1;
+  // FP32 bulk: the warm start and the solution are the FP64 dv64
+  if (s.dv64 && hoch_vec<double>(c, s.dv64, dv)) return 1;
   if (sys_hoch<T>(c)) return 1;
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, c->stream, c->gs, c->slots);
   if (launch_pcg<T>(c, 0)) return 1;
   GridState g;
   HIPCK(c, hipMemcpyAsync(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
-  if (runter_vec<T>(c, dv, s.dv, c->n)) return 1;
+  if (s.dv64 ? runter_vec<double>(c, dv, s.dv64, c->n) : runter_vec<T>(c, dv, s.dv, c->n)) return 1;
   if (g.fehler) {
     unsigned null = 0;
     HIPCK(c, hipMemcpy(&c->gs->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
-    return fehler(c, "grid barrier wait exceeded its spin limit");
+    return fehler(c, barriere_meldung(g.fehler));
   }
   if (lauf) {
     lauf->iterationen = g.iterationen;
@@ -1621,6 +1945,10 @@ static int adj_schritt(gc_ctx* c, double h) {
   sa.b = (V3<T>*)c->adj_b;
   sa.dv = (V3<T>*)c->adj_mu;
   sa.soll_direkt = a.ziel;
+  // the adjoint solve runs on the bulk system alone (its right-hand side has
+  // no FP64 copy); the step's own solution stays readable through sp's dv64
+  sa.schatten = nullptr;
+  sa.dv64 = nullptr;
   sa.boden = 0.0;
   c->adj_sys_host.assign((const unsigned char*)&sa, (const unsigned char*)&sa + sizeof(sa));
   HIPCK(c, hipMemcpyAsync(c->adj_sys_dev, c->adj_sys_host.data(), sizeof(sa), hipMemcpyHostToDevice, st));
@@ -1646,7 +1974,7 @@ static int adj_schritt(gc_ctx* c, double h) {
   if (fe) {
     unsigned null = 0;
     HIPCK(c, hipMemcpy(&c->gs_adj->fehler, &null, sizeof(unsigned), hipMemcpyHostToDevice));
-    return fehler(c, "grid barrier wait exceeded its spin limit (adjoint solve)");
+    return fehler(c, barriere_meldung(fe) + " (adjoint solve)");
   }
   return 0;
 }
@@ -1736,9 +2064,6 @@ static int adj_gradient_lesen(gc_ctx* c, double* d_kanten, double* d_naehte, dou
   return 0;
 }
 
-static V3<double>* zustand_x(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.x : c->sd.x; }
-static V3<double>* zustand_v(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.v : c->sd.v; }
-static V3<double>* zustand_dvv(gc_ctx* c) { return c->opt.praezision == GC_FP32 ? c->sf.dv_vorher : c->sd.dv_vorher; }
 
 extern "C" {
 
@@ -1776,6 +2101,7 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
     melde(std::string("built for gfx950, device is ") + p.gcnArchName);
     return nullptr;
   }
+
   gc_ctx* c = new gc_ctx();
   c->opt = o;
   c->cus = p.multiProcessorCount;
@@ -1785,6 +2111,7 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
   c->lds_budget = (size_t)(o.lds_kb > 0 ? o.lds_kb : 80) * 1024;
   if (c->lds_budget + 1024 > (size_t)p.sharedMemPerBlock) {
     melde("lds_kb exceeds the LDS of a workgroup");
+    delete c;
     return nullptr;
   }
   c->pcg_threads = o.threads > 0 ? o.threads : 1024;
@@ -1923,8 +2250,22 @@ int gc_set_topology(gc_ctx* c, const gc_topologie* t) {
 int gc_set_param(gc_ctx* c, const gc_param* p) {
   if (!c || !p) return 1;
   if (p->cg_max < 0) return fehler(c, "cg_max must be >= 0");
+  const bool selbst_vorher = selbst_an(c);
+  const int stoss_vorher = c->param.selbst_stoss_iterationen;
   c->param = *p;
   c->hat_param = true;
+  // self-collision on or off and the impact iterations decide which kernels a
+  // substep launches: a captured graph of the old sequence is dropped
+  if (selbst_an(c) != selbst_vorher || (selbst_an(c) && p->selbst_stoss_iterationen != stoss_vorher)) folge_neu(c);
+  if (c->sk_bereit) {
+    // the kernels read the spring stiffness factor, the seam-cut switch and
+    // the impact iterations from the device copy of the self-collision tables
+    c->sk.steifigkeit = p->selbst_steifigkeit;
+    c->sk.an_naehten = p->selbst_schnitte_an_naehten;
+    c->sk.stoss_iterationen = p->selbst_stoss_iterationen;
+    HIPCK(c, hipMemcpyAsync(c->sk_dev, &c->sk, sizeof(SelbstDev), hipMemcpyHostToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+  }
   if (c->topo) {
     // seam stiffness is the parameter naht_k (implizit.rs:760, 771)
     for (int k = 0; k < c->nk; ++k)
@@ -2019,10 +2360,7 @@ int gc_set_koerper_lage(gc_ctx* c, int32_t k, const double lage[3]) {
   if (!c || k < 0 || k >= (int)c->koerper.size()) return 1;
   for (int q = 0; q < 3; ++q) c->koerper[k].k.punkt[q] = lage[q];
   alle_param(c);
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
+  folge_neu(c);
   return 0;
 }
 
@@ -2096,6 +2434,7 @@ int gc_solve(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
   if (!c || !dv) return 1;
   return c->opt.praezision == GC_FP32 ? loese<float>(c, dv, lauf) : loese<double>(c, dv, lauf);
 }
+
 
 int gc_apply_A(gc_ctx* c, const double* p, double* out) {
   if (!c || !p || !out) return 1;
@@ -2180,6 +2519,7 @@ int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
     out->durchstoesse = g.durchstoesse;
     out->cg_rundungsgrenze = g.rundungsgrenze;
     out->cg_neustarts = g.neustarts;
+    out->selbst_listen_faktor = c->sk_mal;
   }
   if (reset) {
     g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
@@ -2527,23 +2867,12 @@ int gc_set_steifigkeit(gc_ctx* c, const double* kanten_k, const double* biegung_
 
 int gc_zustand_sichern(gc_ctx* c, int32_t slot) {
   if (!c || !c->topo) return 1;
+  const bool neu = c->zustaende.find(slot) == c->zustaende.end();
   gc_ctx::Zustand& z = c->zustaende[slot];
-  const size_t n = (size_t)c->n;
-  if (!z.x) {
-    z.x = dalloc<V3<double>>(c, n);
-    z.v = dalloc<V3<double>>(c, n);
-    z.dvv = dalloc<V3<double>>(c, n);
-    if (!z.x || !z.v || !z.dvv) {
-      c->zustaende.erase(slot);
-      return fehler(c, "hipMalloc failed (saved state)");
-    }
+  if (zustand_sichern_in(c, z)) {
+    if (neu && !z.x) c->zustaende.erase(slot);
+    return 1;
   }
-  const size_t bytes = n * sizeof(V3<double>);
-  HIPCK(c, hipMemcpyAsync(z.x, zustand_x(c), bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, hipMemcpyAsync(z.v, zustand_v(c), bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, hipMemcpyAsync(z.dvv, zustand_dvv(c), bytes, hipMemcpyDeviceToDevice, c->stream));
-  z.hat_vorher = c->hat_vorher;
-  HIPCK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -2551,14 +2880,7 @@ int gc_zustand_laden(gc_ctx* c, int32_t slot) {
   if (!c || !c->topo) return 1;
   auto it = c->zustaende.find(slot);
   if (it == c->zustaende.end()) return fehler(c, "gc_zustand_laden: no state saved in this slot");
-  const gc_ctx::Zustand& z = it->second;
-  const size_t bytes = (size_t)c->n * sizeof(V3<double>);
-  HIPCK(c, hipMemcpyAsync(zustand_x(c), z.x, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, hipMemcpyAsync(zustand_v(c), z.v, bytes, hipMemcpyDeviceToDevice, c->stream));
-  HIPCK(c, hipMemcpyAsync(zustand_dvv(c), z.dvv, bytes, hipMemcpyDeviceToDevice, c->stream));
-  c->hat_vorher = z.hat_vorher;
-  c->built = false;
-  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (zustand_laden_aus(c, it->second)) return 1;
   alle_param(c);
   return 0;
 }

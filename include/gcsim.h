@@ -127,8 +127,13 @@ typedef struct gc_statistik {
   int64_t durchdringungen;
   int64_t durchdringungsgebiete;
   int64_t durchstoesse;
-  int64_t cg_rundungsgrenze;   /* FP32 solves ended at their rounding floor (no FP64 counterpart) */
-  int64_t cg_neustarts;        /* FP32 true-residual restarts of the PCG (no FP64 counterpart) */
+  int64_t cg_rundungsgrenze;   /* FP32 solves whose iterations stopped lowering the FP64 residual (the
+                                * system too ill-conditioned for FP32) and that FP64 iterations finished
+                                * (no FP64 counterpart) */
+  int64_t cg_neustarts;        /* FP32 restarts of the PCG from the FP64 residual (no FP64 counterpart) */
+  int64_t selbst_listen_faktor; /* capacity of the self-collision lists, in multiples of the default (a call
+                                 * whose lists overflow is taken again with them doubled, up to 16; the
+                                 * reference keeps them in growable vectors). Not reset. */
 } gc_statistik;
 
 /* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):

@@ -440,3 +440,45 @@ def sdf_sample(gitter, q):
     ok = L.or_sdf_sample(_p(w, C.POINTER(C.c_float)), _p(dims, I32), _p(f64(gitter.origin)), C.c_double(gitter.spacing),
                          _p(f64(q)), C.byref(d), _p(g))
     return (d.value, g) if ok else None
+
+
+# ---------------------------------------------------------------------------
+# Measurements of a state (klon3d-sim/src/messung.rs), used by the acceptance
+# checks the reference runs on its GPU backend (tests/gpu.rs:115-206)
+# ---------------------------------------------------------------------------
+def umfang_in_ebene(x, dreiecke, punkt, normale) -> float:
+    """messung.rs:10-31: length of the mesh's cut with the plane
+    {p : (p - punkt) . n = 0}; per triangle that crosses it, the segment between
+    the crossings of its edges (edge k from corner k to corner k+1)."""
+    x = f64(x)
+    n = f64(normale) / np.linalg.norm(normale)
+    summe = 0.0
+    for d in np.asarray(dreiecke):
+        p = x[d]
+        h = (p - f64(punkt)) @ n
+        schnitt = []
+        for k in range(3):
+            a, b = k, (k + 1) % 3
+            if (h[a] < 0.0) != (h[b] < 0.0):
+                t = h[a] / (h[a] - h[b])
+                schnitt.append(p[a] + (p[b] - p[a]) * t)
+        if len(schnitt) == 2:
+            summe += float(np.linalg.norm(schnitt[1] - schnitt[0]))
+    return summe
+
+
+def nahtspalt(x, naht_punkte) -> float:
+    """messung.rs:57-64: the largest distance of a seam pair."""
+    x = f64(x)
+    nr = np.asarray(naht_punkte)
+    if nr.size == 0:
+        return 0.0
+    return float(np.max(np.linalg.norm(x[nr[:, 0]] - x[nr[:, 1]], axis=1)))
+
+
+def mittlere_geschwindigkeit(v, masse) -> float:
+    """messung.rs:113-116 with bewegungsenergie (:84-87): sqrt(2 E / m), the
+    rest criterion of roehre::bis_zur_ruhe."""
+    v, m = f64(v), f64(masse)
+    e = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
+    return float(np.sqrt(2.0 * e / np.sum(m)))
