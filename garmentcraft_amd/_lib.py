@@ -91,6 +91,7 @@ SIGNATUREN = [
     ("gc_set_fest", C.c_int, [C.c_void_p, I32, D, D, D]),
     ("gc_get_fest", C.c_int, [C.c_void_p, I32, D, D, D]),
     ("gc_solve", C.c_int, [C.c_void_p, D, C.POINTER(CgLauf_c)]),
+    ("gc_solve_ohne_speicher", C.c_int, [C.c_void_p, D, C.POINTER(CgLauf_c)]),
     ("gc_apply_A", C.c_int, [C.c_void_p, D, D]),
     ("gc_apply_H", C.c_int, [C.c_void_p, D, D]),
     ("gc_residual", C.c_int, [C.c_void_p, D, D]),

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1808,6 +1809,93 @@ static int loese(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// The solve without allocations (gc_solve_ohne_speicher): the grid state and
+// the reduction/barrier slots of its PCG launch live in this array instead
+// of the context's buffers. Its layout: the grid state at byte 0, the
+// GC_SLOTS slots from byte GC_OS_SLOTS on. Each device holds its own copy;
+// host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name. Every call on a device shares that device's copy, so calls
+// that overlap on one device race.
+// ---------------------------------------------------------------------------
+#define GC_OS_SLOTS 256
+#define GC_OS_BYTES (GC_OS_SLOTS + GC_SLOTS * sizeof(Slot))
+static_assert(sizeof(GridState) <= GC_OS_SLOTS, "the grid state fits before the slots");
+static __device__ __align__(256) unsigned char scratch[GC_OS_BYTES];
+
+#define GC_OS_GERAETE 64
+static std::mutex os_mutex;
+static void* os_adresse[GC_OS_GERAETE] = {};
+
+// The device address of `scratch` on device g (current), fetched once per
+// device. Called by gc_create, so the entry point itself only looks it up.
+static int os_adresse_holen(int g, std::string* err) {
+  if (g < 0 || g >= GC_OS_GERAETE) {
+    *err = "device ordinal beyond the scratch table";
+    return 1;
+  }
+  std::lock_guard<std::mutex> lock(os_mutex);
+  if (os_adresse[g]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return 1;
+  }
+  os_adresse[g] = ptr;
+  return 0;
+}
+
+// Clears the whole grid state (counters, error flag, results) and the
+// barrier counters of a slot array (a fill kernel: no memset node when a
+// caller captures the call).
+__global__ void k_os_leeren(GridState* gs, Slot* slots) {
+  unsigned* w = (unsigned*)gs;
+  for (unsigned k = threadIdx.x; k < sizeof(GridState) / sizeof(unsigned); k += blockDim.x) w[k] = 0u;
+  barriere_leeren(slots);
+}
+
+template <typename T>
+static int loese_ohne_speicher(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
+  if (!c->built) return fehler(c, "no system built (call gc_build first)");
+  if (c->pcg_blocks > GC_MAX_BLOECKE)
+    return fehler(c, "gc_solve_ohne_speicher: " + std::to_string(c->pcg_blocks) + " PCG blocks exceed the " +
+                         std::to_string(GC_MAX_BLOECKE) + " the scratch array holds reductions for");
+  void* basis = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(os_mutex);
+    basis = os_adresse[c->opt.geraet];
+  }
+  if (!basis) return fehler(c, "gc_solve_ohne_speicher: no scratch address for this device");
+  GridState* gs = (GridState*)basis;
+  Slot* slots = (Slot*)((unsigned char*)basis + GC_OS_SLOTS);
+  SysDev<T>& s = sys<T>(c);
+  // the warm start straight into the solution vector: FP64 dv64 in the
+  // FP32-bulk form (the correction dv is cleared by the solve), dv in FP64
+  const size_t bytes = (size_t)c->n * sizeof(V3<double>);
+  V3<double>* ziel = sizeof(T) == 4 ? s.dv64 : (V3<double>*)s.dv;
+  if (!ziel) return fehler(c, "gc_solve_ohne_speicher: the FP32 form needs its FP64 solution vector");
+  if (sys_hoch<T>(c)) return 1;
+  HIPCK(c, hipMemcpyAsync(ziel, dv, bytes, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_os_leeren, dim3(1), dim3(256), 0, c->stream, gs, slots);
+  if (sizeof(T) == 4)
+    launch_verfeinert(c->stream, c->pcg_blocks, c->pcg_threads, c->pcg_lds, (const SysDev<float>*)dsys<T>(c), gs, slots, 0);
+  else
+    hipLaunchKernelGGL(k_pcg<T>, dim3(c->pcg_blocks), dim3(c->pcg_threads), c->pcg_lds, c->stream, dsys<T>(c), gs, slots, 0);
+  HIPCK(c, hipGetLastError());
+  GridState g;
+  HIPCK(c, hipMemcpyAsync(&g, gs, sizeof(GridState), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipMemcpyAsync(dv, ziel, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  if (g.fehler) return fehler(c, barriere_meldung(g.fehler));
+  if (lauf) {
+    lauf->iterationen = g.iterationen;
+    lauf->nicht_konvergiert = g.nicht_konv;
+    lauf->rest = g.rest;
+  }
+  return 0;
+}
+
 template <typename T>
 static int fest_setzen(gc_ctx* c, const int32_t* flag, const double* normale, const double* vn_soll, const double* vk) {
   SysDev<T>& s = sys<T>(c);
@@ -2101,7 +2189,15 @@ gc_ctx* gc_create(const gc_optionen* opt, char* err, int32_t err_len) {
     melde(std::string("built for gfx950, device is ") + p.gcnArchName);
     return nullptr;
   }
-
+  {
+    // the device address of the no-allocation solve's scratch array, once
+    // per device, with the device current
+    std::string e;
+    if (os_adresse_holen(o.geraet, &e)) {
+      melde(e);
+      return nullptr;
+    }
+  }
   gc_ctx* c = new gc_ctx();
   c->opt = o;
   c->cus = p.multiProcessorCount;
@@ -2435,6 +2531,10 @@ int gc_solve(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
   return c->opt.praezision == GC_FP32 ? loese<float>(c, dv, lauf) : loese<double>(c, dv, lauf);
 }
 
+int gc_solve_ohne_speicher(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
+  if (!c || !dv) return 1;
+  return c->opt.praezision == GC_FP32 ? loese_ohne_speicher<float>(c, dv, lauf) : loese_ohne_speicher<double>(c, dv, lauf);
+}
 
 int gc_apply_A(gc_ctx* c, const double* p, double* out) {
   if (!c || !p || !out) return 1;

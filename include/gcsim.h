@@ -190,6 +190,16 @@ int gc_get_fest(gc_ctx* ctx, int32_t* flag, double* normale, double* vn_soll, do
 /* `Loeser::loese` (implizit.rs:308, CpuLoeser at 370-426): modified PCG;
  * dv in = warm start, out = solution. */
 int gc_solve(gc_ctx* ctx, double* dv, gc_cg_lauf* lauf);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it): gc_solve without any allocation. The grid state and the
+ * reduction and barrier slots of its PCG launch (up to 1024 blocks, about
+ * 164 KiB) live in a static device array, one per device, instead of the
+ * context's buffers; everything else is the context's, set up before. The
+ * warm start and the solution are copied straight between dv and the
+ * device's solution vector. Calls that overlap on one device race on that
+ * array; the result equals gc_solve's to the bit. */
+int gc_solve_ohne_speicher(gc_ctx* ctx, double* dv, gc_cg_lauf* lauf);
 /* `Loeser::mal_a` (implizit.rs:310): unfiltered A*p. */
 int gc_apply_A(gc_ctx* ctx, const double* p, double* out);
 /* `System::mal_h` (implizit.rs:185-221): H*p without mass and h^2. */
