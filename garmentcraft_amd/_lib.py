@@ -126,6 +126,8 @@ SIGNATUREN = [
     ("gc_set_steifigkeit", C.c_int, [C.c_void_p, D, D, D, D]),
     ("gc_zustand_sichern", C.c_int, [C.c_void_p, C.c_int32]),
     ("gc_zustand_laden", C.c_int, [C.c_void_p, C.c_int32]),
+    ("gc_entscheidungen", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("gc_entscheidungen_abweichungen", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]),
 ]
 
 _lib = None

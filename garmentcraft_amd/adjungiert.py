@@ -106,15 +106,27 @@ class Rollout:
         self.g.set_steifigkeit(membran_ek_es_g=md, biegung_k=bk, reibung=reib)
         self.theta = np.array(theta, dtype=np.float64)
 
-    def vorwaerts(self, theta) -> np.ndarray:
-        """Final positions (n, 3) after the rollout with theta's material."""
+    def vorwaerts(self, theta, halten: Optional[str] = None) -> np.ndarray:
+        """Final positions (n, 3) after the rollout with theta's material.
+
+        halten="aufnehmen" records every step's discrete decisions (contacts,
+        releases, stick or slip, path entries, push-outs), halten="abspielen"
+        replays the recorded ones instead of deciding: finite differences of
+        the branch the adjoint differentiates."""
         self._setzen(theta)
         g = self.g
         g.laden(START_SLOT)
-        for k in range(self.schritte):
-            if k % self.alle == 0:
-                g.sichern(k // self.alle)
-            g.schritte(self.h, 1)
+        modus = {None: 0, "aufnehmen": 1, "abspielen": 2}[halten]
+        try:
+            for k in range(self.schritte):
+                if k % self.alle == 0:
+                    g.sichern(k // self.alle)
+                if modus:
+                    g.entscheidungen(modus, k)
+                g.schritte(self.h, 1)
+        finally:
+            if modus:
+                g.entscheidungen(0)
         return g.lage()[0]
 
     def rueckwaerts(self, gx, gv=None) -> np.ndarray:

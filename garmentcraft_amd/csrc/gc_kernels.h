@@ -96,6 +96,15 @@ struct FederK {
   T B[9];           // the spring's block
 };
 
+// Decisions of a step held for replay (one body): contact of the step,
+// released after solve 1 / 2, sticking, path entry and its impact (normal
+// velocity removed, tangential one stopped), push-out and its normal velocity
+// removal.
+enum : uint16_t {
+  GC_H_KONTAKT = 1, GC_H_FREI0 = 2, GC_H_FREI1 = 4, GC_H_HAFT = 8, GC_H_EINTRITT = 16, GC_H_EIN_VN = 32,
+  GC_H_EIN_HAFT = 64, GC_H_RAUS = 128, GC_H_RAUS_VN = 256
+};
+
 template <typename T>
 struct SysDev {
   int32_t n, nk, nb, nm;
@@ -166,6 +175,11 @@ struct SysDev {
   // the FP64 system's solution rather than to the FP32-rounded one's.
   V3<double>* dv64;
   const void* schatten;
+  // The step's discrete decisions per point (bits GC_H_*): recorded
+  // (halten_modus 1) or replayed instead of decided (2), for finite
+  // differences on the branch the adjoint differentiates; 0: off.
+  uint16_t* halten;
+  int32_t halten_modus;
   // normal target of held points for the solve; null: vn_soll - v.n of the
   // step (implizit.rs:375-384); set: the adjoint solve's own targets
   const double* soll_direkt;
@@ -232,6 +246,7 @@ struct GridState {
   unsigned int spins_max;    // longest barrier wait (polls) since the last read
   unsigned int selbst_ueberlauf;  // a self-collision list exceeded its capacity
   long long naehe_pd, naehe_kk, selbst_stoesse, durchdringungen, gebiete, durchstoesse;
+  long long halten_abweichungen;  // replayed decisions that the state no longer allowed (taken as decided)
   long long rundungsgrenze;  // FP32 solves stopped at their rounding floor
   long long neustarts;       // FP32 true-residual restarts
   // result of the last solve
@@ -637,6 +652,19 @@ __global__ void k_fest(const SysDev<T>* __restrict__ sp, GridState* gs) {
     double soll = 0.0;
     if (s.w[i] <= 0.0) {
       fl = 2;
+    } else if (s.halten_modus == 2) {  // the recorded contact (one body)
+      V3<double> nor, vk;
+      double tiefe;
+      if (s.halten[i] & GC_H_KONTAKT) {
+        if (kontakt_roh(s.koerper[0], s.x[i], s.koll[i] + s.kontakt_rand, &nor, &tiefe, &vk)) {
+          fl = 1;
+          nn = nor;
+          soll = (tiefe - s.kontakt_rand) / s.h + dot(vk, nor);
+          vkk = vk;
+        } else {
+          atomicAdd((unsigned long long*)&gs->halten_abweichungen, 1ull);
+        }
+      }
     } else {
       for (int b = 0; b < s.n_koerper; ++b) {
         V3<double> nor, vk;
@@ -650,6 +678,7 @@ __global__ void k_fest(const SysDev<T>* __restrict__ sp, GridState* gs) {
         }
       }
     }
+    if (s.halten_modus == 1) s.halten[i] = fl == 1 ? GC_H_KONTAKT : 0;
     s.fflag[i] = fl;
     s.fnorm[i] = cast3<T>(nn);
     s.fnorm64[i] = nn;
@@ -1602,7 +1631,10 @@ __global__ void k_freigabe(const SysDev<T>* __restrict__ sp, GridState* gs, Slot
       rest = cast3<double>(ap) - cast3<double>(s.b[i]);
     }
     const double fn = dot(rest, s.fnorm64[i]) / s.h;
-    if (fn < 0.0) {
+    const uint16_t bit = durchgang == 1 ? GC_H_FREI1 : GC_H_FREI0;
+    const bool los = s.halten_modus == 2 ? (s.halten[i] & bit) != 0 : fn < 0.0;
+    if (s.halten_modus == 1 && los) s.halten[i] |= bit;
+    if (los) {
       // released; after the second solve (no third follows) the point keeps
       // flag 3 = "held in the last solve, free in the update", with the
       // normal force of that solve for the step's adjoint. Every kernel of
@@ -1631,10 +1663,19 @@ template <typename T>
 __global__ void k_update(const SysDev<T>* __restrict__ sp, GridState* gs) {
   const SysDev<T>& s = *sp;
   unsigned kont = 0, haft = 0, stoss = 0;
+  unsigned long long abweichung = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
     const V3<double> dvi = dv_loesung(s, (uint32_t)i);
     s.dv_vorher[i] = dvi;
     if (s.w[i] <= 0.0) continue;
+    // a decision as taken (recorded when halten_modus is 1) or as recorded (2)
+    uint16_t bits = s.halten_modus == 2 ? s.halten[i] : 0;
+    auto wahl = [&](uint16_t bit, bool frei) {
+      const bool j = s.halten_modus == 2 ? (bits & bit) != 0 : frei;
+      if (s.halten_modus == 1 && j) bits |= bit;
+      return j;
+    };
+    if (s.halten_modus == 1) bits = s.halten[i];
     V3<double> v = s.v[i] + dvi;
     if (s.fflag[i] == 1) {
       ++kont;
@@ -1644,7 +1685,7 @@ __global__ void k_update(const SysDev<T>* __restrict__ sp, GridState* gs) {
       const double lt = norm(t);
       const double normal = s.aus_last ? gmax(-dot(s.last[i], n), 0.0) : s.fn_kraft[i];
       const double grenze = s.reib[i] * normal * s.h * s.w[i];
-      if (lt <= grenze) {
+      if (wahl(GC_H_HAFT, lt <= grenze)) {
         ++haft;
         v = v - t;
       } else if (lt > 0.0) {
@@ -1658,16 +1699,25 @@ __global__ void k_update(const SysDev<T>* __restrict__ sp, GridState* gs) {
     if (s.durchstoss_pruefen)
       for (int b = 0; b < s.n_koerper; ++b) {
         V3<double> p, nk, vk;
-        if (durchstoss(s.koerper[b], alt, x, &p, &nk, &vk)) {
+        bool ein = false;
+        if (s.halten_modus == 2) {
+          if (bits & GC_H_EINTRITT) {
+            ein = durchstoss(s.koerper[b], alt, x, &p, &nk, &vk);
+            if (!ein) ++abweichung;
+          }
+        } else {
+          ein = wahl(GC_H_EINTRITT, durchstoss(s.koerper[b], alt, x, &p, &nk, &vk));
+        }
+        if (ein) {
           ++stoss;
           x = p + nk * s.koll[i];
           const V3<double> rel = s.v[i] - vk;
           const double vn = dot(rel, nk);
-          if (vn < 0.0) {
+          if (wahl(GC_H_EIN_VN, vn < 0.0)) {
             V3<double> t = rel - nk * vn;
             const double lt = norm(t);
             const double grenze = s.reib[i] * -vn;
-            t = lt <= grenze ? zero3<double>() : t * (1.0 - grenze / lt);
+            t = wahl(GC_H_EIN_HAFT, lt <= grenze) ? zero3<double>() : t * (1.0 - grenze / lt);
             s.v[i] = vk + t;
           }
         }
@@ -1675,17 +1725,28 @@ __global__ void k_update(const SysDev<T>* __restrict__ sp, GridState* gs) {
     for (int b = 0; b < s.n_koerper; ++b) {
       V3<double> nor, vk;
       double tiefe;
-      if (kontakt(s.koerper[b], x, s.koll[i], &nor, &tiefe, &vk)) {
+      bool raus = false;
+      if (s.halten_modus == 2) {
+        if (bits & GC_H_RAUS) {
+          raus = kontakt_roh(s.koerper[b], x, s.koll[i], &nor, &tiefe, &vk);
+          if (!raus) ++abweichung;
+        }
+      } else {
+        raus = wahl(GC_H_RAUS, kontakt(s.koerper[b], x, s.koll[i], &nor, &tiefe, &vk));
+      }
+      if (raus) {
         x = x + nor * tiefe;
         const double vn = dot(s.v[i] - vk, nor);
-        if (vn < 0.0) s.v[i] = s.v[i] - nor * vn;
+        if (wahl(GC_H_RAUS_VN, vn < 0.0)) s.v[i] = s.v[i] - nor * vn;
       }
     }
     s.x[i] = x;
+    if (s.halten_modus == 1) s.halten[i] = bits;
   }
   if (kont) atomicAdd((unsigned long long*)&gs->kontakte, (unsigned long long)kont);
   if (haft) atomicAdd((unsigned long long*)&gs->haftend, (unsigned long long)haft);
   if (stoss) atomicAdd((unsigned long long*)&gs->durchstoesse, (unsigned long long)stoss);
+  if (abweichung) atomicAdd((unsigned long long*)&gs->halten_abweichungen, abweichung);
 }
 
 // Utility kernels for the Loeser-level entry points.

@@ -396,6 +396,35 @@ GC_HD bool kontakt(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* tief
   return true;
 }
 
+// The contact of kontakt() without its test for depth > 0: the normal and the
+// (possibly negative) depth wherever the body's distance is defined. A step
+// that replays recorded decisions (finite differences of a fixed branch)
+// keeps a contact whose point has moved out by a rounding's width.
+template <typename S>
+GC_HD bool kontakt_roh(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* tiefe, V3<double>* vk) {
+  *vk = V3<double>{k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+  if (k.art == GC_ZYLINDER) {
+    V3<S> ach = {S(k.achse[0]), S(k.achse[1]), S(k.achse[2])};
+    V3<S> a = ach * (S(1.0) / norm(ach));
+    V3<S> d = x - V3<S>{S(k.punkt[0]), S(k.punkt[1]), S(k.punkt[2])};
+    V3<S> quer = d - a * dot(d, a);
+    S r = norm(quer);
+    if (wert(r) < 1e-12) return false;
+    *normale = quer * (S(1.0) / r);
+    *tiefe = S(k.radius) + abstand - r;
+    return true;
+  }
+  S q[3] = {x.x - S(k.punkt[0]), x.y - S(k.punkt[1]), x.z - S(k.punkt[2])};
+  S dd, g[3];
+  if (!sdf_sample(k, q, &dd, g)) return false;
+  V3<S> gv = {g[0], g[1], g[2]};
+  S lg = norm(gv);
+  if (!(wert(lg) > 1e-12)) return false;
+  *normale = gv * (S(1.0) / lg);
+  *tiefe = abstand - dd;
+  return true;
+}
+
 // Entry of the path a -> b into the body: the counterpart of
 // Koerpernetz::durchstoss (netzkoerper.rs:541-596, used at implizit.rs:984-1003)
 // for the grid SDF that stands in for the mesh body. First sign change

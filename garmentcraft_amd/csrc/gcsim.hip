@@ -137,6 +137,8 @@ struct gc_ctx {
     bool hat_vorher = false;
   };
   std::map<int, Zustand> zustaende;
+  // recorded decisions of steps by slot (gc_entscheidungen)
+  std::map<int, uint16_t*> entscheidungen;
   // the state at the start of the current gc_step / gc_step_graph call, with
   // the step counters, for taking the call again after a list overflow
   Zustand anfang;
@@ -2972,6 +2974,45 @@ int gc_zustand_sichern(gc_ctx* c, int32_t slot) {
   if (zustand_sichern_in(c, z)) {
     if (neu && !z.x) c->zustaende.erase(slot);
     return 1;
+  }
+  return 0;
+}
+
+int gc_entscheidungen(gc_ctx* c, int32_t modus, int32_t slot) {
+  if (!c || !c->topo) return 1;
+  if (modus < 0 || modus > 2) return fehler(c, "gc_entscheidungen: modus must be 0 (off), 1 (record) or 2 (replay)");
+  uint16_t* puffer = nullptr;
+  if (modus != 0) {
+    if (c->koerper.size() > 1) return fehler(c, "gc_entscheidungen: decisions are held for at most one body");
+    if (selbst_an(c)) return fehler(c, "gc_entscheidungen: self-collision decisions are not held (switch it off)");
+    auto it = c->entscheidungen.find(slot);
+    if (it == c->entscheidungen.end()) {
+      if (modus == 2) return fehler(c, "gc_entscheidungen: nothing recorded in this slot");
+      puffer = dalloc<uint16_t>(c, (size_t)c->n);
+      if (!puffer) return fehler(c, "hipMalloc failed (decisions)");
+      c->entscheidungen[slot] = puffer;
+    } else {
+      puffer = it->second;
+    }
+  }
+  if (c->opt.praezision == GC_FP32) {
+    c->sf.halten = puffer;
+    c->sf.halten_modus = modus;
+  } else {
+    c->sd.halten = puffer;
+    c->sd.halten_modus = modus;
+  }
+  return 0;
+}
+
+int gc_entscheidungen_abweichungen(gc_ctx* c, int64_t* anzahl, int32_t reset) {
+  if (!c || !anzahl) return 1;
+  GridState g;
+  HIPCK(c, hipMemcpy(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost));
+  *anzahl = g.halten_abweichungen;
+  if (reset) {
+    long long null = 0;
+    HIPCK(c, hipMemcpy(&c->gs->halten_abweichungen, &null, sizeof(null), hipMemcpyHostToDevice));
   }
   return 0;
 }

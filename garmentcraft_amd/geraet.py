@@ -315,6 +315,16 @@ class GeraetSzene:
     def sichern(self, slot: int) -> None:
         self._ck(self.lib.gc_zustand_sichern(self.h, slot), "gc_zustand_sichern")
 
+    def entscheidungen(self, modus: int, slot: int = 0) -> None:
+        """Record (1), replay (2) or stop holding (0) the steps' discrete decisions."""
+        self._ck(self.lib.gc_entscheidungen(self.h, int(modus), int(slot)), "gc_entscheidungen")
+
+    def entscheidungen_abweichungen(self, reset: bool = False) -> int:
+        n = C.c_int64()
+        self._ck(self.lib.gc_entscheidungen_abweichungen(self.h, C.byref(n), 1 if reset else 0),
+                 "gc_entscheidungen_abweichungen")
+        return int(n.value)
+
     def laden(self, slot: int) -> None:
         self._ck(self.lib.gc_zustand_laden(self.h, slot), "gc_zustand_laden")
 

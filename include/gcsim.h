@@ -329,6 +329,18 @@ int gc_adjoint_statistik(gc_ctx* ctx, gc_adjoint_stat* out, int32_t reset);
  * writes these values when it builds the scene (szene.rs:425-461, 602-647). */
 int gc_set_steifigkeit(gc_ctx* ctx, const double* kanten_k, const double* biegung_k, const double* membran_ek_es_g,
                        const double* reibung);
+/* The discrete decisions of a step held fixed, for finite differences of the
+ * branch the adjoint differentiates (no counterpart in the reference): modus 1
+ * records, per point, the step's contact, its releases after solve 1 and 2,
+ * stick or slip, path entry and push-out (with their velocity branches) into
+ * slot `slot`; modus 2 replays that slot's record instead of deciding (a
+ * contact whose point moved out by a rounding's width keeps its normal and
+ * a depth <= 0); modus 0 decides as the reference does. One body at most,
+ * self-collision off. The setting holds for the following steps until
+ * changed. gc_entscheidungen_abweichungen counts replayed decisions the state
+ * no longer allowed (taken as decided instead). */
+int gc_entscheidungen(gc_ctx* ctx, int32_t modus, int32_t slot);
+int gc_entscheidungen_abweichungen(gc_ctx* ctx, int64_t* anzahl, int32_t reset);
 /* Save / restore the state (x, v and the warm start of the next step) in
  * device memory under a slot number, e.g. every k steps of a rollout. */
 int gc_zustand_sichern(gc_ctx* ctx, int32_t slot);
