@@ -255,6 +255,19 @@ struct GridState {
   double rest;
   double delta, ziel, bb;
   double delta_best;         // k_verf: FP64 residual at the FP64 solution dv64
+#ifdef GC_PHASEN
+  // build flag GC_PHASEN: block 0's wall-clock ticks per phase of a PCG
+  // iteration (stage A, stage B, stage C, all-reduce p.Ap, update,
+  // all-reduce r.z) and the number of iterations they cover
+  unsigned long long phasen[6];
+  unsigned long long phasen_it;
+  // the all-reduce's leader (block 0): ticks from its own partial to the
+  // published sums (its wait for the other blocks included), and the count
+  unsigned long long leiter[2];
+  // per block (rank r at 4 r): ticks of stages A-C, of the update, and of the
+  // waits in the two all-reduces (null: not recorded)
+  unsigned long long* pro_block;
+#endif
 };
 
 #define GC_MAX_BLOECKE 1024
@@ -263,61 +276,71 @@ struct alignas(64) Slot {
   unsigned epoche;  // the barrier whose partial v holds (the all-reduce's check)
 };
 
-// Arrival counters of the grid barrier, after the two partial buffers of a
-// system's slot array: one per group of GC_GRUPPE blocks, one for the group
-// leaders, and one release flag per group, each on a 128-byte line of its
-// own. A slot array holds GC_SLOTS entries.
+// A system's slot array (GC_SLOTS entries): the partials of the grid
+// all-reduce, one slot per block for each epoch parity, then one result line
+// per group of GC_GRUPPE blocks, each on a 128-byte line of its own.
 #define GC_GRUPPE 8
 #define GC_BARR_GRUPPEN (GC_MAX_BLOECKE / GC_GRUPPE)
 #define GC_BARR_BASIS (2 * GC_MAX_BLOECKE)
-#define GC_BARR_ZAEHLER (2 * GC_BARR_GRUPPEN + 1)
-#define GC_SLOTS (GC_BARR_BASIS + 2 * GC_BARR_ZAEHLER)
-__device__ __forceinline__ unsigned* barr_zaehler(Slot* slots, unsigned k) {
-  return (unsigned*)&slots[GC_BARR_BASIS + 2 * k];
+#define GC_SLOTS (GC_BARR_BASIS + 2 * GC_BARR_GRUPPEN)
+__device__ __forceinline__ unsigned long long* barr_ergebnis(Slot* slots, unsigned g) {
+  return (unsigned long long*)&slots[GC_BARR_BASIS + 2 * g];
 }
-// Clears the barrier counters (one block; a fill kernel's work, no memset).
+// Clears the partials and result lines (one block; a fill kernel's work, no
+// memset): an all-reduce reads a slot as this barrier's once it carries the
+// barrier's epoch, and the epoch starts anew at 1 after each clear.
 __device__ __forceinline__ void barriere_leeren(Slot* slots) {
-  for (unsigned k = threadIdx.x; k < GC_BARR_ZAEHLER; k += blockDim.x) *barr_zaehler(slots, k) = 0u;
+  for (unsigned k = threadIdx.x; k < 16 * GC_SLOTS; k += blockDim.x) ((unsigned*)slots)[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
-// Grid-wide all-reduce: block partials go into per-block slots, thread 0
-// releases and arrives on one counter that only grows within a launch (a
-// kernel before each launch sets it to 0), relaxed polling until it reaches
-// blocks x barriers so far, thread 0 acquires,
-// then every block sums all slots in the same fixed
-// order (so every block holds the identical value — deterministic and equal to
-// the bit across blocks). Slots are double-buffered by epoch parity: a block
-// can only overwrite parity p after every block has published epoch e+1,
-// i.e. after everyone has read epoch e-1.
-// ---------------------------------------------------------------------------
+// Grid-wide all-reduce. Each block writes its partial into its slot; block 0
+// (the leader) waits for all nb partials, sums them in one fixed order and
+// publishes the sums, so every block holds the identical value
+// (deterministic, equal to the bit across blocks and across a batch's
+// systems of the same launch shape). Slots are double-buffered by epoch
+// parity.
+//
 // Block sums are deterministic: an xor butterfly inside each wave (every lane
 // ends with the same value, IEEE addition being commutative), then the wave
-// sums in wave order.
-// rang, nb: this block's rank among the nb blocks that share gs and slots (the
-// whole grid for one system, a group of blocks in a batched launch).
+// sums in wave order. rang, nb: this block's rank among the nb blocks that
+// share gs and slots (the whole grid for one system, a group of blocks in a
+// batched launch). Wave 0 alone arrives, waits and hands the result to the
+// block through LDS (two workgroup barriers per grid barrier).
 //
-// The block's partial is its waves' sums in wave order; wave 0 alone then
-// arrives, waits, sums the nb partials and hands the result to the block
-// through LDS (two workgroup barriers per grid barrier).
+// Every value travels as two 64-bit words, (epoch << 32) | low half and
+// (epoch << 32) | high half. A 64-bit access is single-copy atomic, so a
+// reader that finds this barrier's epoch in every word of a slot holds this
+// barrier's value, with no counter or flag to order against it: a block's
+// arrival IS its partial's store, and the leader's publication IS the
+// result. The critical path is the last partial's store, the leader's poll
+// that sees it, the leader's stores of the result and the polls that see
+// them (where a counter tree adds two atomic round trips and every block
+// reading all partials contends on their few lines). The leader writes the
+// result into one line per group of GC_GRUPPE consecutive ranks, which that
+// group's blocks poll. A slot holding a later epoch than the barrier's is
+// stale (left from another launch): bit 31 of the result's epoch half then
+// marks the all-reduce failed. Slots and result lines are cleared before each
+// launch that starts the epoch anew.
 //
-// Arrival is a two-level tree: each block writes back its stores (release,
-// the L2 write-back to the coherent level) and adds one to its group's
-// counter (groups of GC_GRUPPE consecutive ranks); the last arrival of a group
-// (seen from the counter's old value) adds one to the leaders' counter; the
-// last leader publishes the barrier's epoch in every group's release flag;
-// a block polls its group's flag. No counter or flag sees more than GC_GRUPPE
-// (or nb / GC_GRUPPE) blocks, where a single counter serialised all nb
-// arrivals and a single flag all nb polls. The counters
-// grow monotonically within a launch and are cleared before each. There is no
-// acquire (no L2 invalidation): everything a block reads that another block
-// wrote in this launch (partials, counters, dv / z / p of points it does not
-// own) is read with agent-scope loads, which the program order after the
-// observed flag puts after every writer's write-back.
+// Release: each block writes back its stores (the L2 write-back to the
+// coherent level) before its partial. There is no acquire (no L2
+// invalidation): everything a block reads that another block wrote in this
+// launch (partials, results, dv / z / p of points it does not own) is read
+// with agent-scope loads, which the program order after the observed result
+// puts after every writer's write-back.
 // VOLL = false: the caller's stores before this all-reduce are not read by
-// other blocks until after the next full one, so only the block's partial
-// (an agent-coherent store) has to be complete before the arrival; the L2
-// write-back of the release is left to that next all-reduce.
+// other blocks until after the next full one, so the block's partial alone
+// is published; the L2 write-back of the release is left to that next
+// all-reduce.
+__device__ __forceinline__ unsigned long long* barr_teil(Slot* buf, unsigned r) {
+  return (unsigned long long*)&buf[r];
+}
+__device__ __forceinline__ unsigned long long barr_wort(double x, int haelfte, unsigned long long marke) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return marke | (haelfte ? (b >> 32) : (b & 0xffffffffull));
+}
+
 template <int NV, bool VOLL = true>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
                                                unsigned rang, unsigned nb) {
@@ -338,73 +361,115 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
   __shared__ int s_fehler;
   if (wave == 0) {
     int fehler = 0;
-    if (lane == 0) {
-      for (int k = 0; k < NV; ++k) {
-        double a = 0.0;
-        for (int w = 0; w < nw; ++w) a += sh[w * NV + k];
-        __hip_atomic_store(&buf[rang].v[k], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      // check: the barrier's epoch beside the partial; a partial left from an
-      // earlier barrier or launch (stale counters) fails the all-reduce
-      __hip_atomic_store(&buf[rang].epoche, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE, g = rang / GC_GRUPPE;
-      const unsigned gsz = nb - g * GC_GRUPPE < GC_GRUPPE ? nb - g * GC_GRUPPE : GC_GRUPPE;
-      unsigned* flagge = barr_zaehler(slots, GC_BARR_GRUPPEN + 1 + g);
-      if constexpr (VOLL) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      else __builtin_amdgcn_s_waitcnt(0x0F70);  // s_waitcnt vmcnt(0): the partial's store is complete
-      const unsigned alt = __hip_atomic_fetch_add(barr_zaehler(slots, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (alt == gsz * epoch - 1u) {  // last of its group
-        const unsigned alt2 =
-            __hip_atomic_fetch_add(barr_zaehler(slots, GC_BARR_GRUPPEN), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (alt2 == ng * epoch - 1u)  // last group: everyone has arrived; wake every group
-          for (unsigned q = 0; q < ng; ++q)
-            __hip_atomic_store(barr_zaehler(slots, GC_BARR_GRUPPEN + 1 + q), epoch, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-      }
-      unsigned spins = 0;
-      while (__hip_atomic_load(flagge, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-        if (++spins > (1u << 22)) { fehler = 1; break; }
-        if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { fehler = 1; break; }
-      }
-      if (fehler) __hip_atomic_store(&gs->fehler, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_max(&gs->spins_max, spins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    fehler = __shfl(fehler, 0, 64);
-    // the nb partials, summed in the same order by every block: lane l takes
-    // l, l + 64, ..., then the wave's butterfly
+    const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE;
+    const unsigned long long marke = (unsigned long long)epoch << 32;
+    // the block's partial: word 2k + h holds half h of value k (the waves'
+    // sums in wave order); lane l stores word l mod 2 NV, so that no branch
+    // splits the wave here (a release fence inside a divergent branch is
+    // duplicated into both sides, and the L2 write-back runs twice)
+    const int kw = lane % (2 * NV);
+    double a = 0.0;
+    for (int w = 0; w < nw; ++w) a += sh[w * NV + (kw >> 1)];
+    if constexpr (VOLL) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+#ifdef GC_PHASEN
+    const unsigned long long t_an = wall_clock64();
+#endif
+    __hip_atomic_store(barr_teil(buf, rang) + kw, barr_wort(a, kw & 1, marke), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
     double acc[NV];
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    bool gleich = true;  // every partial read is of this barrier
-    if (!fehler)
-      for (unsigned b0 = lane; b0 < nb; b0 += 256) {
-        // four partials per lane are loaded before their sums, which keep the
-        // order b0, b0 + 64, ...
-        double x[4][NV];
-        unsigned ep[4];
+    unsigned spins = 0;
+    if (rang == 0) {
+      // the leader: the nb partials, summed in the fixed order (lane l takes
+      // l, l + 64, ..., four slots per round, then the wave's butterfly);
+      // each lane waits for its own slots
+      bool spaet = false;  // a slot of a later barrier: stale
+      for (unsigned b0 = lane; b0 < nb && !fehler; b0 += 256) {
+        unsigned long long x[4][2 * NV];
+        for (;;) {
+          bool da = true;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const bool da = b0 + 64u * u < nb;
+          for (int u = 0; u < 4; ++u) {
+            const bool gibt = b0 + 64u * u < nb;
 #pragma unroll
-          for (int k = 0; k < NV; ++k)
-            x[u][k] = da ? __hip_atomic_load(&buf[b0 + 64u * u].v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-          ep[u] = da ? __hip_atomic_load(&buf[b0 + 64u * u].epoche, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : epoch;
+            for (int k = 0; k < 2 * NV; ++k) {
+              x[u][k] = gibt ? __hip_atomic_load(barr_teil(buf, b0 + 64u * u) + k, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : marke;
+              const unsigned e = (unsigned)(x[u][k] >> 32);
+              da = da && e == epoch;
+              spaet = spaet || (e > epoch && e != 0u);
+            }
+          }
+          if (da || spaet) break;
+          if (++spins > (1u << 22)) { fehler = 1; break; }
+          if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            fehler = 1;
+            break;
+          }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          gleich = gleich && ep[u] == epoch;
+        for (int u = 0; u < 4; ++u)
           if (b0 + 64u * u < nb)
 #pragma unroll
-            for (int k = 0; k < NV; ++k) acc[k] += x[u][k];
+            for (int k = 0; k < NV; ++k)
+              acc[k] += __longlong_as_double((long long)((x[u][2 * k + 1] << 32) | (x[u][2 * k] & 0xffffffffull)));
+      }
+      if (__ballot(fehler == 1) != 0ull) fehler = 1;
+      else if (__ballot(spaet) != 0ull) fehler = 2;
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
+      // publish: lane q < ng writes group q's line (every word carries the
+      // epoch, so the words' order does not matter); a failed all-reduce
+      // publishes too, with bit 31 set, so that no block waits for it
+      const unsigned long long m = ((unsigned long long)(epoch | (fehler ? 0x80000000u : 0u))) << 32;
+      for (unsigned q = lane; q < ng; q += 64) {
+        unsigned long long* z = barr_ergebnis(slots, q);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          __hip_atomic_store(z + 2 * k, barr_wort(acc[k], 0, m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(z + 2 * k + 1, barr_wort(acc[k], 1, m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
+    } else {
+      // lane l reads word l mod 2 NV of the group's line until every word
+      // carries this barrier's epoch
+      const unsigned long long* z = barr_ergebnis(slots, rang / GC_GRUPPE);
+      unsigned long long w = 0;
+      for (;;) {
+        w = __hip_atomic_load(z + kw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool da = ((unsigned)(w >> 32) & 0x7fffffffu) == epoch;
+        if (__ballot(!da) == 0ull) break;
+        if (++spins > (1u << 22)) { fehler = 1; break; }
+        if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          fehler = 1;
+          break;
+        }
+      }
+      // the leader failed (and says why in gs)
+      if (!fehler && __ballot((w >> 63) != 0ull) != 0ull) fehler = 3;
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
+      for (int k = 0; k < NV; ++k) {
+        const unsigned long long lo = __shfl(w & 0xffffffffull, 2 * k, 64);
+        const unsigned long long hi = __shfl(w & 0xffffffffull, 2 * k + 1, 64);
+        acc[k] = __longlong_as_double((long long)((hi << 32) | lo));
+      }
+    }
+#ifdef GC_PHASEN
+    if (lane == 0 && rang == 0) {
+      gs->leiter[0] += wall_clock64() - t_an;
+      gs->leiter[1] += 1ull;
+    }
+#endif
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-    if (!fehler && __ballot(!gleich) != 0ull) fehler = 2;
+    for (int o = 32; o > 0; o >>= 1) spins = max(spins, (unsigned)__shfl_xor((int)spins, o, 64));
     if (lane == 0) {
       for (int k = 0; k < NV; ++k) aus[k] = acc[k];
-      if (fehler == 2) __hip_atomic_store(&gs->fehler, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (fehler == 1 || fehler == 2)
+        __hip_atomic_store(&gs->fehler, (unsigned)fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_max(&gs->spins_max, spins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_fehler = fehler;
     }
   }
@@ -1421,6 +1486,15 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     int it_min = it;
     bool neu_starten = false;
     while (delta > ziel && it < s.cg_max) {
+#ifdef GC_PHASEN
+      const bool uhr = threadIdx.x == 0;
+      unsigned long long tp[7];
+      if (uhr) tp[0] = wall_clock64();
+#define GC_PHASE(k) \
+  if (uhr) tp[k] = wall_clock64();
+#else
+#define GC_PHASE(k)
+#endif
       GC_GM V3<T>* pn = gm(s.p[cur]);
       const GC_GM V3<T>* po = gm(s.p[cur ^ 1]);
       const T tb = T(beta);
@@ -1443,9 +1517,11 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
           for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) ul[v - v0] = richtung<true>(s, gm(s.kachel_v)[v], po, tb, erste);
         }
         __syncthreads();
+        GC_PHASE(1)
         kachel_beitraege(s, kk, slot, ul);
         if (s.selbst) feder_terme(s, lo, hi, [&](uint32_t j) { return richtung<true>(s, j, po, tb, erste); }, fterm);
         __syncthreads();
+        GC_PHASE(2)
         if (zeile_reg) {
           if (hat_zeile) {
             V3<T> q = vor_z;
@@ -1472,9 +1548,11 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         }
         __syncthreads();
       }
+      GC_PHASE(3)
       // p and A p are read by other blocks only in the next operator pass,
       // after the r.z all-reduce, whose release writes them back
       if (!grid_allreduce<1, false>(ca, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+      GC_PHASE(4)
       if (ca[0] <= 2.2250738585072014e-308 || !isfinite(ca[0])) break;
       const double alpha = delta / ca[0];
       // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
@@ -1497,7 +1575,25 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
           neu[0] += (double)dot(ri, zi);
         }
       }
+      GC_PHASE(5)
       if (!grid_allreduce<1>(neu, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
+      GC_PHASE(6)
+#ifdef GC_PHASEN
+      if (uhr && s.n_kacheln <= nb) {  // one tile per block: the phases are one pass each
+        if (rang == 0) {
+          for (int k = 0; k < 6; ++k) gs->phasen[k] += tp[k + 1] - tp[k];
+          gs->phasen_it += 1;
+        }
+        unsigned long long* pb = gs->pro_block;
+        if (pb) {
+          pb[4 * rang] += tp[3] - tp[0];
+          pb[4 * rang + 1] += tp[5] - tp[4];
+          pb[4 * rang + 2] += tp[4] - tp[3];
+          pb[4 * rang + 3] += tp[6] - tp[5];
+        }
+      }
+#endif
+#undef GC_PHASE
       beta = neu[0] / delta;
       delta = neu[0];
       cur ^= 1;

@@ -1111,8 +1111,8 @@ static int anfang_zurueck(gc_ctx* c) {
 // The message of a grid all-reduce's error flag (GridState::fehler).
 static std::string barriere_meldung(unsigned code) {
   if (code == 2)
-    return "grid all-reduce summed partials of another barrier (stale barrier counters); the solve's result is "
-           "not valid";
+    return "grid all-reduce found a partial of a later barrier (slots left from another launch); the solve's "
+           "result is not valid";
   return "grid barrier wait exceeded its spin limit (blocks not co-resident?)";
 }
 
@@ -1820,7 +1820,11 @@ static int loese(gc_ctx* c, double* dv, gc_cg_lauf* lauf) {
 // array's name. Every call on a device shares that device's copy, so calls
 // that overlap on one device race.
 // ---------------------------------------------------------------------------
+#ifdef GC_PHASEN
+#define GC_OS_SLOTS 512
+#else
 #define GC_OS_SLOTS 256
+#endif
 #define GC_OS_BYTES (GC_OS_SLOTS + GC_SLOTS * sizeof(Slot))
 static_assert(sizeof(GridState) <= GC_OS_SLOTS, "the grid state fits before the slots");
 static __device__ __align__(256) unsigned char scratch[GC_OS_BYTES];
@@ -3004,6 +3008,61 @@ int gc_entscheidungen(gc_ctx* c, int32_t modus, int32_t slot) {
   }
   return 0;
 }
+
+#ifdef GC_PHASEN
+// Build flag GC_PHASEN only (a diagnostic, not part of the C ABI's header):
+// block 0's mean microseconds per PCG iteration in stage A, stage B, stage C,
+// the p.Ap all-reduce, the update and the r.z all-reduce, over the
+// iterations since the last reset; returns the iteration count.
+// us7: the six phases of a PCG iteration seen by block 0 (stage A, stage B,
+// stage C, all-reduce p.Ap, update, all-reduce r.z), then the leader's time
+// from its partial to the published sums per all-reduce, in microseconds;
+// returns the iterations they cover
+extern "C" int gc_phasen_lesen(gc_ctx* c, double* us7, int32_t reset) {
+  if (!c || !us7) return -1;
+  double* us6 = us7;
+  GridState g;
+  if (hipMemcpy(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->opt.geraet) != hipSuccess || khz <= 0) return -1;
+  const double it = (double)std::max<unsigned long long>(g.phasen_it, 1ull);
+  for (int k = 0; k < 6; ++k) us6[k] = (double)g.phasen[k] / ((double)khz * 1e-3) / it;
+  us7[6] = (double)g.leiter[0] / ((double)khz * 1e-3) / (double)std::max<unsigned long long>(g.leiter[1], 1ull);
+  if (reset) {
+    for (int k = 0; k < 6; ++k) g.phasen[k] = 0;
+    for (int k = 0; k < 2; ++k) g.leiter[k] = 0;
+    const unsigned long long n = g.phasen_it;
+    g.phasen_it = 0;
+    if (hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    return (int)n;
+  }
+  return (int)g.phasen_it;
+}
+#endif
+
+#ifdef GC_PHASEN
+// out (4 x n, microseconds summed over the recorded iterations): per block,
+// stages A-C, update, wait in the p.Ap all-reduce, wait in the r.z one. The
+// first call (out null) turns the record on.
+extern "C" int gc_phasen_bloecke(gc_ctx* c, double* out, int32_t n, int32_t reset) {
+  if (!c) return -1;
+  GridState g;
+  if (hipMemcpy(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  if (!g.pro_block) {
+    if (hipMalloc((void**)&g.pro_block, 4 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
+    if (hipMemset(g.pro_block, 0, 4 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
+    if (hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice) != hipSuccess) return -1;
+  }
+  if (!out) return 0;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->opt.geraet) != hipSuccess || khz <= 0) return -1;
+  std::vector<unsigned long long> h(4 * GC_MAX_BLOECKE);
+  if (hipMemcpy(h.data(), g.pro_block, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  for (int k = 0; k < 4 * std::min<int>(n, GC_MAX_BLOECKE); ++k) out[k] = (double)h[k] / ((double)khz * 1e-3);
+  if (reset && hipMemset(g.pro_block, 0, 4 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
+  return 0;
+}
+#endif
 
 int gc_entscheidungen_abweichungen(gc_ctx* c, int64_t* anzahl, int32_t reset) {
   if (!c || !anzahl) return 1;
