@@ -25,28 +25,42 @@ WURZEL = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, WURZEL)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+TRAFFIC_JE_ITERATION = 4988614
+TRAFFIC_EINHEIT = "B per PCG iteration (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB x 1024; steps 241-250, 136 x 1024)"
 
 
 def pcg_bytes(s, praezision: str) -> dict:
-    """Algorithmic bytes of one PCG iteration and of the PCG start, from the
-    record layout the kernels stream (each record counted once per pass)."""
+    """Algorithmic bytes of one PCG iteration: every record and vector an
+    iteration must touch, counted once (t = bytes per float):
+    membrane 12 + 21t per triangle (3 indices, f1 f2, w, D^-1, basis,
+    stiffness, area), bending 16 + 13t per hinge (4 indices, 4 gradients, k),
+    seam or mesh edge with k != 0 8 + 5t, point 41t (p in, A p out, mass:
+    7t; the fused vector pass 34t: p, A p, r, dv in, P^-1, filter, dv, r, p
+    out). For t = 4 these are 96, 68, 28 and 164 B. The CSR index stream that
+    the operator also reads is not counted."""
     t = 4 if praezision == "fp32" else 8
-    v3 = 3 * t
     n = s.n
     nm = s.membran_punkte.shape[0]
     nb = s.biegung_punkte.shape[0]
     nk = int(np.count_nonzero(~s.membran_kante)) + s.naht_punkte.shape[0]
-    eintraege = 2 * nk + 4 * nb + 3 * nm
-    membran = 12 + 12 * t + 9 * t            # 3 indices, element, tangent (f1, f2, w)
-    biegung = 16 + 12 * t + t                # 4 indices, 4 gradients, k
-    kante = 8 + 3 * t + 2 * t                # 2 indices, direction, cross factor, k
-    csr = 4 * eintraege + 16 * n              # entries + 4 offsets per point
-    matvec = nm * membran + nb * biegung + nk * kante + csr + n * (v3 + t + 1 + v3 + v3)  # p, mdiag, flag, normal, ap out
-    richtung = n * (v3 + v3 + 1 + v3)         # z, p_old, flag in; p out (normal counted in matvec)
-    update = n * (6 * v3 + 9 * t)             # dv, p, r, ap, pinv in; dv, r, z out
-    start = matvec + n * (v3 + 9 * t + 2 * v3)  # b, pinv; r, z out
-    return {"iteration": matvec + richtung + update, "matvec": matvec, "start": start,
-            "membranen": nm, "biegungen": nb, "kanten": nk, "csr_eintraege": eintraege}
+    membran, biegung, kante, punkt = 12 + 21 * t, 16 + 13 * t, 8 + 5 * t, 41 * t
+    matvec = nm * membran + nb * biegung + nk * kante + n * 7 * t
+    iteration = matvec + n * 34 * t
+    return {"iteration": iteration, "matvec": matvec, "start": iteration,
+            "membranen": nm, "biegungen": nb, "kanten": nk,
+            "je_einheit": {"membrane_B": membran, "bending_B": biegung, "edge_B": kante, "point_B": punkt}}
+
+
+def cpu_modell() -> str:
+    """The host CPU's model name (/proc/cpuinfo), else what platform knows."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for zeile in f:
+                if zeile.lower().startswith("model name"):
+                    return zeile.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def main() -> int:
@@ -101,6 +115,8 @@ def main() -> int:
     step = g.schritte_graph if args.graph else g.schritte
     for k in range(args.warmup):
         step(dt, 1)
+        if (k + 1) % 50 == 0:
+            print(f"[bench] warmup step {k + 1}", file=sys.stderr, flush=True)
     g.statistik(reset=True)
     x_w, v_w = g.lage()
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
@@ -131,14 +147,16 @@ def main() -> int:
         achieved = algo / (ms_pcg * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_pcg", "bytes_per_iteration": b["iteration"],
+                    "kernel": "k_pcg", "bytes_per_iteration": b["iteration"], "bytes_per_unit": b["je_einheit"],
                     "us_per_iteration": round(ms_pcg * 1e3 / max(iters, 1), 3)}
         if args.praezision == "fp32":
-            # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-            # drape, scaled by the counters of a plain copy of known size
+            # a fixed reference measurement, not of this run: rocprofv3 --pmc
+            # FETCH_SIZE and WRITE_SIZE over the k_pcg dispatches of steps
+            # 241-250 of this drape at the default launch shape, scaled by the
+            # counters of a plain copy of known size
             # (profiles/k_pcg_drape40k_pmc.md): bytes per PCG iteration
-            roofline["traffic"] = 5137583
-            roofline["traffic_unit"] = "B per PCG iteration (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB x 1024)"
+            roofline["traffic"] = TRAFFIC_JE_ITERATION
+            roofline["traffic_unit"] = TRAFFIC_EINHEIT
 
     cpu = None
     if not args.no_cpu:
@@ -161,7 +179,7 @@ def main() -> int:
         cpu = {"value": round(n_cpu / (t_d - t_c), 4), "unit": "steps/s", "cores": 1, "kind": "port",
                "sample": f"{n_cpu} steps of the same drape from the state after {args.warmup} warm-up steps, "
                          f"FP64 C++ restatement of CpuLoeser/ImplizitCg::schritt, {st_o['cg_iterationen']} CG iterations",
-               "cpu": platform.processor() or platform.machine()}
+               "cpu": cpu_modell(), "build": "g++ -O3 (oracle/Makefile)"}
 
     out = {
         "metric": "implicit steps/s @40k verts, PCG rtol 1e-6; batched/adjoint vs plain GPU vs CPU",
@@ -179,6 +197,7 @@ def main() -> int:
         "config": {"workload": "drape: 28 woven panels, 24 seams, ellipsoid-cylinder grid SDF 256^3 at 3 mm, "
                                "friction 0.3, " + ("self-collision on, " if selbst else "self-collision off, ") +
                                "dt 1/120, rtol 1e-6",
+                   "timed_steps": f"{args.warmup + 1}-{args.warmup + args.steps}",
                    "punkte": s.n, "dreiecke": int(s.dreiecke.shape[0]), "biegungen": b["biegungen"],
                    "selbstkollision": selbst, "graph": bool(args.graph), "praezision": args.praezision,
                    "cg_iterationen_je_schritt": round(iters / args.steps, 1), "loesungen": loesungen,
