@@ -51,6 +51,29 @@ class Param(C.Structure):
     ]
 
 
+class NahtEingabe_c(C.Structure):
+    _fields_ = [
+        ("n_teile", C.c_int32), ("teil_kanten", I32), ("kante_punkte", I32), ("punkte", D),
+        ("n_naehte", C.c_int32), ("seite_stuecke", I32), ("stueck_teil", I32), ("stueck_rueck", I32),
+        ("stueck_kanten", I32), ("kanten", I32),
+        ("kante_mm", C.c_double), ("sehne_mm", C.c_double), ("laenge_mm", C.c_double), ("fang_mm", C.c_double),
+        ("min_rand_mm", C.c_double),
+    ]
+
+
+class NahtAusgabe_c(C.Structure):
+    _fields_ = [
+        ("kap_rand", C.c_int64), ("n_rand", C.c_int64), ("rand_teil", I32), ("rand_kante", I32), ("rand_s", D),
+        ("rand_xy", D), ("rand_knoten", I32),
+        ("kap_stellen", C.c_int64), ("n_stellen", C.c_int64), ("naht_stellen", I32), ("stelle_art", I32),
+        ("stelle_knoten", I32), ("stelle_rho", D), ("stelle_versatz_mm", D),
+        ("kap_teilnaehte", C.c_int64), ("n_teilnaehte", C.c_int64), ("naht_teilnaehte", I32),
+        ("teilnaht_teile", I32), ("teilnaht_paare", I32),
+        ("kap_paare", C.c_int64), ("n_paare", C.c_int64), ("paar_punkte", I32), ("paar_rho", D),
+        ("naht_laengen_mm", D), ("fehler", C.c_char * 256),
+    ]
+
+
 class CgLauf_c(C.Structure):
     _fields_ = [("iterationen", C.c_int32), ("nicht_konvergiert", C.c_int32), ("rest", C.c_double)]
 
@@ -128,6 +151,7 @@ SIGNATUREN = [
     ("gc_zustand_laden", C.c_int, [C.c_void_p, C.c_int32]),
     ("gc_entscheidungen", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("gc_entscheidungen_abweichungen", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]),
+    ("gc_naht_planen", C.c_int, [C.POINTER(NahtEingabe_c), C.POINTER(NahtAusgabe_c)]),
 ]
 
 _lib = None

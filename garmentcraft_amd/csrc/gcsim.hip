@@ -8,6 +8,7 @@
 #include "../../include/gcsim.h"
 #include "gc_adjoint.h"
 #include "gc_selbst.h"
+#include "gc_naht.h"
 
 #include <hip/hip_runtime.h>
 

@@ -346,6 +346,65 @@ int gc_entscheidungen_abweichungen(gc_ctx* ctx, int64_t* anzahl, int32_t reset);
 int gc_zustand_sichern(gc_ctx* ctx, int32_t slot);
 int gc_zustand_laden(gc_ctx* ctx, int32_t slot);
 
+/* ------------------------------------------------------------------------
+ * Seam pairing by length fraction (klon3d-mesh rand.rs:297-501, Randplan
+ * rand.rs:503-589, pairs emitted as in lib.rs:351-381). Host code: no device
+ * is touched. Input: pattern pieces whose boundary edges are polylines in
+ * mm, and seams whose two sides are runs of whole edges (Nahtseite). Output:
+ * every piece's boundary points (the first points of its mesh, in key order
+ * of edge and arc length, rand.rs:536-583), every seam's stations
+ * (Nahtstelle: end, shared, only A, only B) and its sub-seams (Teilnaht) with
+ * the stitched point pairs (Punktpaar). Boundary splits the mesher's
+ * refinement asks for (Randplan::teile_auf, rand.rs:591-604) are not taken.
+ * ---------------------------------------------------------------------- */
+typedef struct gc_naht_eingabe {
+  int32_t n_teile;
+  const int32_t* teil_kanten;   /* n_teile + 1: edges of piece t are teil_kanten[t] .. teil_kanten[t+1] - 1 */
+  const int32_t* kante_punkte;  /* n_kanten + 1: polyline points of edge e */
+  const double* punkte;         /* x, y (mm) per polyline point */
+  int32_t n_naehte;
+  const int32_t* seite_stuecke; /* 2 n_naehte + 1: side A of seam n is 2n, side B 2n + 1; its runs */
+  const int32_t* stueck_teil;   /* per run: the piece */
+  const int32_t* stueck_rueck;  /* per run: 1 = the seam runs the edges backwards */
+  const int32_t* stueck_kanten; /* runs + 1: the run's edges (indices within the piece) */
+  const int32_t* kanten;
+  double kante_mm;              /* target edge length (10) */
+  double sehne_mm;              /* largest chord height (0.1) */
+  double laenge_mm;             /* largest length loss per boundary edge (0.1) */
+  double fang_mm;               /* stations at most this far apart on A and B are merged (2) */
+  double min_rand_mm;           /* shortest boundary piece a split may make (0.5) */
+} gc_naht_eingabe;
+
+typedef struct gc_naht_ausgabe {
+  /* capacities in (entries), counts out */
+  int64_t kap_rand, n_rand;
+  int32_t* rand_teil;           /* n_teile + 1: boundary points of piece t (offsets) */
+  int32_t* rand_kante;          /* per boundary point: its edge */
+  double* rand_s;               /* per boundary point: arc length from the edge's start (mm) */
+  double* rand_xy;              /* per boundary point: x, y (mm) */
+  int32_t* rand_knoten;         /* per edge (n_kanten): the boundary point of the node it starts at */
+  int64_t kap_stellen, n_stellen;
+  int32_t* naht_stellen;        /* n_naehte + 1 */
+  int32_t* stelle_art;          /* 0 end, 1 shared, 2 only A, 3 only B (Stellenart) */
+  int32_t* stelle_knoten;       /* piece and edge whose start node the station is on A, then on B; -1 -1 for a new point */
+  double* stelle_rho;           /* length fraction on A and B */
+  double* stelle_versatz_mm;    /* (rho_A - rho_B) x mean side length */
+  int64_t kap_teilnaehte, n_teilnaehte;
+  int32_t* naht_teilnaehte;     /* n_naehte + 1 */
+  int32_t* teilnaht_teile;      /* piece on A, piece on B */
+  int32_t* teilnaht_paare;      /* n_teilnaehte + 1 */
+  int64_t kap_paare, n_paare;
+  int32_t* paar_punkte;         /* boundary point on A, on B (indices within the piece's points) */
+  double* paar_rho;             /* length fraction on A, on B */
+  double* naht_laengen_mm;      /* 2 per seam: length of side A, of side B (mm) */
+  char fehler[256];             /* the reason of a failed plan */
+} gc_naht_ausgabe;
+/* 0: planned; 1: the pattern cannot be planned (reason in fehler, as the
+ * reference's Err: an edge in two seams, stations that coincide, a section
+ * across an edge boundary); 2: a capacity is too small (all counts are set,
+ * nothing else written); -1: bad arguments. */
+int gc_naht_planen(const gc_naht_eingabe* ein, gc_naht_ausgabe* aus);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,9 +43,10 @@ def test_keine_gpu_meldet_sich_laut():
 
 # header struct -> its ctypes mirror in garmentcraft_amd/_lib.py
 STRUKTUREN = {"gc_optionen": "Optionen", "gc_topologie": "Topologie", "gc_param": "Param", "gc_cg_lauf": "CgLauf_c",
-              "gc_statistik": "Statistik_c", "gc_adjoint_stat": "AdjointStat_c"}
+              "gc_statistik": "Statistik_c", "gc_adjoint_stat": "AdjointStat_c", "gc_naht_eingabe": "NahtEingabe_c",
+              "gc_naht_ausgabe": "NahtAusgabe_c"}
 C_TYPEN = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "double": ctypes.c_double, "int64_t": ctypes.c_int64,
-           "uint64_t": ctypes.c_uint64}
+           "uint64_t": ctypes.c_uint64, "char": ctypes.c_char}
 
 
 def strukturen():
@@ -59,11 +60,11 @@ def strukturen():
             decl = " ".join(decl.split())
             if not decl:
                 continue
-            m = re.match(r"(?:const )?(\w+)\s*(\*?)\s*(\w+)(?:\[(\d+)\])?$", decl)
+            m = re.match(r"(?:const )?(\w+)\s*(\*?)\s*(\w+(?:\s*,\s*\w+)*)(?:\[(\d+)\])?$", decl)
             assert m, (name, decl)
             typ, zeiger, feld, laenge = m.groups()
             for f in feld.split(","):
-                felder.append((f, typ, bool(zeiger), int(laenge) if laenge else 0))
+                felder.append((f.strip(), typ, bool(zeiger), int(laenge) if laenge else 0))
         aus[name] = felder
     return aus
 
