@@ -88,6 +88,30 @@ class GitterSdf:
 
 
 @dataclass
+class Netz:
+    """A closed, consistently oriented triangle mesh as body (koerper.rs:25,
+    netzkoerper.rs:145-177), rigid, moved by ``versatz``.
+
+    ``punkte`` (n, 3) in m, ``dreiecke`` (m, 3) with outward normals;
+    ``suchtiefe`` is how deep inside a cloth point is still found
+    (netzkoerper.rs:44-46, default 10 mm). ``punktgeschwindigkeit`` (n, 3) is
+    the velocity field of a deforming body (netzkoerper.rs:454-489), or None
+    for one velocity of the whole body.
+    """
+    punkte: np.ndarray
+    dreiecke: np.ndarray
+    suchtiefe: float = 0.01
+    versatz: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    geschwindigkeit: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    punktgeschwindigkeit: Optional[np.ndarray] = None
+
+    def verschoben(self, t: float) -> "Netz":
+        """koerper.rs:122-133."""
+        v = tuple(self.versatz[k] + self.geschwindigkeit[k] * t for k in range(3))
+        return Netz(self.punkte, self.dreiecke, self.suchtiefe, v, self.geschwindigkeit, self.punktgeschwindigkeit)
+
+
+@dataclass
 class Szene:
     """szene.rs:197-269 as numpy arrays. Constraint groups are colour-ordered."""
     x: np.ndarray                      # (n, 3) f64

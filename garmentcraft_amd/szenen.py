@@ -9,6 +9,8 @@ follow the reference's own test scenes.
 * :func:`drape` — random convex-ish panels sewn into strips and dropped onto an
   ellipsoid-cylinder body given as a grid SDF.
 * :func:`koerper_sdf` — that body's signed distance sampled on a regular grid.
+* :func:`netz_zylinder`, :func:`netz_ellipsoid`, :func:`koerper_netz` — closed
+  triangle meshes as bodies (the reference's mesh body, netzkoerper.rs).
 
 Meshes are built here (structured grids, or a jittered hexagonal lattice
 triangulated by Delaunay); the reference meshes its patterns with
@@ -23,7 +25,7 @@ from typing import List, Optional, Tuple
 import numpy as np
 
 from .material import RADICAL_RUBBER_04, Material, gewebe, latex
-from .szene import GitterSdf, Simparameter, Szene, Teil, Teilnaht, Zylinder, bauen
+from .szene import GitterSdf, Netz, Simparameter, Szene, Teil, Teilnaht, Zylinder, bauen
 
 # ---------------------------------------------------------------------------
 # hanging tube
@@ -381,6 +383,86 @@ def drape_material() -> Material:
     (2500/1500/250 N/m), 0.2 kg/m^2, bending warp/weft/bias 1.2/0.8/0.9 uN*m,
     friction 0.3."""
     return gewebe(5.0, 3.0, 0.5, 0.5e-3, 0.20, 1.2e-6, 0.8e-6, 0.9e-6, 0.3)
+
+
+def netz_zylinder(radius: float, z_unten: float, z_oben: float, segmente: int, ringe: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Closed cylinder (axis z) as a triangle mesh with outward normals:
+    ``segmente`` points per ring, ``ringe`` sections between ``z_unten`` and
+    ``z_oben``, caps as fans about a centre point (netzkoerper.rs:934-964; the
+    same points and triangles in the same order)."""
+    s, r = max(segmente, 3), max(ringe, 1)
+    p = []
+    for i in range(r + 1):
+        z = z_unten + (z_oben - z_unten) * i / r
+        for j in range(s):
+            w = 2.0 * math.pi * j / s
+            p.append((radius * math.cos(w), radius * math.sin(w), z))
+    unten = len(p)
+    p.append((0.0, 0.0, z_unten))
+    oben = len(p)
+    p.append((0.0, 0.0, z_oben))
+
+    def i_(i, j):
+        return i * s + j % s
+
+    d = []
+    for i in range(r):
+        for j in range(s):
+            a, b, c, e = i_(i, j), i_(i, j + 1), i_(i + 1, j + 1), i_(i + 1, j)
+            d.append((a, b, c))
+            d.append((a, c, e))
+    for j in range(s):
+        d.append((unten, i_(0, j + 1), i_(0, j)))
+        d.append((oben, i_(r, j), i_(r, j + 1)))
+    return np.array(p, dtype=np.float64), np.array(d, dtype=np.uint32)
+
+
+def netz_ellipsoid(c, r, segmente: int, ringe: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Axis-aligned ellipsoid (centre ``c``, semi-axes ``r``) as a closed
+    latitude-longitude mesh with outward normals: ``ringe`` - 1 rings of
+    ``segmente`` points plus the two poles."""
+    s, n = max(segmente, 3), max(ringe, 2)
+    p = [(c[0], c[1], c[2] - r[2])]
+    for i in range(1, n):
+        th = math.pi * i / n - 0.5 * math.pi
+        for j in range(s):
+            w = 2.0 * math.pi * j / s
+            p.append((c[0] + r[0] * math.cos(th) * math.cos(w), c[1] + r[1] * math.cos(th) * math.sin(w),
+                      c[2] + r[2] * math.sin(th)))
+    p.append((c[0], c[1], c[2] + r[2]))
+    sued, nord = 0, len(p) - 1
+
+    def i_(i, j):  # ring i in 1..n-1
+        return 1 + (i - 1) * s + j % s
+
+    d = []
+    for j in range(s):
+        d.append((sued, i_(1, j + 1), i_(1, j)))
+    for i in range(1, n - 1):
+        for j in range(s):
+            a, b, cc, e = i_(i, j), i_(i, j + 1), i_(i + 1, j + 1), i_(i + 1, j)
+            d.append((a, b, cc))
+            d.append((a, cc, e))
+    for j in range(s):
+        d.append((nord, i_(n - 1, j), i_(n - 1, j + 1)))
+    return np.array(p, dtype=np.float64), np.array(d, dtype=np.uint32)
+
+
+def koerper_netz(form: Optional[Koerperform] = None, kante: float = 0.01, suchtiefe: float = 0.01) -> List[Netz]:
+    """The drape body as two closed triangle meshes of about ``kante`` edge
+    length: the torso ellipsoid and the neck cylinder, each its own body (the
+    reference steps a list of bodies, the last contact found wins)."""
+    k = form or Koerperform()
+    umfang = 2.0 * math.pi * max(k.ell_r[0], k.ell_r[1])
+    seg = max(8, int(round(umfang / kante)))
+    ringe = max(4, int(round(math.pi * k.ell_r[2] / kante)))
+    ep, ed = netz_ellipsoid(k.ell_c, k.ell_r, seg, ringe)
+    zseg = max(8, int(round(2.0 * math.pi * k.cyl_r / kante)))
+    zr = max(1, int(round((k.cyl_z1 - k.cyl_z0) / kante)))
+    zp, zd = netz_zylinder(k.cyl_r, k.cyl_z0, k.cyl_z1, zseg, zr)
+    zp[:, 0] += k.cyl_c[0]
+    zp[:, 1] += k.cyl_c[1]
+    return [Netz(ep, ed, suchtiefe), Netz(zp, zd, suchtiefe)]
 
 
 def drape(cfg: Optional[DrapeKonfig] = None, sdf: Optional[GitterSdf] = None, mit_koerper: bool = True) -> Tuple[Szene, dict]:

@@ -54,8 +54,13 @@ class Statistik_c(C.Structure):
     ]
 
 
+class Naechster_c(C.Structure):
+    _fields_ = [("abstand", C.c_double), ("punkt", C.c_double * 3), ("normale", C.c_double * 3),
+                ("dreieck", C.c_uint32), ("region", C.c_int32)]
+
+
 def bauen_falls_noetig() -> None:
-    src = [os.path.join(_HIER, f) for f in ("gcoracle.cpp", "gcoracle.h", "selbstkollision.h")]
+    src = [os.path.join(_HIER, f) for f in ("gcoracle.cpp", "gcoracle.h", "selbstkollision.h", "netzkoerper.h")]
     if not os.path.exists(_SO) or any(os.path.getmtime(s) > os.path.getmtime(_SO) for s in src):
         subprocess.run(["make", "-C", _HIER, "-s"], check=True)
 
@@ -103,6 +108,25 @@ def lib():
         L.or_sys_mal_a.argtypes = [C.c_void_p, D, D]
         L.or_sys_loese.argtypes = [C.c_void_p, D, I32, I32, D]
         L.or_sys_rest.argtypes = [C.c_void_p, D, D]
+        L.or_netz_neu.restype = C.c_void_p
+        L.or_netz_neu.argtypes = [C.c_int32, D, C.c_int32, U32, C.c_double, C.c_char_p, C.c_int32]
+        L.or_netz_frei.argtypes = [C.c_void_p]
+        L.or_netz_nachfuehren.argtypes = [C.c_void_p, D, C.c_char_p, C.c_int32]
+        L.or_netz_punktgeschwindigkeit.argtypes = [C.c_void_p, D, C.c_int32]
+        I64 = C.POINTER(C.c_int64)
+        L.or_netz_info.argtypes = [C.c_void_p, I64, D, D, D, I64]
+        L.or_netz_gitter.argtypes = [C.c_void_p, C.c_int32, I64, I64, U32, U32]
+        L.or_netz_normalen.argtypes = [C.c_void_p, D, D, U32, D]
+        L.or_netz_naechster.argtypes = [C.c_void_p, D, C.c_double, C.POINTER(Naechster_c)]
+        L.or_netz_naechster_alle.argtypes = [C.c_void_p, D, C.POINTER(Naechster_c)]
+        L.or_netz_abstand_ohne_grenze.argtypes = [C.c_void_p, D, C.POINTER(Naechster_c)]
+        L.or_netz_durchstoss.argtypes = [C.c_void_p, D, D, D, D, D, D]
+        L.or_netz_geschwindigkeit_an.argtypes = [C.c_void_p, C.c_int32, D, D, D]
+        L.or_netz_geschwindigkeit_an.restype = C.c_double
+        L.or_netz_punkte_gegen_dreiecke.argtypes = [C.c_void_p, C.c_double, D, D, C.c_int32, U32, C.c_int32, D, D, D]
+        L.or_netz_punkte_gegen_dreiecke.restype = C.c_int32
+        L.or_netz_kontakt.argtypes = [C.c_void_p, D, D, D, C.c_double, D, D, D]
+        L.or_sim_koerper_netz.argtypes = [C.c_void_p, C.c_void_p, D, D]
     return _lib
 
 
@@ -168,7 +192,7 @@ class OrakelSim:
         self.n = s.n
         self.h = C.c_void_p(L.or_sim_neu(C.byref(c), C.byref(p) if p is not None else None))
         self._keep = []
-        from garmentcraft_amd.szene import GitterSdf, Zylinder
+        from garmentcraft_amd.szene import GitterSdf, Netz, Zylinder
         for kb in s.koerper:
             if isinstance(kb, Zylinder):
                 L.or_sim_koerper_zylinder(self.h, _p(f64(kb.punkt)), _p(f64(kb.achse)), kb.radius, _p(f64(kb.geschwindigkeit)))
@@ -177,6 +201,11 @@ class OrakelSim:
                 dims = np.array(kb.dims, dtype=np.int32)
                 L.or_sim_koerper_sdf(self.h, _p(w, C.POINTER(C.c_float)), _p(dims, I32), _p(f64(kb.origin)), kb.spacing,
                                      _p(f64(kb.versatz)), _p(f64(kb.geschwindigkeit)))
+            elif isinstance(kb, Netz):
+                nk = OrakelNetz(kb.punkte, kb.dreiecke, kb.suchtiefe)
+                if kb.punktgeschwindigkeit is not None:
+                    nk.punktgeschwindigkeit(kb.punktgeschwindigkeit)
+                L.or_sim_koerper_netz(self.h, nk.h, _p(f64(kb.versatz)), _p(f64(kb.geschwindigkeit)))
             else:
                 raise TypeError(kb)
 
@@ -315,6 +344,114 @@ class OrakelSim:
         a, b = f64(x0), f64(x1)
         m = L.or_selbst_stoesse(self.h, _p(a), _p(b), 0, None, None, None, None, None, None)
         return _stoesse_lesen(lambda *q: L.or_selbst_stoesse(self.h, _p(a), _p(b), *q), m)
+
+
+class OrakelNetz:
+    """The oracle's triangle-mesh body (netzkoerper.rs ``Koerpernetz``)."""
+
+    def __init__(self, punkte, dreiecke, suchtiefe: float = 0.01):
+        L = lib()
+        p = f64(punkte).reshape(-1, 3)
+        d = np.ascontiguousarray(dreiecke, dtype=np.uint32).reshape(-1, 3)
+        buf = C.create_string_buffer(256)
+        self.h = None
+        h = L.or_netz_neu(p.shape[0], _p(p), d.shape[0], _p(d, U32), float(suchtiefe), buf, 256)
+        if not h:
+            raise ValueError(buf.value.decode())
+        self.h = C.c_void_p(h)
+        self.n_punkte = p.shape[0]
+        self.n_dreiecke = d.shape[0]
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.or_netz_frei(self.h)
+            self.h = None
+
+    def nachfuehren(self, punkte):
+        p = f64(punkte).reshape(-1, 3)
+        buf = C.create_string_buffer(256)
+        if lib().or_netz_nachfuehren(self.h, _p(p), buf, 256):
+            raise ValueError(buf.value.decode())
+
+    def punktgeschwindigkeit(self, v):
+        if v is None or len(v) == 0:
+            lib().or_netz_punktgeschwindigkeit(self.h, None, 0)
+            return
+        v = f64(v).reshape(-1, 3)
+        if lib().or_netz_punktgeschwindigkeit(self.h, _p(v), v.shape[0]):
+            raise ValueError(f"velocity for {v.shape[0]} points, the mesh has {self.n_punkte}")
+
+    def info(self) -> dict:
+        rk, z = C.c_int64(), C.c_double()
+        vol = C.c_double()
+        u = np.zeros(3)
+        zahl = np.zeros(3, dtype=np.int64)
+        lib().or_netz_info(self.h, C.byref(rk), C.byref(vol), _p(u), C.byref(z), zahl.ctypes.data_as(C.POINTER(C.c_int64)))
+        return {"randkanten": rk.value, "volumen": vol.value, "ursprung": u, "zelle": z.value, "zahl": zahl}
+
+    def gitter(self, punkte: bool = False):
+        na, ne = C.c_int64(), C.c_int64()
+        lib().or_netz_gitter(self.h, int(punkte), C.byref(na), C.byref(ne), None, None)
+        a = np.zeros(na.value, dtype=np.uint32)
+        e = np.zeros(ne.value, dtype=np.uint32)
+        lib().or_netz_gitter(self.h, int(punkte), C.byref(na), C.byref(ne), _p(a, U32), _p(e, U32))
+        return a, e
+
+    def normalen(self):
+        fn = np.zeros((self.n_dreiecke, 3))
+        pn = np.zeros((self.n_punkte, 3))
+        nb = np.zeros((self.n_dreiecke, 3), dtype=np.uint32)
+        kg = np.zeros((self.n_dreiecke, 4))
+        lib().or_netz_normalen(self.h, _p(fn), _p(pn), _p(nb, U32), _p(kg))
+        return {"flaechennormale": fn, "punktnormale": pn, "nachbar": nb, "kugel": kg}
+
+    @staticmethod
+    def _aus(o: Naechster_c) -> dict:
+        return {"abstand": o.abstand, "punkt": np.array(o.punkt[:]), "normale": np.array(o.normale[:]),
+                "dreieck": int(o.dreieck), "region": int(o.region)}
+
+    def naechster(self, x, radius: float):
+        o = Naechster_c()
+        if not lib().or_netz_naechster(self.h, _p(f64(x)), float(radius), C.byref(o)):
+            return None
+        return self._aus(o)
+
+    def naechster_alle(self, x) -> dict:
+        o = Naechster_c()
+        lib().or_netz_naechster_alle(self.h, _p(f64(x)), C.byref(o))
+        return self._aus(o)
+
+    def abstand_ohne_grenze(self, x) -> dict:
+        o = Naechster_c()
+        lib().or_netz_abstand_ohne_grenze(self.h, _p(f64(x)), C.byref(o))
+        return self._aus(o)
+
+    def durchstoss(self, a, b, starr=(0.0, 0.0, 0.0)):
+        p, n, vk = np.zeros(3), np.zeros(3), np.zeros(3)
+        if not lib().or_netz_durchstoss(self.h, _p(f64(a)), _p(f64(b)), _p(f64(starr)), _p(p), _p(n), _p(vk)):
+            return None
+        return p, n, vk
+
+    def geschwindigkeit_an(self, t: int, p, starr=(0.0, 0.0, 0.0)):
+        o = np.zeros(3)
+        lib().or_netz_geschwindigkeit_an(self.h, int(t), _p(f64(p)), _p(f64(starr)), _p(o))
+        return o
+
+    def kontakt(self, x, abstand: float, versatz=(0.0, 0.0, 0.0), geschw=(0.0, 0.0, 0.0)):
+        n, t, vk = np.zeros(3), C.c_double(), np.zeros(3)
+        if not lib().or_netz_kontakt(self.h, _p(f64(versatz)), _p(f64(geschw)), _p(f64(x)), float(abstand), _p(n),
+                                     C.byref(t), _p(vk)):
+            return None
+        return n, t.value, vk
+
+    def punkte_gegen_dreiecke(self, tief, stoff_dreiecke, w, x, v, versatz=(0.0, 0.0, 0.0), geschw=(0.0, 0.0, 0.0)):
+        d = np.ascontiguousarray(stoff_dreiecke, dtype=np.uint32).reshape(-1, 3)
+        x = f64(x).reshape(-1, 3).copy()
+        v = f64(v).reshape(-1, 3).copy()
+        w = f64(w)
+        z = lib().or_netz_punkte_gegen_dreiecke(self.h, float(tief), _p(f64(versatz)), _p(f64(geschw)), d.shape[0],
+                                                _p(d, U32), x.shape[0], _p(w), _p(x), _p(v))
+        return int(z), x, v
 
 
 def _naehe_lesen(fn, m) -> dict:

@@ -14,7 +14,9 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <set>
+#include <string>
 #include <vector>
 
 namespace {
@@ -355,9 +357,12 @@ M3 m3_inv(const M3& m) {
 }
 
 // ---------------------------------------------------------------------------
-// Bodies (koerper.rs) and the sampled grid SDF that replaces the mesh body
+// Bodies (koerper.rs): the cylinder, the triangle-mesh body (netzkoerper.rs)
+// and the sampled grid SDF
 // ---------------------------------------------------------------------------
 struct Kontakt { V3 normale; double tiefe; V3 geschwindigkeit; };
+
+#include "netzkoerper.h"
 
 bool sdf_sample(const float* werte, const int dims[3], const double origin[3], double spacing, const double q[3], double* d, double grad[3]) {
   // Cell and fraction per axis; outside the node box there is no field.
@@ -393,7 +398,8 @@ bool sdf_sample(const float* werte, const int dims[3], const double origin[3], d
 }
 
 struct Koerper {
-  int art = 0;  // 0 = Zylinder (koerper.rs:19), 1 = grid SDF (replaces Netz, koerper.rs:25)
+  // 0 = Zylinder (koerper.rs:19), 1 = grid SDF, 2 = Netz (koerper.rs:25)
+  int art = 0;
   V3 punkt{}, achse{}, geschwindigkeit{};
   double radius = 0;
   std::vector<float> werte;
@@ -401,6 +407,7 @@ struct Koerper {
   double origin[3] = {0, 0, 0};
   double spacing = 0;
   V3 versatz{};
+  std::shared_ptr<const Koerpernetz> netz;  // shared like Arc<Koerpernetz>
 
   // koerper.rs:69-91. The grid SDF answers like the analytic cylinder: at any
   // depth inside its box, with the normalised field gradient as normal.
@@ -413,6 +420,15 @@ struct Koerper {
       double tiefe = radius + abstand - r;
       if (tiefe <= 0.0 || r < 1e-12) return false;
       *out = {mul(quer, 1.0 / r), tiefe, geschwindigkeit};
+      return true;
+    }
+    if (art == 2) {  // koerper.rs:82-90
+      Naechster n;
+      if (!netz->naechster(sub(x, versatz), std::fmax(netz->suchtiefe, abstand), &n)) return false;
+      double tiefe = abstand - n.abstand;
+      V3 vk = netz->geschwindigkeit_an(n.dreieck, n.punkt, geschwindigkeit);
+      if (!(tiefe > 0.0)) return false;
+      *out = {n.normale, tiefe, vk};
       return true;
     }
     V3 q = sub(x, versatz);
@@ -435,6 +451,12 @@ struct Koerper {
   // cylinder has none (koerper.rs:101).
   bool durchstoss(V3 a, V3 b, V3* p, V3* n, V3* vk) const {
     if (art == 0) return false;
+    if (art == 2) {  // koerper.rs:101-110
+      V3 q;
+      if (!netz->durchstoss(sub(a, versatz), sub(b, versatz), geschwindigkeit, &q, n, vk)) return false;
+      *p = add(q, versatz);
+      return true;
+    }
     V3 d = sub(b, a);
     if (dot(d, d) < 1e-24) return false;
     auto abstand = [&](V3 x, double* out) {
@@ -1285,6 +1307,7 @@ int or_sim_koerper_sdf(or_sim* s, const float* werte, const int dims[3], const d
   s->s.koerper.push_back(k);
   return (int)s->s.koerper.size() - 1;
 }
+int or_sim_koerper_netz(or_sim* s, const or_netz* k, const double versatz[3], const double geschw[3]);
 void or_sim_param(or_sim* s, const or_param* p) { s->param = *p; }
 void or_sim_lage(const or_sim* s, double* x, double* v) {
   for (size_t i = 0; i < s->s.x.size(); ++i) { if (x) put(x + 3 * i, s->s.x[i]); if (v) put(v + 3 * i, s->s.v[i]); }
@@ -1511,6 +1534,154 @@ int32_t or_netz_stoesse(int32_t n, const double* x0, double* x1, const double* w
   if (korrekturen) *korrekturen = (int32_t)z;
   for (int i = 0; i < n; ++i) put(x1 + 3 * i, b[i]);
   return (int32_t)l.size();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// C API of the mesh body
+// ---------------------------------------------------------------------------
+struct or_netz { Koerpernetz k; };
+
+static void fehler_setzen(const std::string& f, char* buf, int32_t len) {
+  if (buf && len > 0) {
+    size_t n = std::min<size_t>(f.size(), (size_t)len - 1);
+    std::memcpy(buf, f.data(), n);
+    buf[n] = 0;
+  }
+}
+static void naechster_aus(const Naechster& n, or_naechster* o) {
+  o->abstand = n.abstand;
+  put(o->punkt, n.punkt);
+  put(o->normale, n.normale);
+  o->dreieck = n.dreieck;
+  o->region = n.region;
+}
+
+extern "C" {
+
+or_netz* or_netz_neu(int32_t n_punkte, const double* punkte, int32_t n_dreiecke, const uint32_t* dreiecke, double suchtiefe,
+                     char* fehler, int32_t fehler_len) {
+  std::vector<V3> p((size_t)n_punkte);
+  for (int32_t i = 0; i < n_punkte; ++i) p[i] = v3(punkte + 3 * i);
+  std::vector<Dreieck> d((size_t)n_dreiecke);
+  for (int32_t t = 0; t < n_dreiecke; ++t) d[t] = {dreiecke[3 * t], dreiecke[3 * t + 1], dreiecke[3 * t + 2]};
+  or_netz* o = new or_netz();
+  std::string f = o->k.neu(std::move(p), std::move(d), suchtiefe);
+  if (!f.empty()) {
+    fehler_setzen(f, fehler, fehler_len);
+    delete o;
+    return nullptr;
+  }
+  return o;
+}
+void or_netz_frei(or_netz* k) { delete k; }
+int or_netz_nachfuehren(or_netz* k, const double* punkte, char* fehler, int32_t fehler_len) {
+  std::vector<V3> p(k->k.punkte.size());
+  for (size_t i = 0; i < p.size(); ++i) p[i] = v3(punkte + 3 * i);
+  std::string f = k->k.nachfuehren(p);
+  if (!f.empty()) {
+    fehler_setzen(f, fehler, fehler_len);
+    return 1;
+  }
+  return 0;
+}
+int or_netz_punktgeschwindigkeit(or_netz* k, const double* v, int32_t n) {  // netzkoerper.rs:454-466
+  if (n != 0 && (size_t)n != k->k.punkte.size()) return 1;
+  k->k.punktgeschwindigkeit.clear();
+  for (int32_t i = 0; i < n; ++i) k->k.punktgeschwindigkeit.push_back(v3(v + 3 * i));
+  return 0;
+}
+void or_netz_info(const or_netz* k, int64_t* randkanten, double* volumen, double ursprung[3], double* zelle, int64_t zahl[3]) {
+  *randkanten = (int64_t)k->k.randkanten;
+  *volumen = k->k.volumen;
+  put(ursprung, k->k.gitter.ursprung);
+  *zelle = k->k.gitter.zelle;
+  for (int a = 0; a < 3; ++a) zahl[a] = (int64_t)k->k.gitter.zahl[a];
+}
+void or_netz_gitter(const or_netz* k, int32_t punkte, int64_t* n_anfang, int64_t* n_eintraege, uint32_t* anfang, uint32_t* eintraege) {
+  const Gitter& g = punkte ? k->k.punktgitter : k->k.gitter;
+  *n_anfang = (int64_t)g.anfang.size();
+  *n_eintraege = (int64_t)g.eintraege.size();
+  if (anfang) std::memcpy(anfang, g.anfang.data(), g.anfang.size() * 4);
+  if (eintraege) std::memcpy(eintraege, g.eintraege.data(), g.eintraege.size() * 4);
+}
+void or_netz_normalen(const or_netz* k, double* flaechennormale, double* punktnormale, uint32_t* nachbar, double* kugel4) {
+  const Koerpernetz& n = k->k;
+  for (size_t t = 0; t < n.dreiecke.size(); ++t) {
+    if (flaechennormale) put(flaechennormale + 3 * t, n.flaechennormale[t]);
+    if (nachbar)
+      for (int e = 0; e < 3; ++e) nachbar[3 * t + e] = n.nachbar[t][e];
+    if (kugel4) {
+      put(kugel4 + 4 * t, n.kugel[t].m);
+      kugel4[4 * t + 3] = n.kugel[t].r;
+    }
+  }
+  if (punktnormale)
+    for (size_t i = 0; i < n.punkte.size(); ++i) put(punktnormale + 3 * i, n.punktnormale[i]);
+}
+int or_netz_naechster(const or_netz* k, const double x[3], double radius, or_naechster* out) {
+  Naechster n;
+  if (!k->k.naechster(v3(x), radius, &n)) return 0;
+  naechster_aus(n, out);
+  return 1;
+}
+void or_netz_naechster_alle(const or_netz* k, const double x[3], or_naechster* out) { naechster_aus(k->k.naechster_alle(v3(x)), out); }
+void or_netz_abstand_ohne_grenze(const or_netz* k, const double x[3], or_naechster* out) {
+  naechster_aus(k->k.abstand_ohne_grenze(v3(x)), out);
+}
+int or_netz_durchstoss(const or_netz* k, const double a[3], const double b[3], const double starr[3], double p[3], double n[3],
+                       double vk[3]) {
+  V3 pp, nn, vv;
+  if (!k->k.durchstoss(v3(a), v3(b), v3(starr), &pp, &nn, &vv)) return 0;
+  put(p, pp);
+  put(n, nn);
+  put(vk, vv);
+  return 1;
+}
+double or_netz_geschwindigkeit_an(const or_netz* k, int32_t t, const double p[3], const double starr[3], double out[3]) {
+  put(out, k->k.geschwindigkeit_an((size_t)t, v3(p), v3(starr)));
+  return 0.0;
+}
+int32_t or_netz_punkte_gegen_dreiecke(const or_netz* k, double tief, const double versatz[3], const double geschw[3], int32_t nd,
+                                      const uint32_t* dreiecke, int32_t n, const double* w, double* x, double* v) {
+  std::vector<Dreieck> d((size_t)nd);
+  for (int32_t t = 0; t < nd; ++t) d[t] = {dreiecke[3 * t], dreiecke[3 * t + 1], dreiecke[3 * t + 2]};
+  std::vector<double> ww(w, w + n);
+  std::vector<V3> xx(n), vv(n);
+  for (int32_t i = 0; i < n; ++i) {
+    xx[i] = v3(x + 3 * i);
+    vv[i] = v3(v + 3 * i);
+  }
+  size_t z = k->k.punkte_gegen_dreiecke(tief, v3(versatz), v3(geschw), d, ww, xx, vv);
+  for (int32_t i = 0; i < n; ++i) {
+    put(x + 3 * i, xx[i]);
+    put(v + 3 * i, vv[i]);
+  }
+  return (int32_t)z;
+}
+int or_netz_kontakt(const or_netz* k, const double versatz[3], const double geschw[3], const double x[3], double abstand,
+                    double normale[3], double* tiefe, double vk[3]) {
+  Koerper kb;
+  kb.art = 2;
+  kb.netz = std::shared_ptr<const Koerpernetz>(&k->k, [](const Koerpernetz*) {});
+  kb.versatz = v3(versatz);
+  kb.geschwindigkeit = v3(geschw);
+  Kontakt kt;
+  if (!kb.kontakt(v3(x), abstand, &kt)) return 0;
+  put(normale, kt.normale);
+  *tiefe = kt.tiefe;
+  put(vk, kt.geschwindigkeit);
+  return 1;
+}
+int or_sim_koerper_netz(or_sim* s, const or_netz* k, const double versatz[3], const double geschw[3]) {
+  Koerper kb;
+  kb.art = 2;
+  kb.netz = std::make_shared<const Koerpernetz>(k->k);
+  kb.versatz = v3(versatz);
+  kb.geschwindigkeit = v3(geschw);
+  s->s.koerper.push_back(kb);
+  return (int)s->s.koerper.size() - 1;
 }
 
 }  // extern "C"

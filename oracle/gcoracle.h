@@ -52,6 +52,39 @@ double or_koerper_sdf(const double ell_c[3], const double ell_r[3], const double
 int    or_sdf_sample(const float* werte, const int dims[3], const double origin[3], double spacing,
                      const double q[3], double* d, double grad[3]);
 
+/* ---- triangle-mesh body (netzkoerper.rs) ---- */
+typedef struct or_netz or_netz;
+typedef struct or_naechster {
+  double abstand;     /* outside positive */
+  double punkt[3];
+  double normale[3];
+  uint32_t dreieck;
+  int32_t region;     /* 0..2 corner k, 3..5 edge k (p0p1, p1p2, p2p0), 6 face */
+} or_naechster;
+/* Koerpernetz::neu (netzkoerper.rs:297-388); NULL and a message on failure. */
+or_netz* or_netz_neu(int32_t n_punkte, const double* punkte, int32_t n_dreiecke, const uint32_t* dreiecke, double suchtiefe,
+                     char* fehler, int32_t fehler_len);
+void     or_netz_frei(or_netz* k);
+int      or_netz_nachfuehren(or_netz* k, const double* punkte, char* fehler, int32_t fehler_len);
+int      or_netz_punktgeschwindigkeit(or_netz* k, const double* v, int32_t n); /* n = 0 clears */
+/* randkanten, volumen, grid origin[3], cell, counts[3] */
+void     or_netz_info(const or_netz* k, int64_t* randkanten, double* volumen, double ursprung[3], double* zelle, int64_t zahl[3]);
+/* Grid of triangles (punkte = 0) or of points (punkte = 1): sizes, then the arrays. */
+void     or_netz_gitter(const or_netz* k, int32_t punkte, int64_t* n_anfang, int64_t* n_eintraege, uint32_t* anfang, uint32_t* eintraege);
+void     or_netz_normalen(const or_netz* k, double* flaechennormale, double* punktnormale, uint32_t* nachbar, double* kugel4);
+int      or_netz_naechster(const or_netz* k, const double x[3], double radius, or_naechster* out);
+void     or_netz_naechster_alle(const or_netz* k, const double x[3], or_naechster* out);
+void     or_netz_abstand_ohne_grenze(const or_netz* k, const double x[3], or_naechster* out);
+int      or_netz_durchstoss(const or_netz* k, const double a[3], const double b[3], const double starr[3], double p[3],
+                            double n[3], double vk[3]);
+double   or_netz_geschwindigkeit_an(const or_netz* k, int32_t t, const double p[3], const double starr[3], double out[3]);
+/* Koerpernetz::punkte_gegen_dreiecke on cloth triangles; x and v in place. */
+int32_t  or_netz_punkte_gegen_dreiecke(const or_netz* k, double tief, const double versatz[3], const double geschw[3],
+                                       int32_t nd, const uint32_t* dreiecke, int32_t n, const double* w, double* x, double* v);
+/* Koerper::kontakt of a mesh body at offset `versatz` (koerper.rs:82-90). */
+int      or_netz_kontakt(const or_netz* k, const double versatz[3], const double geschw[3], const double x[3], double abstand,
+                         double normale[3], double* tiefe, double vk[3]);
+
 /* ---- simulation ---- */
 typedef struct or_szene {
   int32_t n;
@@ -118,6 +151,8 @@ void    or_sim_frei(or_sim* s);
 int     or_sim_koerper_zylinder(or_sim* s, const double punkt[3], const double achse[3], double radius, const double geschw[3]);
 int     or_sim_koerper_sdf(or_sim* s, const float* werte, const int dims[3], const double origin[3], double spacing,
                            const double versatz[3], const double geschw[3]);
+/* A mesh body (a copy of k, shared between the frames like Arc<Koerpernetz>). */
+int     or_sim_koerper_netz(or_sim* s, const or_netz* k, const double versatz[3], const double geschw[3]);
 void    or_sim_param(or_sim* s, const or_param* p);
 void    or_sim_lage(const or_sim* s, double* x, double* v);
 void    or_sim_setze_lage(or_sim* s, const double* x, const double* v);
