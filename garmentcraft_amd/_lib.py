@@ -94,6 +94,11 @@ class AdjointStat_c(C.Structure):
                 ("abweichungen", C.c_int64), ("eintritte", C.c_int64), ("rest_max", C.c_double)]
 
 
+class NetzAngaben_c(C.Structure):
+    _fields_ = [("n_punkte", C.c_int32), ("n_dreiecke", C.c_int32), ("randkanten", C.c_int64), ("volumen", C.c_double),
+                ("ursprung", C.c_double * 3), ("zelle", C.c_double), ("zahl", C.c_int64 * 3), ("n_eintraege", C.c_int64)]
+
+
 # (name, restype, argtypes) for every entry point of include/gcsim.h
 SIGNATUREN = [
     ("gc_create", C.c_void_p, [C.POINTER(Optionen), C.c_char_p, C.c_int32]),
@@ -152,6 +157,13 @@ SIGNATUREN = [
     ("gc_entscheidungen", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("gc_entscheidungen_abweichungen", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]),
     ("gc_naht_planen", C.c_int, [C.POINTER(NahtEingabe_c), C.POINTER(NahtAusgabe_c)]),
+    ("gc_add_netz", C.c_int, [C.c_void_p, D, C.c_int32, U32, C.c_int32, C.c_double, D, D]),
+    ("gc_netz_nachfuehren", C.c_int, [C.c_void_p, C.c_int32, D]),
+    ("gc_netz_punktgeschwindigkeit", C.c_int, [C.c_void_p, C.c_int32, D, C.c_int32]),
+    ("gc_netz_info", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(NetzAngaben_c)]),
+    ("gc_netz_lesen", C.c_int, [C.c_void_p, C.c_int32, D, D, U32, D, U32, U32]),
+    ("gc_netz_naechster", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, D, C.c_double, I32, D, D, D, I32, I32]),
+    ("gc_netz_durchstoss", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, D, D, D, I32, D, D, D]),
 ]
 
 _lib = None

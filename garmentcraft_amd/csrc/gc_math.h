@@ -316,12 +316,14 @@ GC_HD V3<T> m3_mal(const T m[9], V3<T> v) {  // implizit.rs:441-443
 
 // ---------------------------------------------------------------------------
 // Bodies. A body is a struct of plain values so the whole list rides in the
-// kernel arguments.
+// kernel arguments; a mesh body's arrays are reached through one pointer.
 // ---------------------------------------------------------------------------
-enum { GC_ZYLINDER = 0, GC_SDF = 1 };
+#include "gc_netz.h"
+
+enum { GC_ZYLINDER = 0, GC_SDF = 1, GC_NETZ = 2 };
 struct Koerper {
   int32_t art;
-  double punkt[3];          // cylinder axis point / SDF offset (versatz)
+  double punkt[3];          // cylinder axis point / SDF and mesh offset (versatz)
   double achse[3];          // cylinder axis (any length)
   double radius;
   double geschwindigkeit[3];
@@ -329,6 +331,7 @@ struct Koerper {
   int32_t dims[3];
   double origin[3];
   double spacing;
+  const NetzDev* netz;      // mesh body (device memory)
 };
 #define GC_MAX_KOERPER 4
 
@@ -367,10 +370,36 @@ GC_HD bool sdf_sample(const Koerper& k, const S q[3], S* d, S grad[3]) {
   return true;
 }
 
-// koerper.rs:69-91 (cylinder) and the same query on the grid SDF.
+// Koerper::kontakt of a mesh body (koerper.rs:82-90): the nearest surface
+// point within max(suchtiefe, abstand), depth abstand - signed distance, the
+// surface velocity at the nearest point. `roh` keeps a contact of depth <= 0.
+GC_HD bool netz_kontakt(const Koerper& k, V3<double> x, double abstand, bool roh, V3<double>* normale, double* tiefe,
+                        V3<double>* vk) {
+  const NetzDev& N = *k.netz;
+  const V3<double> versatz = {k.punkt[0], k.punkt[1], k.punkt[2]};
+  const V3<double> starr = {k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+  nk::Treffer t;
+  if (!nk::naechster(N, x - versatz, gmax(N.suchtiefe, abstand), &t)) return false;
+  const double tf = abstand - t.abstand;
+  if (!roh && !(tf > 0.0)) return false;
+  *vk = nk::geschwindigkeit_an(N, t.dreieck, t.punkt, starr);
+  *normale = t.normale;
+  *tiefe = tf;
+  return true;
+}
+
+template <typename S> struct IstDouble { static constexpr bool wert = false; };
+template <> struct IstDouble<double> { static constexpr bool wert = true; };
+
+// koerper.rs:69-91 (cylinder, mesh) and the same query on the grid SDF. The
+// mesh body answers in FP64 only (the adjoint refuses it).
 template <typename S>
 GC_HD bool kontakt(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* tiefe, V3<double>* vk) {
   *vk = V3<double>{k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+  if (k.art == GC_NETZ) {
+    if constexpr (IstDouble<S>::wert) return netz_kontakt(k, x, abstand, false, normale, tiefe, vk);
+    else return false;
+  }
   if (k.art == GC_ZYLINDER) {
     V3<S> ach = {S(k.achse[0]), S(k.achse[1]), S(k.achse[2])};
     V3<S> a = ach * (S(1.0) / norm(ach));
@@ -403,6 +432,10 @@ GC_HD bool kontakt(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* tief
 template <typename S>
 GC_HD bool kontakt_roh(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* tiefe, V3<double>* vk) {
   *vk = V3<double>{k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+  if (k.art == GC_NETZ) {
+    if constexpr (IstDouble<S>::wert) return netz_kontakt(k, x, abstand, true, normale, tiefe, vk);
+    else return false;
+  }
   if (k.art == GC_ZYLINDER) {
     V3<S> ach = {S(k.achse[0]), S(k.achse[1]), S(k.achse[2])};
     V3<S> a = ach * (S(1.0) / norm(ach));
@@ -435,6 +468,14 @@ GC_HD bool kontakt_roh(const Koerper& k, V3<S> x, S abstand, V3<S>* normale, S* 
 GC_HD bool durchstoss(const Koerper& k, V3<double> a, V3<double> b, V3<double>* p, V3<double>* n, V3<double>* vk,
                       double* t_hit = nullptr) {
   if (k.art == GC_ZYLINDER) return false;
+  if (k.art == GC_NETZ) {  // koerper.rs:101-110, netzkoerper.rs:541-596
+    const V3<double> versatz = {k.punkt[0], k.punkt[1], k.punkt[2]};
+    const V3<double> starr = {k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+    V3<double> q;
+    if (!nk::durchstoss(*k.netz, a - versatz, b - versatz, starr, &q, n, vk, t_hit)) return false;
+    *p = q + versatz;
+    return true;
+  }
   const V3<double> d = b - a;
   if (dot(d, d) < 1e-24) return false;
   auto abstand = [&](V3<double> x, double* out) {

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -26,9 +27,36 @@ namespace {
 
 const int BUILD_THREADS = 256;
 
+// A mesh body: its topology on the host, its arrays on the device.
+struct Netzhost {
+  int n = 0, nt = 0;
+  double suchtiefe = 0.0;
+  std::vector<uint32_t> dreiecke;
+  std::vector<double> punkte;  // the current pose (host copy: volume, hull)
+  int64_t randkanten = 0;
+  double volumen = 0.0;
+  bool hat_gitter = false;
+  NetzDev bild{};  // host image of the device struct
+  double *punkte_d = nullptr, *fn_d = nullptr, *pn_d = nullptr, *kugel_d = nullptr, *pv_d = nullptr;
+  uint32_t *dreiecke_d = nullptr, *nachbar_d = nullptr, *inz_anfang_d = nullptr, *inz_d = nullptr;
+  uint32_t *anfang_d = nullptr, *zaehler_d = nullptr, *cursor_d = nullptr, *roh_d = nullptr, *eintraege_d = nullptr;
+  size_t nz_kap = 0, e_kap = 0;
+  NetzDev* dev = nullptr;
+  void frei() {
+    for (void* q : {(void*)punkte_d, (void*)fn_d, (void*)pn_d, (void*)kugel_d, (void*)pv_d, (void*)dreiecke_d,
+                    (void*)nachbar_d, (void*)inz_anfang_d, (void*)inz_d, (void*)anfang_d, (void*)zaehler_d,
+                    (void*)cursor_d, (void*)roh_d, (void*)eintraege_d, (void*)dev})
+      if (q) (void)hipFree(q);
+    punkte_d = fn_d = pn_d = kugel_d = pv_d = nullptr;
+    dreiecke_d = nachbar_d = inz_anfang_d = inz_d = anfang_d = zaehler_d = cursor_d = roh_d = eintraege_d = nullptr;
+    dev = nullptr;
+  }
+};
+
 struct Koerperhost {
   Koerper k;
   float* werte_dev = nullptr;
+  std::shared_ptr<Netzhost> netz;
 };
 
 }  // namespace
@@ -200,8 +228,10 @@ static void frei_alles(gc_ctx* c) {
   c->allocs.clear();
   for (void* p : c->sk_allocs) (void)hipFree(p);
   c->sk_allocs.clear();
-  for (auto& k : c->koerper)
+  for (auto& k : c->koerper) {
     if (k.werte_dev) (void)hipFree(k.werte_dev);
+    if (k.netz) k.netz->frei();
+  }
 }
 
 template <typename T> static SysDev<T>& sys(gc_ctx* c);
@@ -2018,6 +2048,8 @@ static int adj_schritt(gc_ctx* c, double h) {
   if (!(h > 0.0)) return fehler(c, "h must be > 0");
   if (selbst_an(c))
     return fehler(c, "the adjoint does not differentiate cloth self-collision: switch it off (gc_param.selbstkollision = 0)");
+  for (const auto& k : c->koerper)
+    if (k.k.art == GC_NETZ) return fehler(c, "the adjoint does not differentiate the mesh body's contact; use a grid SDF body");
   if (c->param.normalkraft_aus_last)
     return fehler(c, "the adjoint takes the normal force from the residual (gc_param.normalkraft_aus_last = 0)");
   if (adj_vorbereiten<T>(c)) return 1;
@@ -2457,6 +2489,387 @@ int gc_add_sdf(gc_ctx* c, const float* werte, const int32_t dims[3], const doubl
   c->koerper.push_back(kh);
   alle_param(c);
   return (int)c->koerper.size() - 1;
+}
+
+// --- mesh body (netzkoerper.rs) ----------------------------------------------
+}  // extern "C"
+
+// Koerpernetz::neu's checks and topology (netzkoerper.rs:297-348): every
+// undirected edge has one or two triangles, two of them run it in opposite
+// directions; the neighbour across each side (the triangle itself at an open
+// edge). Also each point's incidences (3t + k, ascending) for its normal.
+static std::string netz_topologie(Netzhost& h, std::vector<uint32_t>& nachbar, std::vector<uint32_t>& inz_anfang,
+                                  std::vector<uint32_t>& inz) {
+  if (h.nt == 0) return "mesh body without triangles";
+  if (!(h.suchtiefe > 0.0)) return "search depth " + std::to_string(h.suchtiefe) + " m must be positive";
+  for (int t = 0; t < h.nt; ++t)
+    for (int k = 0; k < 3; ++k)
+      if (h.dreiecke[3 * t + k] >= (uint32_t)h.n)
+        return "triangle " + std::to_string(t) + " refers to a point >= " + std::to_string(h.n);
+  struct Seite {
+    uint64_t schluessel;
+    uint32_t t;
+    uint8_t k;
+    bool r;
+  };
+  std::vector<Seite> seiten;
+  seiten.reserve(3 * (size_t)h.nt);
+  for (int t = 0; t < h.nt; ++t)
+    for (uint8_t k = 0; k < 3; ++k) {
+      const uint32_t i = h.dreiecke[3 * t + k], j = h.dreiecke[3 * t + (k + 1) % 3];
+      const uint64_t lo = std::min(i, j), hi = std::max(i, j);
+      seiten.push_back({(lo << 32) | hi, (uint32_t)t, k, i < j});
+    }
+  std::stable_sort(seiten.begin(), seiten.end(), [](const Seite& a, const Seite& b) { return a.schluessel < b.schluessel; });
+  nachbar.assign(3 * (size_t)h.nt, 0u);
+  h.randkanten = 0;
+  for (size_t a = 0; a < seiten.size();) {
+    size_t b = a;
+    while (b < seiten.size() && seiten[b].schluessel == seiten[a].schluessel) ++b;
+    const uint32_t i = (uint32_t)(seiten[a].schluessel >> 32), j = (uint32_t)seiten[a].schluessel;
+    if (b - a == 1) {
+      h.randkanten += 1;
+      nachbar[3 * seiten[a].t + seiten[a].k] = seiten[a].t;
+    } else if (b - a == 2) {
+      if (seiten[a].r == seiten[a + 1].r)
+        return "mesh not consistently oriented at the edge " + std::to_string(i) + "-" + std::to_string(j);
+      nachbar[3 * seiten[a].t + seiten[a].k] = seiten[a + 1].t;
+      nachbar[3 * seiten[a + 1].t + seiten[a + 1].k] = seiten[a].t;
+    } else {
+      return "edge " + std::to_string(i) + "-" + std::to_string(j) + " has " + std::to_string(b - a) + " triangles";
+    }
+    a = b;
+  }
+  inz_anfang.assign((size_t)h.n + 1, 0u);
+  for (int t = 0; t < h.nt; ++t)
+    for (int k = 0; k < 3; ++k) inz_anfang[h.dreiecke[3 * t + k] + 1] += 1;
+  for (int i = 0; i < h.n; ++i) inz_anfang[i + 1] += inz_anfang[i];
+  inz.assign(inz_anfang[h.n], 0u);
+  std::vector<uint32_t> frei(inz_anfang.begin(), inz_anfang.end() - 1);
+  for (int t = 0; t < h.nt; ++t)
+    for (int k = 0; k < 3; ++k) inz[frei[h.dreiecke[3 * t + k]]++] = (uint32_t)(3 * t + k);
+  return "";
+}
+
+// Enclosed volume of the pose, summed in triangle order (netzkoerper.rs:282).
+static double netz_volumen(const Netzhost& h) {
+  double vol = 0.0;
+  for (int t = 0; t < h.nt; ++t) {
+    const double* q0 = &h.punkte[3 * h.dreiecke[3 * t]];
+    const double* q1 = &h.punkte[3 * h.dreiecke[3 * t + 1]];
+    const double* q2 = &h.punkte[3 * h.dreiecke[3 * t + 2]];
+    const double cx = q1[1] * q2[2] - q1[2] * q2[1], cy = q1[2] * q2[0] - q1[0] * q2[2], cz = q1[0] * q2[1] - q1[1] * q2[0];
+    vol += (q0[0] * cx + q0[1] * cy + q0[2] * cz) / 6.0;
+  }
+  return vol;
+}
+
+// The grid raster of the pose (netzkoerper.rs:820-850): cells 2 x suchtiefe
+// over the points' box widened by suchtiefe, joined to the previous raster
+// (it never shrinks).
+static void netz_raster(Netzhost& h) {
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < h.n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::fmin(lo[a], h.punkte[3 * i + a]);
+      hi[a] = std::fmax(hi[a], h.punkte[3 * i + a]);
+    }
+  const double zelle = 2.0 * h.suchtiefe;
+  double ursprung[3], ende[3];
+  for (int a = 0; a < 3; ++a) {
+    ursprung[a] = lo[a] - h.suchtiefe;
+    ende[a] = hi[a] + h.suchtiefe;
+  }
+  if (h.hat_gitter)
+    for (int a = 0; a < 3; ++a) {
+      ursprung[a] = std::fmin(ursprung[a], h.bild.ursprung[a]);
+      ende[a] = std::fmax(ende[a], h.bild.ursprung[a] + h.bild.zelle * (double)h.bild.zahl[a]);
+    }
+  for (int a = 0; a < 3; ++a) {
+    const double z = std::ceil((ende[a] - ursprung[a]) / zelle);
+    const int64_t zahl = z > 0.0 ? (int64_t)z : 0;  // `as usize`, then max(1)
+    h.bild.zahl[a] = std::max<int64_t>(zahl, 1);
+    h.bild.ursprung[a] = ursprung[a];
+  }
+  h.bild.zelle = zelle;
+  h.hat_gitter = true;
+}
+
+template <typename X>
+static int netz_alloc(gc_ctx* c, X** p, size_t count) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (hipMalloc((void**)p, std::max<size_t>(1, count) * sizeof(X)) != hipSuccess) return fehler(c, "hipMalloc failed (mesh body)");
+  return 0;
+}
+
+// Everything that depends on the points (netzkoerper.rs:257-291, 820-912):
+// face normals, spheres, point normals and the grid of triangle lists, built
+// on the device from the pose in h.punkte.
+static int netz_pose(gc_ctx* c, Netzhost& h) {
+  const double vol = netz_volumen(h);
+  if (!(vol > 0.0))
+    return fehler(c, "volume " + std::to_string(vol) + " m^3 <= 0: the normals point inwards");
+  h.volumen = vol;
+  netz_raster(h);
+  const int64_t nz = h.bild.zahl[0] * h.bild.zahl[1] * h.bild.zahl[2];
+  if (nz <= 0 || nz > (int64_t)1 << 31) return fehler(c, "mesh body grid of " + std::to_string(nz) + " cells");
+  hipStream_t st = c->stream;
+  HIPCK(c, hipMemcpyAsync(h.punkte_d, h.punkte.data(), h.punkte.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  if ((size_t)nz + 1 > h.nz_kap) {
+    if (netz_alloc(c, &h.anfang_d, (size_t)nz + 1) || netz_alloc(c, &h.zaehler_d, (size_t)nz) ||
+        netz_alloc(c, &h.cursor_d, (size_t)nz))
+      return 1;
+    h.nz_kap = (size_t)nz + 1;
+  }
+  h.bild.anfang = h.anfang_d;
+  h.bild.pv = h.pv_d;
+  HIPCK(c, hipMemcpyAsync(h.dev, &h.bild, sizeof(NetzDev), hipMemcpyHostToDevice, st));
+  const unsigned gt = (unsigned)std::min<int64_t>(4096, (h.nt + 255) / 256), gp = (unsigned)std::min<int64_t>(4096, (h.n + 255) / 256);
+  hipLaunchKernelGGL(k_nk_flaechen, dim3(gt), dim3(256), 0, st, h.punkte_d, h.dreiecke_d, h.nt, h.fn_d, h.kugel_d);
+  hipLaunchKernelGGL(k_nk_punktnormalen, dim3(gp), dim3(256), 0, st, h.punkte_d, h.dreiecke_d, h.fn_d, h.inz_anfang_d, h.inz_d,
+                     h.n, h.pn_d);
+  HIPCK(c, hipMemsetAsync(h.zaehler_d, 0, (size_t)nz * sizeof(uint32_t), st));
+  hipLaunchKernelGGL(k_nk_zaehlen, dim3(gt), dim3(256), 0, st, (const NetzDev*)h.dev, h.punkte_d, h.dreiecke_d, h.nt, h.zaehler_d);
+  hipLaunchKernelGGL(k_nk_scan, dim3(1), dim3(1024), 0, st, h.zaehler_d, nz, h.anfang_d);
+  HIPCK(c, hipGetLastError());
+  uint32_t gesamt = 0;
+  HIPCK(c, hipMemcpyAsync(&gesamt, h.anfang_d + nz, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipStreamSynchronize(st));
+  if (gesamt > h.e_kap || !h.eintraege_d) {
+    if (netz_alloc(c, &h.roh_d, gesamt) || netz_alloc(c, &h.eintraege_d, gesamt)) return 1;
+    h.e_kap = gesamt;
+  }
+  h.bild.eintraege = h.eintraege_d;
+  HIPCK(c, hipMemcpyAsync(h.dev, &h.bild, sizeof(NetzDev), hipMemcpyHostToDevice, st));
+  HIPCK(c, hipMemcpyAsync(h.cursor_d, h.anfang_d, (size_t)nz * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_nk_fuellen, dim3(gt), dim3(256), 0, st, (const NetzDev*)h.dev, h.punkte_d, h.dreiecke_d, h.nt, h.cursor_d,
+                     h.roh_d);
+  if (gesamt > 0)
+    hipLaunchKernelGGL(k_nk_ordnen, dim3((unsigned)std::min<uint32_t>(4096, (gesamt + 255) / 256)), dim3(256), 0, st, h.anfang_d,
+                       nz, h.roh_d, h.eintraege_d);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" {
+
+int gc_add_netz(gc_ctx* c, const double* punkte, int32_t n_punkte, const uint32_t* dreiecke, int32_t n_dreiecke, double suchtiefe,
+                const double versatz[3], const double v[3]) {
+  if (!c) return -1;
+  if (c->koerper.size() >= GC_MAX_KOERPER) {
+    c->err = "too many bodies";
+    return -1;
+  }
+  if (n_punkte < 0 || n_dreiecke < 0 || (n_punkte > 0 && !punkte) || (n_dreiecke > 0 && !dreiecke)) {
+    c->err = "gc_add_netz: bad arrays";
+    return -1;
+  }
+  auto h = std::make_shared<Netzhost>();
+  h->n = n_punkte;
+  h->nt = n_dreiecke;
+  h->suchtiefe = suchtiefe;
+  h->dreiecke.assign(dreiecke, dreiecke + 3 * (size_t)n_dreiecke);
+  h->punkte.assign(punkte, punkte + 3 * (size_t)n_punkte);
+  std::vector<uint32_t> nachbar, inz_anfang, inz;
+  const std::string f = netz_topologie(*h, nachbar, inz_anfang, inz);
+  if (!f.empty()) {
+    c->err = f;
+    return -1;
+  }
+  const size_t n = (size_t)h->n, nt = (size_t)h->nt;
+  if (netz_alloc(c, &h->punkte_d, 3 * n) || netz_alloc(c, &h->fn_d, 3 * nt) || netz_alloc(c, &h->pn_d, 3 * n) ||
+      netz_alloc(c, &h->kugel_d, 4 * nt) || netz_alloc(c, &h->dreiecke_d, 3 * nt) || netz_alloc(c, &h->nachbar_d, 3 * nt) ||
+      netz_alloc(c, &h->inz_anfang_d, n + 1) || netz_alloc(c, &h->inz_d, inz.size()) || netz_alloc(c, &h->dev, 1)) {
+    h->frei();
+    return -1;
+  }
+  hipStream_t st = c->stream;
+  if (hipMemcpyAsync(h->dreiecke_d, h->dreiecke.data(), 3 * nt * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(h->nachbar_d, nachbar.data(), 3 * nt * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(h->inz_anfang_d, inz_anfang.data(), (n + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+      (!inz.empty() && hipMemcpyAsync(h->inz_d, inz.data(), inz.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)) {
+    c->err = "hipMemcpy failed (mesh body)";
+    h->frei();
+    return -1;
+  }
+  NetzDev& b = h->bild;
+  b.punkte = h->punkte_d;
+  b.dreiecke = h->dreiecke_d;
+  b.fn = h->fn_d;
+  b.nachbar = h->nachbar_d;
+  b.pn = h->pn_d;
+  b.kugel = h->kugel_d;
+  b.pv = nullptr;
+  b.suchtiefe = suchtiefe;
+  b.n_punkte = n_punkte;
+  b.n_dreiecke = n_dreiecke;
+  if (netz_pose(c, *h)) {
+    h->frei();
+    return -1;
+  }
+  Koerperhost kh;
+  std::memset(&kh.k, 0, sizeof(Koerper));
+  kh.k.art = GC_NETZ;
+  kh.k.netz = h->dev;
+  for (int q = 0; q < 3; ++q) {
+    kh.k.punkt[q] = versatz[q];
+    kh.k.geschwindigkeit[q] = v[q];
+  }
+  kh.netz = h;
+  c->koerper.push_back(kh);
+  alle_param(c);
+  return (int)c->koerper.size() - 1;
+}
+
+static Netzhost* netz_von(gc_ctx* c, int32_t k) {
+  if (!c || k < 0 || k >= (int32_t)c->koerper.size() || !c->koerper[(size_t)k].netz) {
+    if (c) c->err = "body " + std::to_string(k) + " is not a mesh body";
+    return nullptr;
+  }
+  return c->koerper[(size_t)k].netz.get();
+}
+
+int gc_netz_nachfuehren(gc_ctx* c, int32_t k, const double* punkte) {
+  Netzhost* h = netz_von(c, k);
+  if (!h) return 1;
+  std::vector<double> alt = std::move(h->punkte);
+  h->punkte.assign(punkte, punkte + 3 * (size_t)h->n);
+  if (netz_volumen(*h) <= 0.0 || !(netz_volumen(*h) > 0.0)) {
+    h->punkte = std::move(alt);
+    return fehler(c, "volume <= 0 after the deformation: the mesh has turned inside out");
+  }
+  if (h->pv_d) {  // the field belonged to the old pose (netzkoerper.rs:409)
+    (void)hipFree(h->pv_d);
+    h->pv_d = nullptr;
+  }
+  return netz_pose(c, *h);
+}
+
+int gc_netz_punktgeschwindigkeit(gc_ctx* c, int32_t k, const double* v, int32_t n) {
+  Netzhost* h = netz_von(c, k);
+  if (!h) return 1;
+  if (n != 0 && n != h->n)
+    return fehler(c, "velocity for " + std::to_string(n) + " points, the mesh has " + std::to_string(h->n));
+  if (n == 0) {
+    if (h->pv_d) (void)hipFree(h->pv_d);
+    h->pv_d = nullptr;
+  } else {
+    if (!h->pv_d && netz_alloc(c, &h->pv_d, 3 * (size_t)n)) return 1;
+    HIPCK(c, hipMemcpyAsync(h->pv_d, v, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  h->bild.pv = h->pv_d;
+  HIPCK(c, hipMemcpyAsync(h->dev, &h->bild, sizeof(NetzDev), hipMemcpyHostToDevice, c->stream));
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int gc_netz_info(gc_ctx* c, int32_t k, gc_netz_angaben* out) {
+  Netzhost* h = netz_von(c, k);
+  if (!h || !out) return 1;
+  out->n_punkte = h->n;
+  out->n_dreiecke = h->nt;
+  out->randkanten = h->randkanten;
+  out->volumen = h->volumen;
+  for (int a = 0; a < 3; ++a) {
+    out->ursprung[a] = h->bild.ursprung[a];
+    out->zahl[a] = h->bild.zahl[a];
+  }
+  out->zelle = h->bild.zelle;
+  uint32_t gesamt = 0;
+  const int64_t nz = h->bild.zahl[0] * h->bild.zahl[1] * h->bild.zahl[2];
+  HIPCK(c, hipMemcpy(&gesamt, h->anfang_d + nz, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  out->n_eintraege = gesamt;
+  return 0;
+}
+
+int gc_netz_lesen(gc_ctx* c, int32_t k, double* flaechennormale, double* punktnormale, uint32_t* nachbar, double* kugel,
+                  uint32_t* anfang, uint32_t* eintraege) {
+  Netzhost* h = netz_von(c, k);
+  if (!h) return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)h->n, nt = (size_t)h->nt;
+  const int64_t nz = h->bild.zahl[0] * h->bild.zahl[1] * h->bild.zahl[2];
+  uint32_t gesamt = 0;
+  HIPCK(c, hipMemcpy(&gesamt, h->anfang_d + nz, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (flaechennormale) HIPCK(c, hipMemcpy(flaechennormale, h->fn_d, 3 * nt * sizeof(double), hipMemcpyDeviceToHost));
+  if (punktnormale) HIPCK(c, hipMemcpy(punktnormale, h->pn_d, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  if (nachbar) HIPCK(c, hipMemcpy(nachbar, h->nachbar_d, 3 * nt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (kugel) HIPCK(c, hipMemcpy(kugel, h->kugel_d, 4 * nt * sizeof(double), hipMemcpyDeviceToHost));
+  if (anfang) HIPCK(c, hipMemcpy(anfang, h->anfang_d, ((size_t)nz + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (eintraege && gesamt) HIPCK(c, hipMemcpy(eintraege, h->eintraege_d, (size_t)gesamt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"
+
+// Device scratch for one query call (host arrays in and out).
+struct NetzAbfrage {
+  std::vector<void*> p;
+  ~NetzAbfrage() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <typename X>
+  X* neu(size_t count) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(1, count) * sizeof(X)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return (X*)q;
+  }
+};
+
+extern "C" {
+
+int gc_netz_naechster(gc_ctx* c, int32_t k, int32_t n, const double* x, double radius, int32_t* da, double* abstand,
+                      double* punkt, double* normale, int32_t* dreieck, int32_t* region) {
+  Netzhost* h = netz_von(c, k);
+  if (!h) return 1;
+  if (n <= 0) return 0;
+  const size_t m = (size_t)n;
+  NetzAbfrage q;
+  double *xd = q.neu<double>(3 * m), *ad = q.neu<double>(m), *pd = q.neu<double>(3 * m), *nd = q.neu<double>(3 * m);
+  int32_t *dd = q.neu<int32_t>(m), *td = q.neu<int32_t>(m), *rd = q.neu<int32_t>(m);
+  if (!xd || !ad || !pd || !nd || !dd || !td || !rd) return fehler(c, "hipMalloc failed (mesh query)");
+  hipStream_t st = c->stream;
+  HIPCK(c, hipMemcpyAsync(xd, x, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_nk_naechster, dim3((unsigned)std::min<size_t>(4096, (m + 255) / 256)), dim3(256), 0, st,
+                     (const NetzDev*)h->dev, n, xd, radius, dd, ad, pd, nd, td, rd);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipMemcpyAsync(da, dd, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (abstand) HIPCK(c, hipMemcpyAsync(abstand, ad, m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (punkt) HIPCK(c, hipMemcpyAsync(punkt, pd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (normale) HIPCK(c, hipMemcpyAsync(normale, nd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (dreieck) HIPCK(c, hipMemcpyAsync(dreieck, td, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (region) HIPCK(c, hipMemcpyAsync(region, rd, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+int gc_netz_durchstoss(gc_ctx* c, int32_t k, int32_t n, const double* a, const double* b, const double starr[3], int32_t* da,
+                       double* p, double* normale, double* vk) {
+  Netzhost* h = netz_von(c, k);
+  if (!h) return 1;
+  if (n <= 0) return 0;
+  const size_t m = (size_t)n;
+  NetzAbfrage q;
+  double *ad = q.neu<double>(3 * m), *bd = q.neu<double>(3 * m), *pd = q.neu<double>(3 * m), *nd = q.neu<double>(3 * m),
+         *vd = q.neu<double>(3 * m);
+  int32_t* dd = q.neu<int32_t>(m);
+  if (!ad || !bd || !pd || !nd || !vd || !dd) return fehler(c, "hipMalloc failed (mesh query)");
+  hipStream_t st = c->stream;
+  HIPCK(c, hipMemcpyAsync(ad, a, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCK(c, hipMemcpyAsync(bd, b, 3 * m * sizeof(double), hipMemcpyHostToDevice, st));
+  const V3<double> sv = {starr[0], starr[1], starr[2]};
+  hipLaunchKernelGGL(k_nk_durchstoss, dim3((unsigned)std::min<size_t>(4096, (m + 255) / 256)), dim3(256), 0, st,
+                     (const NetzDev*)h->dev, n, ad, bd, sv, dd, pd, nd, vd);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipMemcpyAsync(da, dd, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (p) HIPCK(c, hipMemcpyAsync(p, pd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (normale) HIPCK(c, hipMemcpyAsync(normale, nd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (vk) HIPCK(c, hipMemcpyAsync(vk, vd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipStreamSynchronize(st));
+  return 0;
 }
 
 int gc_set_koerper_lage(gc_ctx* c, int32_t k, const double lage[3]) {

@@ -15,7 +15,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib as L
-from .szene import GitterSdf, Szene, Zylinder
+from .szene import GitterSdf, Netz, Szene, Zylinder
 
 
 @dataclass
@@ -163,11 +163,70 @@ class GeraetSzene:
             dims = np.array(kb.dims, dtype=np.int32)
             i = self.lib.gc_add_sdf(self.h, _p(w, L.F32), _p(dims, L.I32), _p(_f64(kb.origin)), kb.spacing,
                                     _p(_f64(kb.versatz)), _p(_f64(kb.geschwindigkeit)))
+        elif isinstance(kb, Netz):
+            pk = _f64(kb.punkte).reshape(-1, 3)
+            dk = np.ascontiguousarray(kb.dreiecke, dtype=np.uint32).reshape(-1, 3)
+            i = self.lib.gc_add_netz(self.h, _p(pk), pk.shape[0], _p(dk, L.U32), dk.shape[0], kb.suchtiefe,
+                                     _p(_f64(kb.versatz)), _p(_f64(kb.geschwindigkeit)))
+            if i >= 0 and kb.punktgeschwindigkeit is not None:
+                self.netz_punktgeschwindigkeit(i, kb.punktgeschwindigkeit)
         else:
             raise TypeError(f"unknown body {type(kb)}")
         if i < 0:
             raise L.GcFehler("add body: " + self.lib.gc_last_error(self.h).decode())
         return i
+
+    # -- mesh bodies (netzkoerper.rs) ---------------------------------------
+    def netz_nachfuehren(self, k: int, punkte) -> None:
+        self._ck(self.lib.gc_netz_nachfuehren(self.h, k, _p(_f64(punkte))), "gc_netz_nachfuehren")
+
+    def netz_punktgeschwindigkeit(self, k: int, v) -> None:
+        if v is None:
+            self._ck(self.lib.gc_netz_punktgeschwindigkeit(self.h, k, None, 0), "gc_netz_punktgeschwindigkeit")
+        else:
+            a = _f64(v).reshape(-1, 3)
+            self._ck(self.lib.gc_netz_punktgeschwindigkeit(self.h, k, _p(a), a.shape[0]), "gc_netz_punktgeschwindigkeit")
+
+    def netz_info(self, k: int) -> dict:
+        a = L.NetzAngaben_c()
+        self._ck(self.lib.gc_netz_info(self.h, k, C.byref(a)), "gc_netz_info")
+        return {"n_punkte": a.n_punkte, "n_dreiecke": a.n_dreiecke, "randkanten": a.randkanten, "volumen": a.volumen,
+                "ursprung": np.array(a.ursprung[:]), "zelle": a.zelle, "zahl": np.array(a.zahl[:], dtype=np.int64),
+                "n_eintraege": a.n_eintraege}
+
+    def netz_lesen(self, k: int) -> dict:
+        i = self.netz_info(k)
+        n, nt = i["n_punkte"], i["n_dreiecke"]
+        nz = int(np.prod(i["zahl"]))
+        fn, pn = np.zeros((nt, 3)), np.zeros((n, 3))
+        nb = np.zeros((nt, 3), dtype=np.uint32)
+        kg = np.zeros((nt, 4))
+        an = np.zeros(nz + 1, dtype=np.uint32)
+        ei = np.zeros(max(1, i["n_eintraege"]), dtype=np.uint32)
+        self._ck(self.lib.gc_netz_lesen(self.h, k, _p(fn), _p(pn), _p(nb, L.U32), _p(kg), _p(an, L.U32), _p(ei, L.U32)),
+                 "gc_netz_lesen")
+        return {"flaechennormale": fn, "punktnormale": pn, "nachbar": nb, "kugel": kg, "anfang": an,
+                "eintraege": ei[:i["n_eintraege"]]}
+
+    def netz_naechster(self, k: int, x, radius: float) -> dict:
+        x = _f64(x).reshape(-1, 3)
+        m = x.shape[0]
+        da = np.zeros(m, dtype=np.int32)
+        ab, pu, no = np.zeros(m), np.zeros((m, 3)), np.zeros((m, 3))
+        dr, rg = np.zeros(m, dtype=np.int32), np.zeros(m, dtype=np.int32)
+        self._ck(self.lib.gc_netz_naechster(self.h, k, m, _p(x), radius, _p(da, L.I32), _p(ab), _p(pu), _p(no),
+                                            _p(dr, L.I32), _p(rg, L.I32)), "gc_netz_naechster")
+        return {"da": da.astype(bool), "abstand": ab, "punkt": pu, "normale": no, "dreieck": dr, "region": rg}
+
+    def netz_durchstoss(self, k: int, a, b, starr=(0.0, 0.0, 0.0)) -> dict:
+        a = _f64(a).reshape(-1, 3)
+        b = _f64(b).reshape(-1, 3)
+        m = a.shape[0]
+        da = np.zeros(m, dtype=np.int32)
+        p, no, vk = np.zeros((m, 3)), np.zeros((m, 3)), np.zeros((m, 3))
+        self._ck(self.lib.gc_netz_durchstoss(self.h, k, m, _p(a), _p(b), _p(_f64(starr)), _p(da, L.I32), _p(p), _p(no),
+                                             _p(vk)), "gc_netz_durchstoss")
+        return {"da": da.astype(bool), "p": p, "normale": no, "vk": vk}
 
     def set_state(self, x=None, v=None) -> None:
         self._ck(self.lib.gc_set_state(self.h, _p(_f64(x)) if x is not None else None,

@@ -154,8 +154,9 @@ int gc_set_naht_ruhe(gc_ctx* ctx, const double* ruhe);
 /* Held points (w = 0) are set by callers that clamp (tests/kontinuum.rs:382-386). */
 int gc_set_w(gc_ctx* ctx, const double* w);
 
-/* Bodies (koerper.rs:16-26). A sampled grid SDF stands where the reference has
- * a triangle-mesh body; it answers `Koerper::kontakt` (koerper.rs:69-91).
+/* Bodies (koerper.rs:16-26): the cylinder, the triangle-mesh body
+ * (gc_add_netz below) and a sampled grid SDF, which answers
+ * `Koerper::kontakt` (koerper.rs:69-91) like the mesh does.
  * werte: float[nx*ny*nz], x fastest. Returns the body index or -1. */
 int gc_add_zylinder(gc_ctx* ctx, const double punkt[3], const double achse[3], double radius, const double geschwindigkeit[3]);
 int gc_add_sdf(gc_ctx* ctx, const float* werte, const int32_t dims[3], const double origin[3], double spacing,
@@ -168,6 +169,51 @@ int gc_set_koerper_lage(gc_ctx* ctx, int32_t koerper, const double lage[3]);
  * outlive this one. No counterpart in the reference, which has one scene per
  * solver. Returns the body index or -1. */
 int gc_add_sdf_von(gc_ctx* ctx, const gc_ctx* quelle, int32_t k, const double versatz[3], const double geschwindigkeit[3]);
+
+/* A closed, consistently oriented triangle mesh as body: `Koerper::Netz`
+ * (koerper.rs:25-26) over `Koerpernetz::neu` (netzkoerper.rs:297-388).
+ * punkte double[3n] (m), dreiecke uint32[3m] with outward normals, suchtiefe
+ * (netzkoerper.rs:46, 10 mm in the reference) how deep inside a point is still
+ * found. Rejected like `neu` rejects: no triangles, suchtiefe <= 0, an index
+ * out of range, an edge of three or more triangles, an inconsistently oriented
+ * edge, volume <= 0. Contacts are `Koerper::kontakt` (koerper.rs:82-90), path
+ * entries `Koerpernetz::durchstoss` (netzkoerper.rs:541-596). The body point
+ * pass (`punkte_gegen_dreiecke`, netzkoerper.rs:625-665; off by default,
+ * implizit.rs:523) is not built. Returns the body index or -1. */
+int gc_add_netz(gc_ctx* ctx, const double* punkte, int32_t n_punkte, const uint32_t* dreiecke, int32_t n_dreiecke,
+                double suchtiefe, const double versatz[3], const double geschwindigkeit[3]);
+/* `Koerpernetz::nachfuehren` (netzkoerper.rs:390-418): a new pose of the same
+ * points; normals, spheres and grid are rebuilt on the device, the grid never
+ * shrinks, a velocity field is cleared. */
+int gc_netz_nachfuehren(gc_ctx* ctx, int32_t koerper, const double* punkte);
+/* `Koerpernetz::setze_punktgeschwindigkeit` (netzkoerper.rs:454-466): one
+ * velocity per point (double[3n]); n = 0 clears it (one velocity for the body). */
+int gc_netz_punktgeschwindigkeit(gc_ctx* ctx, int32_t koerper, const double* v, int32_t n);
+typedef struct gc_netz_angaben {
+  int32_t n_punkte, n_dreiecke;
+  int64_t randkanten;   /* edges with one triangle (open borders) */
+  double volumen;       /* enclosed volume, m^3 */
+  double ursprung[3];   /* grid origin */
+  double zelle;         /* cell edge, 2 x suchtiefe */
+  int64_t zahl[3];      /* cells per axis */
+  int64_t n_eintraege;  /* triangle entries over all cells */
+} gc_netz_angaben;
+int gc_netz_info(gc_ctx* ctx, int32_t koerper, gc_netz_angaben* out);
+/* The device arrays (each may be NULL): face normals double[3m], point normals
+ * double[3n], neighbour per side uint32[3m], spheres double[4m] (centroid,
+ * radius), grid starts uint32[cells + 1], grid entries uint32[n_eintraege]. */
+int gc_netz_lesen(gc_ctx* ctx, int32_t koerper, double* flaechennormale, double* punktnormale, uint32_t* nachbar,
+                  double* kugel, uint32_t* anfang, uint32_t* eintraege);
+/* `Koerpernetz::naechster` (netzkoerper.rs:503-531) for n points in the
+ * mesh's frame: da[i] = 1 if a surface point lies closer than radius; then the
+ * signed distance, nearest point, outward normal, triangle and region
+ * (0..2 corner k, 3..5 edge k, 6 face). Output arrays other than da may be NULL. */
+int gc_netz_naechster(gc_ctx* ctx, int32_t koerper, int32_t n, const double* x, double radius, int32_t* da, double* abstand,
+                      double* punkt, double* normale, int32_t* dreieck, int32_t* region);
+/* `Koerpernetz::durchstoss` (netzkoerper.rs:541-596) for n segments a -> b in
+ * the mesh's frame: entry point, face normal, surface velocity there. */
+int gc_netz_durchstoss(gc_ctx* ctx, int32_t koerper, int32_t n, const double* a, const double* b, const double starr[3],
+                       int32_t* da, double* p, double* normale, double* vk);
 
 /* State: positions and velocities (Szene::x, Szene::v). */
 int gc_set_state(gc_ctx* ctx, const double* x, const double* v);

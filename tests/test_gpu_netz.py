@@ -1,0 +1,253 @@
+"""The mesh body on the device (gc_add_netz, netzkoerper.rs) against the
+oracle's restatement of `Koerpernetz`.
+
+* The per-pose build: face normals, spheres, neighbours, the grid's starts
+  and entries equal to the bit; point normals (the device's atan2) to 1e-15.
+* The queries: `naechster` on the reference's own test points (the cube's
+  20,000 xorshift points, the cylinder's 20,000 points of
+  `gitter_wie_alle_dreiecke`) and `durchstoss` on random segments, equal to
+  the bit with the same triangle and region; the cube's distance also equals
+  the analytic signed distance to 1e-12 (netzkoerper.rs:1000-1027).
+* A new pose (`nachfuehren`): the grown grid as the oracle's.
+* Whole steps on a drape falling onto the two-mesh body: contacts, releases,
+  sticking and path entries equal per step, positions to 1e-9 m (FP64) and
+  1e-4 m (FP32) from the oracle's state.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from garmentcraft_amd import szenen
+from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
+from garmentcraft_amd.szene import Netz
+
+from test_orakel_netzkoerper import abstand_wuerfel, wuerfel, zufall
+
+pytestmark = pytest.mark.gpu
+
+H = 1.0 / 120.0
+P = dict(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6)
+
+
+def _traeger(koerper):
+    """A device context that carries only bodies (a one-point scene)."""
+    s, _ = szenen.drape(szenen.DrapeKonfig(spalten=1, zeilen=1, zelle_mm=40.0, hoehe_m=2.0), mit_koerper=False)
+    s.koerper = list(koerper)
+    return GeraetSzene(s, ImplizitParameter(**P), praezision="fp64")
+
+
+def _netze():
+    p, d = wuerfel()
+    zp, zd = szenen.netz_zylinder(0.09, -0.3, 0.1, 96, 20)
+    ell = szenen.koerper_netz(kante=0.015)
+    return [("wuerfel", p, d, 0.5), ("zylinder", zp, zd, 0.01), ("rumpf", ell[0].punkte, ell[0].dreiecke, 0.01)]
+
+
+@pytest.mark.parametrize("name,p,d,tiefe", _netze(), ids=[n[0] for n in _netze()])
+def test_aufbau_wie_orakel(name, p, d, tiefe):
+    o = O.OrakelNetz(p, d, tiefe)
+    g = _traeger([Netz(p, d, tiefe)])
+    a, b = o.normalen(), g.netz_lesen(0)
+    for k in ("flaechennormale", "nachbar", "kugel"):
+        assert np.array_equal(a[k], b[k]), (name, k)
+    dp = float(np.max(np.abs(a["punktnormale"] - b["punktnormale"])))
+    print(name, "point normals, largest difference", dp)
+    assert dp <= 1e-15
+    io, ig = o.info(), g.netz_info(0)
+    assert io["randkanten"] == ig["randkanten"] and io["volumen"] == ig["volumen"]
+    assert np.array_equal(io["ursprung"], ig["ursprung"]) and io["zelle"] == ig["zelle"]
+    assert np.array_equal(io["zahl"], ig["zahl"])
+    an, ei = o.gitter(False)
+    assert np.array_equal(an, b["anfang"]) and np.array_equal(ei, b["eintraege"]), name
+    print(name, "cells", int(np.prod(ig["zahl"])), "entries", ig["n_eintraege"])
+    g.close()
+
+
+def _vergleich(o_erg, g_erg, i):
+    if o_erg is None:
+        assert not g_erg["da"][i], i
+        return 0
+    assert g_erg["da"][i], i
+    assert g_erg["abstand"][i] == o_erg["abstand"], (i, g_erg["abstand"][i], o_erg["abstand"])
+    assert np.array_equal(g_erg["punkt"][i], o_erg["punkt"]), i
+    assert np.array_equal(g_erg["normale"][i], o_erg["normale"]), i
+    assert g_erg["dreieck"][i] == o_erg["dreieck"] and g_erg["region"][i] == o_erg["region"], i
+    return 1
+
+
+def test_naechster_wuerfel_wie_orakel_und_geschlossen():  # netzkoerper.rs:1000-1027
+    p, d = wuerfel()
+    o = O.OrakelNetz(p, d, 0.5)
+    g = _traeger([Netz(p, d, 0.5)])
+    s = [0x123456789ABCDEF1]
+    x = np.array([[3.0 * zufall(s) - 1.5 for _ in range(3)] for _ in range(20000)])
+    # radius larger than the box's diagonal: every point finds its nearest
+    erg = g.netz_naechster(0, x, 10.0)
+    assert erg["da"].all()
+    for i in range(x.shape[0]):
+        _vergleich(o.naechster(x[i], 10.0), erg, i)
+        assert abs(erg["abstand"][i] - abstand_wuerfel(x[i], 1.0)) < 1e-12, i
+    g.close()
+
+
+def test_naechster_zylinder_wie_orakel():  # netzkoerper.rs:1029-1055
+    p, d = szenen.netz_zylinder(0.09, -0.3, 0.1, 96, 20)
+    o = O.OrakelNetz(p, d, 0.01)
+    g = _traeger([Netz(p, d, 0.01)])
+    s = [0xFEEDBEEFDEADCAFE]
+    x = []
+    for _ in range(20000):
+        r = 0.07 + 0.04 * zufall(s)
+        w = 2.0 * math.pi * zufall(s)
+        x.append([r * math.cos(w), r * math.sin(w), -0.32 + 0.44 * zufall(s)])
+    x = np.array(x)
+    erg = g.netz_naechster(0, x, 0.01)
+    gefunden = sum(_vergleich(o.naechster(x[i], 0.01), erg, i) for i in range(x.shape[0]))
+    innen = int(np.sum(erg["da"] & (erg["abstand"] < 0.0)))
+    print("found", gefunden, "inside", innen, "of", x.shape[0])
+    assert innen > 1000 and gefunden - innen > 1000 and x.shape[0] - gefunden > 1000
+    g.close()
+
+
+def test_durchstoss_wie_orakel():  # netzkoerper.rs:1057-1077 and random paths
+    p, d = szenen.netz_zylinder(0.09, -0.3, 0.1, 96, 20)
+    o = O.OrakelNetz(p, d, 0.01)
+    g = _traeger([Netz(p, d, 0.01)])
+    rng = np.random.default_rng(11)
+    a = rng.uniform([-0.15, -0.15, -0.35], [0.15, 0.15, 0.15], (4000, 3))
+    b = a + rng.normal(scale=0.05, size=(4000, 3))
+    starr = np.array([0.1, -0.2, 0.05])
+    erg = g.netz_durchstoss(0, a, b, starr)
+    treffer = 0
+    for i in range(a.shape[0]):
+        r = o.durchstoss(a[i], b[i], starr)
+        if r is None:
+            assert not erg["da"][i], i
+            continue
+        treffer += 1
+        assert erg["da"][i], i
+        assert np.array_equal(erg["p"][i], r[0]) and np.array_equal(erg["normale"][i], r[1]), i
+        assert np.array_equal(erg["vk"][i], r[2]), i
+    print("entries", treffer, "of", a.shape[0])
+    assert treffer > 100
+    # the reference's cases on the cube
+    pw, dw = wuerfel()
+    gw = _traeger([Netz(pw, dw, 0.5)])
+    e = gw.netz_durchstoss(0, [[0.2, 0.3, 3.0], [0.2, 0.3, 0.0], [0.2, 0.3, 3.0], [0.5, 0.5, 2.0]],
+                           [[0.2, 0.3, -3.0], [0.2, 0.3, -3.0], [0.2, 0.3, 1.5], [0.5, 0.5, 0.0]])
+    assert list(e["da"]) == [True, False, False, True]
+    assert np.linalg.norm(e["p"][0] - [0.2, 0.3, 1.0]) < 1e-12 and abs(e["normale"][0][2] - 1.0) < 1e-12
+    assert abs(e["p"][3][2] - 1.0) < 1e-12
+    g.close()
+    gw.close()
+
+
+def test_nachfuehren_wie_orakel():
+    """netzkoerper.rs:390-418: a moved pose rebuilds the arrays, the grid grows."""
+    p, d = szenen.netz_zylinder(0.15, -0.4, 0.4, 24, 8)
+    o = O.OrakelNetz(p, d, 0.01)
+    g = _traeger([Netz(p, d, 0.01)])
+    q = p.copy()
+    q[:, 0] += 0.3
+    q[:, 2] *= 0.7
+    o.nachfuehren(q)
+    g.netz_nachfuehren(0, q)
+    a, b = o.normalen(), g.netz_lesen(0)
+    for k in ("flaechennormale", "kugel"):
+        assert np.array_equal(a[k], b[k]), k
+    io, ig = o.info(), g.netz_info(0)
+    assert np.array_equal(io["ursprung"], ig["ursprung"]) and np.array_equal(io["zahl"], ig["zahl"])
+    an, ei = o.gitter(False)
+    assert np.array_equal(an, b["anfang"]) and np.array_equal(ei, b["eintraege"])
+    g.close()
+
+
+def test_fehlerhafte_netze_abgelehnt():  # netzkoerper.rs:1219-1232
+    from garmentcraft_amd._lib import GcFehler
+    p, d = wuerfel()
+    g = _traeger([])
+    d2 = d.copy()
+    d2[0] = [d[0, 0], d[0, 2], d[0, 1]]
+    for dd in (d2, d[:, [0, 2, 1]], np.vstack([d, d[:1]])):
+        with pytest.raises(GcFehler):
+            g.add_koerper(Netz(p, dd, 0.1))
+    with pytest.raises(GcFehler):
+        g.add_koerper(Netz(p, d, 0.0))
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def drape_auf_netz():
+    cfg = szenen.DrapeKonfig(spalten=2, zeilen=2, zelle_mm=160.0, hoehe_m=0.335, jitter_mm=10.0, versatz_mm=(0.0, 0.0))
+    s, _ = szenen.drape(cfg, mit_koerper=False)
+    s.koerper = szenen.koerper_netz(kante=0.015)
+    return s
+
+
+@pytest.mark.parametrize("praezision,rtol,tol_x", [("fp64", 1e-10, 1e-9), ("fp32", 1e-6, 1e-4)])
+def test_drape_auf_netz_einzelschritte_wie_orakel(drape_auf_netz, praezision, rtol, tol_x):
+    """Each step from the oracle's state on the two-mesh body: contacts from
+    the mesh query (koerper.rs:82-90), releases, sticking points and positions."""
+    s = drape_auf_netz
+    pp = dict(P, cg_toleranz=rtol)
+    o = O.OrakelSim(s, O.param(**pp))
+    g = GeraetSzene(s, ImplizitParameter(**pp), praezision=praezision)
+    kontakt_schritte = 0
+    schlechteste = 0.0
+    for k in range(30):
+        x, v = o.lage()
+        g.set_state(x, v)
+        g.set_dv_vorher(o.dv_vorher())
+        vor = o.statistik()
+        g.statistik(reset=True)
+        o.schritt(H)
+        g.schritte(H, 1)
+        nach, sg = o.statistik(), g.statistik()
+        assert np.array_equal(o.sys_fest()["flag"], g.fest()["flag"]), k
+        for name in ("kontakte", "haftend", "freigegeben", "durchstoesse"):
+            assert nach[name] - vor[name] == sg[name], (k, name)
+        kontakt_schritte += (nach["kontakte"] - vor["kontakte"]) > 0
+        xa, _ = o.lage()
+        xb, _ = g.lage()
+        d = float(np.max(np.linalg.norm(xa - xb, axis=1)))
+        schlechteste = max(schlechteste, d)
+        assert d <= tol_x, (k, d)
+    print(praezision, "worst single-step difference", schlechteste, "steps with contact", kontakt_schritte)
+    assert kontakt_schritte >= 10
+    g.close()
+
+
+def test_drape_auf_netz_durchstoss_wie_orakel(drape_auf_netz):
+    """The cloth thrown at the mesh body at 3 m/s: path entries by
+    Moeller-Trumbore (netzkoerper.rs:541-596) set onto the skin
+    (implizit.rs:984-1003), FP64, equal counts, positions to 1e-9 m."""
+    s = drape_auf_netz
+    pp = dict(P, cg_toleranz=1e-10)
+    o = O.OrakelSim(s, O.param(**pp))
+    for _ in range(8):
+        o.schritt(H)
+    x, v = o.lage()
+    v = v.copy()
+    v[:, 2] -= 3.0
+    o.setze_lage(x, v)
+    g = GeraetSzene(s, ImplizitParameter(**pp), praezision="fp64")
+    summe = 0
+    for k in range(6):
+        x, v = o.lage()
+        g.set_state(x, v)
+        g.set_dv_vorher(o.dv_vorher())
+        vor = o.statistik()
+        g.statistik(reset=True)
+        o.schritt(H)
+        g.schritte(H, 1)
+        nach = o.statistik()
+        assert nach["durchstoesse"] - vor["durchstoesse"] == g.statistik()["durchstoesse"], k
+        summe += nach["durchstoesse"] - vor["durchstoesse"]
+        xa, _ = o.lage()
+        xb, _ = g.lage()
+        assert float(np.max(np.linalg.norm(xa - xb, axis=1))) <= 1e-9, k
+    print("entries", summe)
+    assert summe >= 5
+    g.close()
