@@ -164,6 +164,7 @@ SIGNATUREN = [
     ("gc_netz_lesen", C.c_int, [C.c_void_p, C.c_int32, D, D, U32, D, U32, U32]),
     ("gc_netz_naechster", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, D, C.c_double, I32, D, D, D, I32, I32]),
     ("gc_netz_durchstoss", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, D, D, D, I32, D, D, D]),
+    ("gc_netz_sdf_backen", C.c_int, [C.c_void_p, C.c_int32, I32, D, C.c_double, C.c_double, F32]),
 ]
 
 _lib = None

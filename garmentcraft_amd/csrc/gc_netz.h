@@ -466,3 +466,95 @@ __global__ void k_nk_durchstoss(const NetzDev* __restrict__ gp, int n, const dou
     vk[3 * (int64_t)i + 2] = v.z;
   }
 }
+
+namespace nk {
+
+// netzkoerper.rs:770-782: the nearest surface point over all triangles.
+GC_HD Treffer naechster_alle(const NetzDev& N, V3<double> x) {
+  double bl = INFINITY;
+  int64_t bt = 0;
+  V3<double> bq = zero3<double>();
+  int breg = NK_FLAECHE;
+  for (int64_t t = 0; t < N.n_dreiecke; ++t) {
+    const uint32_t* d = N.dreiecke + 3 * t;
+    int reg;
+    const V3<double> q = naechster_auf_dreieck(x, lade3(N.punkte, d[0]), lade3(N.punkte, d[1]), lade3(N.punkte, d[2]), &reg);
+    const double l = norm(x - q);
+    if (l < bl) {
+      bl = l;
+      bt = t;
+      bq = q;
+      breg = reg;
+    }
+  }
+  return ergebnis(N, x, bt, bq, breg);
+}
+
+// netzkoerper.rs:741-767: the signed distance without a bound; the search
+// cube doubles until it holds a triangle, the full scan answers beyond the grid.
+GC_HD Treffer abstand_ohne_grenze(const NetzDev& N, V3<double> x) {
+  const double xs[3] = {x.x, x.y, x.z};
+  double quadrat = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double e = N.ursprung[k] + N.zelle * (double)N.zahl[k];
+    const double d = fmax(fabs(xs[k] - N.ursprung[k]), fabs(xs[k] - e));
+    quadrat += d * d;
+  }
+  const double weit = sqrt(quadrat) + N.zelle;
+  double r = fmax(N.suchtiefe, N.zelle);
+  for (;;) {
+    int64_t rr[3][2];
+    const V3<double> rv = {r, r, r};
+    if (bereich(N, x - rv, x + rv, rr)) {
+      Treffer t;
+      if (naechster(N, x, r, &t)) return t;
+    }
+    if (r >= weit) return naechster_alle(N, x);
+    r = fmin(r * 2.0, weit);
+  }
+}
+
+}  // namespace nk
+
+// The mesh's signed distance on a regular grid (float32, x fastest): exact at
+// every node closer than `band` to the surface; beyond it +-band with the
+// node's side. Pass 1, one thread per node: the bounded query.
+__global__ void k_nk_backen_band(const NetzDev* __restrict__ gp, V3<double> origin, double spacing, int nx, int ny,
+                                 int64_t anzahl, double band, float* __restrict__ werte) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < anzahl; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ix = q % nx, iy = (q / nx) % ny, iz = q / ((int64_t)nx * ny);
+    const V3<double> x = {origin.x + spacing * (double)ix, origin.y + spacing * (double)iy, origin.z + spacing * (double)iz};
+    nk::Treffer t;
+    werte[q] = nk::naechster(*gp, x, band, &t) ? (float)t.abstand : NAN;
+  }
+}
+
+// Pass 2, one thread per x-line: the side of every node beyond the band. With
+// band >= spacing no surface crossing lies between two neighbouring nodes of
+// which one is beyond the band (the crossing would be within one spacing of
+// it), so a node beyond the band is on the side of its neighbour. A line's
+// first node beyond the band takes its side from outside the mesh's box (+),
+// or from the unbounded query.
+__global__ void k_nk_backen_seiten(const NetzDev* __restrict__ gp, V3<double> origin, double spacing, int nx, int ny, int nz,
+                                   double band, V3<double> huelle_lo, V3<double> huelle_hi, float* __restrict__ werte) {
+  const int64_t linien = (int64_t)ny * nz;
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < linien; l += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t iy = l % ny, iz = l / ny;
+    float* w = werte + l * nx;
+    double seite = 0.0;  // unknown
+    for (int ix = 0; ix < nx; ++ix) {
+      const float v = w[ix];
+      if (v == v) {
+        seite = v < 0.0f ? -1.0 : 1.0;
+        continue;
+      }
+      if (seite == 0.0) {
+        const V3<double> x = {origin.x + spacing * (double)ix, origin.y + spacing * (double)iy, origin.z + spacing * (double)iz};
+        const bool draussen = x.x < huelle_lo.x || x.y < huelle_lo.y || x.z < huelle_lo.z || x.x > huelle_hi.x ||
+                              x.y > huelle_hi.y || x.z > huelle_hi.z;
+        seite = draussen ? 1.0 : (nk::abstand_ohne_grenze(*gp, x).abstand < 0.0 ? -1.0 : 1.0);
+      }
+      w[ix] = (float)(seite * band);
+    }
+  }
+}

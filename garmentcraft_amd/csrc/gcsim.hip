@@ -36,6 +36,7 @@ struct Netzhost {
   int64_t randkanten = 0;
   double volumen = 0.0;
   bool hat_gitter = false;
+  double huelle_lo[3] = {0, 0, 0}, huelle_hi[3] = {0, 0, 0};  // the pose's point box
   NetzDev bild{};  // host image of the device struct
   double *punkte_d = nullptr, *fn_d = nullptr, *pn_d = nullptr, *kugel_d = nullptr, *pv_d = nullptr;
   uint32_t *dreiecke_d = nullptr, *nachbar_d = nullptr, *inz_anfang_d = nullptr, *inz_d = nullptr;
@@ -2574,6 +2575,10 @@ static void netz_raster(Netzhost& h) {
       lo[a] = std::fmin(lo[a], h.punkte[3 * i + a]);
       hi[a] = std::fmax(hi[a], h.punkte[3 * i + a]);
     }
+  for (int a = 0; a < 3; ++a) {
+    h.huelle_lo[a] = lo[a];
+    h.huelle_hi[a] = hi[a];
+  }
   const double zelle = 2.0 * h.suchtiefe;
   double ursprung[3], ende[3];
   for (int a = 0; a < 3; ++a) {
@@ -2868,6 +2873,31 @@ int gc_netz_durchstoss(gc_ctx* c, int32_t k, int32_t n, const double* a, const d
   if (p) HIPCK(c, hipMemcpyAsync(p, pd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
   if (normale) HIPCK(c, hipMemcpyAsync(normale, nd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
   if (vk) HIPCK(c, hipMemcpyAsync(vk, vd, 3 * m * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+int gc_netz_sdf_backen(gc_ctx* c, int32_t k, const int32_t dims[3], const double origin[3], double spacing, double band,
+                       float* werte) {
+  Netzhost* h = netz_von(c, k);
+  if (!h) return 1;
+  if (dims[0] < 2 || dims[1] < 2 || dims[2] < 2 || !(spacing > 0.0))
+    return fehler(c, "gc_netz_sdf_backen: at least 2 nodes per axis and a positive spacing");
+  if (!(band >= spacing)) return fehler(c, "gc_netz_sdf_backen: the band must be at least one spacing wide");
+  const int64_t anzahl = (int64_t)dims[0] * dims[1] * dims[2];
+  NetzAbfrage q;
+  float* wd = q.neu<float>((size_t)anzahl);
+  if (!wd) return fehler(c, "hipMalloc failed (SDF bake)");
+  hipStream_t st = c->stream;
+  const V3<double> o = {origin[0], origin[1], origin[2]};
+  hipLaunchKernelGGL(k_nk_backen_band, dim3((unsigned)std::min<int64_t>(65536, (anzahl + 255) / 256)), dim3(256), 0, st,
+                     (const NetzDev*)h->dev, o, spacing, dims[0], dims[1], anzahl, band, wd);
+  const int64_t linien = (int64_t)dims[1] * dims[2];
+  const V3<double> lo = {h->huelle_lo[0], h->huelle_lo[1], h->huelle_lo[2]}, hi = {h->huelle_hi[0], h->huelle_hi[1], h->huelle_hi[2]};
+  hipLaunchKernelGGL(k_nk_backen_seiten, dim3((unsigned)std::min<int64_t>(65536, (linien + 127) / 128)), dim3(128), 0, st,
+                     (const NetzDev*)h->dev, o, spacing, dims[0], dims[1], dims[2], band, lo, hi, wd);
+  HIPCK(c, hipGetLastError());
+  HIPCK(c, hipMemcpyAsync(werte, wd, (size_t)anzahl * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCK(c, hipStreamSynchronize(st));
   return 0;
 }

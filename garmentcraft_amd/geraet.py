@@ -228,6 +228,16 @@ class GeraetSzene:
                                              _p(vk)), "gc_netz_durchstoss")
         return {"da": da.astype(bool), "p": p, "normale": no, "vk": vk}
 
+    def netz_sdf_backen(self, k: int, dims, origin, spacing: float, band: float) -> GitterSdf:
+        """The mesh body k's signed distance on a node grid of dims (nx, ny,
+        nz), exact within ``band`` of the surface, as a grid-SDF body."""
+        nx, ny, nz = (int(a) for a in dims)
+        w = np.zeros((nz, ny, nx), dtype=np.float32)
+        d = np.array([nx, ny, nz], dtype=np.int32)
+        self._ck(self.lib.gc_netz_sdf_backen(self.h, k, _p(d, L.I32), _p(_f64(origin)), spacing, band, _p(w, L.F32)),
+                 "gc_netz_sdf_backen")
+        return GitterSdf(werte=w, origin=tuple(float(a) for a in origin), spacing=spacing)
+
     def set_state(self, x=None, v=None) -> None:
         self._ck(self.lib.gc_set_state(self.h, _p(_f64(x)) if x is not None else None,
                                        _p(_f64(v)) if v is not None else None), "gc_set_state")

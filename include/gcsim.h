@@ -214,6 +214,14 @@ int gc_netz_naechster(gc_ctx* ctx, int32_t koerper, int32_t n, const double* x, 
  * the mesh's frame: entry point, face normal, surface velocity there. */
 int gc_netz_durchstoss(gc_ctx* ctx, int32_t koerper, int32_t n, const double* a, const double* b, const double starr[3],
                        int32_t* da, double* p, double* normale, double* vk);
+/* The mesh's signed distance on a node grid (werte float[nx*ny*nz], x fastest,
+ * node (i,j,k) at origin + spacing*(i,j,k), in the mesh's frame), for a
+ * gc_add_sdf body. Exact (`naechster`, sign by the pseudonormal) at nodes closer
+ * than `band` to the surface, +-band beyond it with the node's side; band must
+ * be at least one spacing. No counterpart in the reference, whose mesh body is
+ * queried directly; it feeds the grid-SDF body from a mesh. */
+int gc_netz_sdf_backen(gc_ctx* ctx, int32_t koerper, const int32_t dims[3], const double origin[3], double spacing, double band,
+                       float* werte);
 
 /* State: positions and velocities (Szene::x, Szene::v). */
 int gc_set_state(gc_ctx* ctx, const double* x, const double* v);

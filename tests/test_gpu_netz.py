@@ -251,3 +251,36 @@ def test_drape_auf_netz_durchstoss_wie_orakel(drape_auf_netz):
     print("entries", summe)
     assert summe >= 5
     g.close()
+
+
+@pytest.mark.parametrize("ausschnitt", [False, True], ids=["ganz", "angeschnitten"])
+def test_sdf_backen_wie_orakel(ausschnitt):
+    """gc_netz_sdf_backen: nodes within the band carry the oracle's signed
+    distance (`naechster`, rounded to float32); nodes beyond it carry +-band
+    with the side of the oracle's unbounded query (netzkoerper.rs:741-767).
+    'angeschnitten' starts the grid inside the body, so lines begin inside."""
+    form = szenen.Koerperform()
+    ell = szenen.koerper_netz(form, kante=0.015)[0]
+    o = O.OrakelNetz(ell.punkte, ell.dreiecke, 0.01)
+    g = _traeger([ell])
+    sp, band = 0.02, 0.03
+    origin = np.array([-0.05, -0.05, -0.1]) if ausschnitt else np.array([-0.26, -0.2, -0.38])
+    dims = (14, 12, 18) if ausschnitt else (27, 21, 32)
+    sdf = g.netz_sdf_backen(0, dims, origin, sp, band)
+    w = sdf.werte
+    nb = innen = 0
+    for iz in range(dims[2]):
+        for iy in range(dims[1]):
+            for ix in range(dims[0]):
+                x = origin + sp * np.array([ix, iy, iz])
+                n = o.naechster(x, band)
+                if n is not None:
+                    nb += 1
+                    assert w[iz, iy, ix] == np.float32(n["abstand"]), (ix, iy, iz)
+                else:
+                    seite = 1.0 if o.abstand_ohne_grenze(x)["abstand"] >= 0.0 else -1.0
+                    innen += seite < 0
+                    assert w[iz, iy, ix] == np.float32(seite * band), (ix, iy, iz, w[iz, iy, ix])
+    print("nodes", w.size, "in the band", nb, "inside beyond it", innen)
+    assert nb > 100 and innen > 10
+    g.close()
