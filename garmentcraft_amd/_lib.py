@@ -99,6 +99,19 @@ class NetzAngaben_c(C.Structure):
                 ("ursprung", C.c_double * 3), ("zelle", C.c_double), ("zahl", C.c_int64 * 3), ("n_eintraege", C.c_int64)]
 
 
+class XpbdTopologie(C.Structure):
+    _fields_ = [
+        ("n_abstand", C.c_int32), ("abstand_punkte", U32), ("abstand_ruhe", D), ("abstand_nachgiebigkeit", D),
+        ("membran_kante", C.POINTER(C.c_uint8)),
+        ("n_naehte", C.c_int32), ("naht_punkte", U32), ("naht_ruhe", D), ("naht_nachgiebigkeit", C.c_double),
+        ("n_biegungen", C.c_int32), ("biegung_punkte", U32), ("biegung_ruhe", D), ("biegung_nachgiebigkeit", D),
+    ]
+
+
+class XpbdStatistik_c(C.Structure):
+    _fields_ = [("kontakte", C.c_int64), ("haftend", C.c_int64), ("tiefe", C.c_double), ("tangential", C.c_double)]
+
+
 # (name, restype, argtypes) for every entry point of include/gcsim.h
 SIGNATUREN = [
     ("gc_create", C.c_void_p, [C.POINTER(Optionen), C.c_char_p, C.c_int32]),
@@ -165,6 +178,9 @@ SIGNATUREN = [
     ("gc_netz_naechster", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, D, C.c_double, I32, D, D, D, I32, I32]),
     ("gc_netz_durchstoss", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, D, D, D, I32, D, D, D]),
     ("gc_netz_sdf_backen", C.c_int, [C.c_void_p, C.c_int32, I32, D, C.c_double, C.c_double, F32]),
+    ("gc_xpbd_topology", C.c_int, [C.c_void_p, C.POINTER(XpbdTopologie)]),
+    ("gc_xpbd_bild", C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.POINTER(XpbdStatistik_c)]),
+    ("gc_xpbd_laeufe", C.c_int32, [C.c_void_p, I32, I32, I32, I32]),
 ]
 
 _lib = None

@@ -9,6 +9,7 @@
 #include "gc_adjoint.h"
 #include "gc_selbst.h"
 #include "gc_naht.h"
+#include "gc_xpbd.h"
 
 #include <hip/hip_runtime.h>
 
@@ -54,6 +55,19 @@ struct Netzhost {
   }
 };
 
+// The XPBD path's own constraint lists (cpu.rs), cut into runs and levels
+// whose constraints share no point.
+struct XpbdHost {
+  bool bereit = false;
+  int na = 0, nn = 0, nb = 0, nm = 0;
+  std::vector<int> a_laeufe, n_laeufe, b_laeufe, m_stufen;  // run / level starts, last entry = count
+  uint32_t *a_p = nullptr, *n_p = nullptr, *b_p = nullptr, *m_reihe = nullptr;
+  double *a_ruhe = nullptr, *a_alpha = nullptr, *n_ruhe = nullptr, *b_ruhe = nullptr, *b_alpha = nullptr;
+  double n_alpha = 0.0;
+  V3<double> *delta = nullptr, *korr = nullptr, *last = nullptr;
+  XpStatDev* st = nullptr;
+};
+
 struct Koerperhost {
   Koerper k;
   float* werte_dev = nullptr;
@@ -83,6 +97,7 @@ struct gc_ctx {
   double msum = 0.0;
   gc_param param{};
   std::vector<Koerperhost> koerper;
+  XpbdHost xp;
   // device
   std::vector<void*> allocs;
   SysDev<double> sd{};
@@ -2899,6 +2914,195 @@ int gc_netz_sdf_backen(gc_ctx* c, int32_t k, const int32_t dims[3], const double
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipMemcpyAsync(werte, wd, (size_t)anzahl * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+// --- XPBD path (cpu.rs:76-246) ------------------------------------------------
+}  // extern "C"
+
+// Consecutive runs of a constraint list in which no two constraints share a
+// point: a run closes before the first constraint that touches a point of it.
+template <int N>
+static std::vector<int> laeufe(const uint32_t* p, int anzahl, int n) {
+  std::vector<int> start{0};
+  std::vector<int> marke((size_t)n, -1);
+  int lauf = 0;
+  for (int c = 0; c < anzahl; ++c) {
+    bool teilt = false;
+    for (int k = 0; k < N; ++k) teilt = teilt || marke[p[N * (size_t)c + k]] == lauf;
+    if (teilt) {
+      start.push_back(c);
+      ++lauf;
+    }
+    for (int k = 0; k < N; ++k) marke[p[N * (size_t)c + k]] = lauf;
+  }
+  start.push_back(anzahl);
+  return start;
+}
+
+template <typename T>
+static int xpbd_bild(gc_ctx* c, double bild_dt, int unterschritte, gc_xpbd_statistik* out) {
+  XpbdHost& x = c->xp;
+  SysDev<T>& s = sys<T>(c);
+  const int n = unterschritte > 1 ? unterschritte : 1;
+  const double h = bild_dt / (double)n, h2 = h * h;
+  const V3<double> g = {c->param.schwerkraft[0], c->param.schwerkraft[1], c->param.schwerkraft[2]};
+  const double daempf = fmax(1.0 - c->param.daempfung * h, 0.0);
+  hipStream_t st = c->stream;
+  const size_t np = (size_t)c->n;
+  HIPCK(c, hipMemsetAsync(x.delta, 0, np * sizeof(V3<double>), st));
+  HIPCK(c, hipMemsetAsync(x.korr, 0, np * sizeof(V3<double>), st));
+  HIPCK(c, hipMemsetAsync(x.last, 0, np * sizeof(V3<double>), st));
+  HIPCK(c, hipMemsetAsync(x.st, 0, sizeof(XpStatDev), st));
+  const unsigned gp = gitter(c->n);
+  auto zahl = [](int lo, int hi) { return dim3((unsigned)std::max(1, std::min(4096, (hi - lo + 255) / 256))); };
+  std::vector<Koerper> anfang;
+  for (auto& k : c->koerper) anfang.push_back(k.k);
+  for (int k = 0; k < n; ++k) {
+    const double weg = (double)(k + 1) * h;  // koerper.rs:122-133 from the frame start
+    for (size_t b = 0; b < c->koerper.size(); ++b)
+      for (int q = 0; q < 3; ++q) c->koerper[b].k.punkt[q] = anfang[b].punkt[q] + anfang[b].geschwindigkeit[q] * weg;
+    alle_param(c);
+    if (sys_hoch<T>(c)) return 1;
+    hipLaunchKernelGGL(k_xp_last_start, dim3(gp), dim3(256), 0, st, c->n, s.masse, g, x.last);
+    for (size_t r = 0; r + 1 < x.a_laeufe.size(); ++r) {
+      const int lo = x.a_laeufe[r], hi = x.a_laeufe[r + 1];
+      hipLaunchKernelGGL(k_xp_last_abstand, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.a_p, x.a_ruhe, x.a_alpha, s.x, x.delta, x.last);
+    }
+    for (size_t r = 0; r + 1 < x.m_stufen.size(); ++r) {
+      const int lo = x.m_stufen[r], hi = x.m_stufen[r + 1];
+      hipLaunchKernelGGL(k_xp_last_membran, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.m_reihe, s.m_p, s.m_el64, s.x, x.delta,
+                         x.korr, x.last);
+    }
+    hipLaunchKernelGGL(k_xp_vorhersage, dim3(gp), dim3(256), 0, st, c->n, s.w, g, h, daempf, s.v, x.delta, x.korr);
+    for (size_t r = 0; r + 1 < x.a_laeufe.size(); ++r) {
+      const int lo = x.a_laeufe[r], hi = x.a_laeufe[r + 1];
+      hipLaunchKernelGGL(k_xp_abstand, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.a_p, x.a_ruhe, (const double*)x.a_alpha, 0.0, h2,
+                         s.x, x.delta, x.korr, s.w);
+    }
+    for (size_t r = 0; r + 1 < x.m_stufen.size(); ++r) {
+      const int lo = x.m_stufen[r], hi = x.m_stufen[r + 1];
+      hipLaunchKernelGGL(k_xp_membran, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.m_reihe, s.m_p, s.m_el64, h2, s.x, x.delta, x.korr,
+                         s.w);
+    }
+    for (size_t r = 0; r + 1 < x.n_laeufe.size(); ++r) {
+      const int lo = x.n_laeufe[r], hi = x.n_laeufe[r + 1];
+      hipLaunchKernelGGL(k_xp_abstand, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.n_p, x.n_ruhe, (const double*)nullptr, x.n_alpha,
+                         h2, s.x, x.delta, x.korr, s.w);
+    }
+    for (size_t r = 0; r + 1 < x.b_laeufe.size(); ++r) {
+      const int lo = x.b_laeufe[r], hi = x.b_laeufe[r + 1];
+      hipLaunchKernelGGL(k_xp_biegung, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.b_p, x.b_ruhe, x.b_alpha, h2, s.x, x.delta, x.korr,
+                         s.w);
+    }
+    hipLaunchKernelGGL(k_xp_kontakt<T>, dim3(gp), dim3(256), 0, st, dsys<T>(c), h, h2, x.delta, x.korr, x.last, x.st);
+    hipLaunchKernelGGL(k_xp_ende, dim3(gp), dim3(256), 0, st, c->n, s.w, h, s.v, x.delta, x.korr);
+    HIPCK(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_xp_lage, dim3(gp), dim3(256), 0, st, c->n, s.x, x.delta);
+  HIPCK(c, hipGetLastError());
+  XpStatDev stat{};
+  HIPCK(c, hipMemcpyAsync(&stat, x.st, sizeof(XpStatDev), hipMemcpyDeviceToHost, st));
+  HIPCK(c, hipStreamSynchronize(st));
+  if (out) {
+    out->kontakte = (int64_t)stat.kontakte;
+    out->haftend = (int64_t)stat.haftend;
+    out->tiefe = stat.tiefe;
+    out->tangential = stat.tangential;
+  }
+  return 0;
+}
+
+extern "C" {
+
+int gc_xpbd_topology(gc_ctx* c, const gc_xpbd_topologie* t) {
+  if (!c || !t) return 1;
+  if (!c->topo) return fehler(c, "gc_xpbd_topology: the context's topology must be set first");
+  if (c->xp.bereit) return fehler(c, "gc_xpbd_topology: already set");
+  const int n = c->n;
+  auto pruefe = [&](const uint32_t* idx, int anzahl, int je) {
+    for (size_t q = 0; q < (size_t)anzahl * je; ++q)
+      if (idx[q] >= (uint32_t)n) return false;
+    return true;
+  };
+  if (t->n_abstand < 0 || t->n_naehte < 0 || t->n_biegungen < 0 || !pruefe(t->abstand_punkte, t->n_abstand, 2) ||
+      !pruefe(t->naht_punkte, t->n_naehte, 2) || !pruefe(t->biegung_punkte, t->n_biegungen, 4))
+    return fehler(c, "gc_xpbd_topology: a constraint refers to a point out of range");
+  XpbdHost& x = c->xp;
+  // distance constraints without the ones a membrane carries (cpu.rs:105-109, 148-152)
+  std::vector<uint32_t> ap;
+  std::vector<double> ar, aa;
+  for (int e = 0; e < t->n_abstand; ++e) {
+    if (t->membran_kante && t->membran_kante[e]) continue;
+    ap.push_back(t->abstand_punkte[2 * e]);
+    ap.push_back(t->abstand_punkte[2 * e + 1]);
+    ar.push_back(t->abstand_ruhe[e]);
+    aa.push_back(t->abstand_nachgiebigkeit[e]);
+  }
+  x.na = (int)ar.size();
+  x.nn = t->n_naehte;
+  x.nb = t->n_biegungen;
+  x.nm = (int)(c->m_p.size() / 3);
+  x.a_laeufe = laeufe<2>(ap.data(), x.na, n);
+  x.n_laeufe = laeufe<2>(t->naht_punkte, x.nn, n);
+  x.b_laeufe = laeufe<4>(t->biegung_punkte, x.nb, n);
+  // membrane elements by dependency level (cpu.rs:123-135, 160-189)
+  std::vector<int> stufe((size_t)x.nm), zuletzt((size_t)n, -1);
+  int stufen = 0;
+  for (int e = 0; e < x.nm; ++e) {
+    int l = 0;
+    for (int k = 0; k < 3; ++k) l = std::max(l, zuletzt[c->m_p[3 * (size_t)e + k]] + 1);
+    stufe[e] = l;
+    for (int k = 0; k < 3; ++k) zuletzt[c->m_p[3 * (size_t)e + k]] = l;
+    stufen = std::max(stufen, l + 1);
+  }
+  x.m_stufen.assign((size_t)stufen + 1, 0);
+  for (int e = 0; e < x.nm; ++e) x.m_stufen[stufe[e] + 1] += 1;
+  for (int l = 0; l < stufen; ++l) x.m_stufen[l + 1] += x.m_stufen[l];
+  std::vector<uint32_t> reihe((size_t)x.nm);
+  std::vector<int> frei(x.m_stufen.begin(), x.m_stufen.end() - 1);
+  for (int e = 0; e < x.nm; ++e) reihe[frei[stufe[e]]++] = (uint32_t)e;
+  x.n_alpha = t->naht_nachgiebigkeit;
+  x.a_p = dalloc<uint32_t>(c, ap.size());
+  x.a_ruhe = dalloc<double>(c, ar.size());
+  x.a_alpha = dalloc<double>(c, aa.size());
+  x.n_p = dalloc<uint32_t>(c, 2 * (size_t)x.nn);
+  x.n_ruhe = dalloc<double>(c, (size_t)x.nn);
+  x.b_p = dalloc<uint32_t>(c, 4 * (size_t)x.nb);
+  x.b_ruhe = dalloc<double>(c, (size_t)x.nb);
+  x.b_alpha = dalloc<double>(c, (size_t)x.nb);
+  x.m_reihe = dalloc<uint32_t>(c, reihe.size());
+  x.delta = dalloc<V3<double>>(c, (size_t)n);
+  x.korr = dalloc<V3<double>>(c, (size_t)n);
+  x.last = dalloc<V3<double>>(c, (size_t)n);
+  x.st = dalloc<XpStatDev>(c, 1);
+  if (!x.a_p || !x.a_ruhe || !x.a_alpha || !x.n_p || !x.n_ruhe || !x.b_p || !x.b_ruhe || !x.b_alpha || !x.m_reihe || !x.delta ||
+      !x.korr || !x.last || !x.st)
+    return fehler(c, "hipMalloc failed (XPBD)");
+  if (hoch(c, x.a_p, ap.data(), ap.size()) || hoch(c, x.a_ruhe, ar.data(), ar.size()) || hoch(c, x.a_alpha, aa.data(), aa.size()) ||
+      hoch(c, x.n_p, t->naht_punkte, 2 * (size_t)x.nn) || hoch(c, x.n_ruhe, t->naht_ruhe, (size_t)x.nn) ||
+      hoch(c, x.b_p, t->biegung_punkte, 4 * (size_t)x.nb) || hoch(c, x.b_ruhe, t->biegung_ruhe, (size_t)x.nb) ||
+      hoch(c, x.b_alpha, t->biegung_nachgiebigkeit, (size_t)x.nb) || hoch(c, x.m_reihe, reihe.data(), reihe.size()))
+    return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  x.bereit = true;
+  return 0;
+}
+
+int gc_xpbd_bild(gc_ctx* c, double bild_dt, int32_t unterschritte, gc_xpbd_statistik* out) {
+  if (!c) return 1;
+  if (!c->xp.bereit || !c->hat_param) return fehler(c, "gc_xpbd_bild: XPBD topology and parameters must be set first");
+  if (!(bild_dt > 0.0)) return fehler(c, "gc_xpbd_bild: bild_dt must be > 0");
+  return c->opt.praezision == GC_FP32 ? xpbd_bild<float>(c, bild_dt, unterschritte, out)
+                                      : xpbd_bild<double>(c, bild_dt, unterschritte, out);
+}
+
+int32_t gc_xpbd_laeufe(gc_ctx* c, int32_t* abstand, int32_t* naehte, int32_t* biegungen, int32_t* membran_stufen) {
+  if (!c || !c->xp.bereit) return 1;
+  if (abstand) *abstand = (int32_t)c->xp.a_laeufe.size() - 1;
+  if (naehte) *naehte = (int32_t)c->xp.n_laeufe.size() - 1;
+  if (biegungen) *biegungen = (int32_t)c->xp.b_laeufe.size() - 1;
+  if (membran_stufen) *membran_stufen = (int32_t)c->xp.m_stufen.size() - 1;
   return 0;
 }
 

@@ -238,6 +238,46 @@ class GeraetSzene:
                  "gc_netz_sdf_backen")
         return GitterSdf(werte=w, origin=tuple(float(a) for a in origin), spacing=spacing)
 
+    # -- the XPBD path (cpu.rs:76-246) -------------------------------------
+    def xpbd_topologie(self, s: Szene) -> None:
+        """Hand the scene's XPBD constraint lists (colour-ordered, with their
+        compliances) to the device."""
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        t = L.XpbdTopologie()
+        t.n_abstand = s.abstand_punkte.shape[0]
+        t.abstand_punkte = _p(arr(s.abstand_punkte, np.uint32), L.U32)
+        t.abstand_ruhe = _p(arr(s.abstand_ruhe, np.float64))
+        t.abstand_nachgiebigkeit = _p(arr(s.abstand_nachgiebigkeit, np.float64))
+        t.membran_kante = _p(arr(s.membran_kante, np.uint8), C.POINTER(C.c_uint8))
+        t.n_naehte = s.naht_punkte.shape[0]
+        t.naht_punkte = _p(arr(s.naht_punkte, np.uint32), L.U32)
+        t.naht_ruhe = _p(arr(s.naht_ruhe, np.float64))
+        t.naht_nachgiebigkeit = s.param.naht_nachgiebigkeit
+        t.n_biegungen = s.biegung_punkte.shape[0]
+        t.biegung_punkte = _p(arr(s.biegung_punkte, np.uint32), L.U32)
+        t.biegung_ruhe = _p(arr(s.biegung_ruhe, np.float64))
+        t.biegung_nachgiebigkeit = _p(arr(s.biegung_nachgiebigkeit, np.float64))
+        self._ck(self.lib.gc_xpbd_topology(self.h, C.byref(t)), "gc_xpbd_topology")
+
+    def xpbd_bild(self, unterschritte: Optional[int] = None, bild_dt: Optional[float] = None) -> dict:
+        """One XPBD frame (CpuReferenz::bild); returns its contact statistics."""
+        st = L.XpbdStatistik_c()
+        n = self._szene_param.unterschritte if unterschritte is None else unterschritte
+        dt = self._szene_param.bild_dt if bild_dt is None else bild_dt
+        self._ck(self.lib.gc_xpbd_bild(self.h, dt, int(n), C.byref(st)), "gc_xpbd_bild")
+        return {"kontakte": st.kontakte, "haftend": st.haftend, "tiefe": st.tiefe, "tangential": st.tangential}
+
+    def xpbd_laeufe(self) -> dict:
+        a, n, b, m = (C.c_int32() for _ in range(4))
+        self._ck(self.lib.gc_xpbd_laeufe(self.h, C.byref(a), C.byref(n), C.byref(b), C.byref(m)), "gc_xpbd_laeufe")
+        return {"abstand": a.value, "naehte": n.value, "biegungen": b.value, "membran_stufen": m.value}
+
     def set_state(self, x=None, v=None) -> None:
         self._ck(self.lib.gc_set_state(self.h, _p(_f64(x)) if x is not None else None,
                                        _p(_f64(v)) if v is not None else None), "gc_set_state")

@@ -40,11 +40,17 @@ from .material import Material
 
 @dataclass
 class Simparameter:
-    """szene.rs:95-126 (the XPBD-only fields are left out)."""
+    """szene.rs:95-126 (without the seam ease switch)."""
     bild_dt: float = 1.0 / 60.0
     schwerkraft: Tuple[float, float, float] = (0.0, 0.0, -9.81)
     daempfung: float = 0.0
     naht_nachgiebigkeit: float = 0.0
+    # substeps per frame of the XPBD path (one constraint round each)
+    unterschritte: int = 20
+
+
+# szene.rs:83: an XPBD edge is "soft" when alpha / h^2 >= 9 (w_i + w_j)
+WEICH_FAKTOR = 9.0
 
 
 @dataclass
@@ -150,6 +156,19 @@ class Szene:
 
     def gesamtmasse(self) -> float:
         return float(np.sum(self.masse))
+
+    def unterschritte_weich(self, faktor: float = WEICH_FAKTOR) -> int:
+        """szene.rs:794-807: XPBD substeps per frame so that every distance
+        constraint lies in XPBD's soft range, alpha / h^2 >= faktor (w_i + w_j)."""
+        h_max = math.inf
+        for c in range(self.abstand_punkte.shape[0]):
+            i, j = int(self.abstand_punkte[c, 0]), int(self.abstand_punkte[c, 1])
+            wsum = self.w[i] + self.w[j]
+            if wsum > 0.0:
+                h_max = min(h_max, math.sqrt(self.abstand_nachgiebigkeit[c] / (faktor * wsum)))
+        if not math.isfinite(h_max):
+            return self.param.unterschritte
+        return int(max(math.ceil(self.param.bild_dt / h_max), 1.0))
 
     def kopie(self) -> "Szene":
         import copy

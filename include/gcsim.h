@@ -223,6 +223,43 @@ int gc_netz_durchstoss(gc_ctx* ctx, int32_t koerper, int32_t n, const double* a,
 int gc_netz_sdf_backen(gc_ctx* ctx, int32_t koerper, const int32_t dims[3], const double origin[3], double spacing, double band,
                        float* werte);
 
+/* The XPBD path, the reference's other `Rechenweg` (`CpuReferenz`,
+ * cpu.rs:76-246), on the device. Its constraint lists as `Szene` holds them
+ * (szene.rs:127-160, colour-ordered): distance constraints with rest length,
+ * compliance (m/N) and the flag of those a membrane carries (skipped);
+ * seams with their rest and one compliance (Simparameter::naht_nachgiebigkeit,
+ * szene.rs:545); bending with rest angle and compliance. Masses, inverse
+ * masses, collision distances, friction, membranes and bodies are the
+ * context's. Call once, after gc_set_topology. */
+typedef struct gc_xpbd_topologie {
+  int32_t n_abstand;
+  const uint32_t* abstand_punkte;        /* 2 per constraint */
+  const double* abstand_ruhe;
+  const double* abstand_nachgiebigkeit;
+  const uint8_t* membran_kante;          /* may be NULL */
+  int32_t n_naehte;
+  const uint32_t* naht_punkte;           /* 2 per seam */
+  const double* naht_ruhe;
+  double naht_nachgiebigkeit;
+  int32_t n_biegungen;
+  const uint32_t* biegung_punkte;        /* 4 per constraint */
+  const double* biegung_ruhe;
+  const double* biegung_nachgiebigkeit;
+} gc_xpbd_topologie;
+int gc_xpbd_topology(gc_ctx* ctx, const gc_xpbd_topologie* t);
+/* Contacts of the last XPBD frame (`Kontaktstatistik`, cpu.rs:63-73). */
+typedef struct gc_xpbd_statistik {
+  int64_t kontakte, haftend;
+  double tiefe, tangential;
+} gc_xpbd_statistik;
+/* One frame of `CpuReferenz::bild` (cpu.rs:226-245): `unterschritte`
+ * substeps of bild_dt / unterschritte, gravity and damping from gc_param,
+ * bodies moved from their frame-start offsets per substep. */
+int gc_xpbd_bild(gc_ctx* ctx, double bild_dt, int32_t unterschritte, gc_xpbd_statistik* out);
+/* Runs (distance, seams, bending) and membrane dependency levels the frame
+ * launches one after another (a diagnostic). No counterpart in the reference. */
+int32_t gc_xpbd_laeufe(gc_ctx* ctx, int32_t* abstand, int32_t* naehte, int32_t* biegungen, int32_t* membran_stufen);
+
 /* State: positions and velocities (Szene::x, Szene::v). */
 int gc_set_state(gc_ctx* ctx, const double* x, const double* v);
 int gc_get_state(gc_ctx* ctx, double* x, double* v);

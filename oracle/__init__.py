@@ -54,6 +54,10 @@ class Statistik_c(C.Structure):
     ]
 
 
+class XpbdStatistik_c(C.Structure):
+    _fields_ = [("kontakte", C.c_int64), ("haftend", C.c_int64), ("tiefe", C.c_double), ("tangential", C.c_double)]
+
+
 class Naechster_c(C.Structure):
     _fields_ = [("abstand", C.c_double), ("punkt", C.c_double * 3), ("normale", C.c_double * 3),
                 ("dreieck", C.c_uint32), ("region", C.c_int32)]
@@ -127,6 +131,7 @@ def lib():
         L.or_netz_punkte_gegen_dreiecke.restype = C.c_int32
         L.or_netz_kontakt.argtypes = [C.c_void_p, D, D, D, C.c_double, D, D, D]
         L.or_sim_koerper_netz.argtypes = [C.c_void_p, C.c_void_p, D, D]
+        L.or_sim_xpbd_bild.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.POINTER(XpbdStatistik_c)]
     return _lib
 
 
@@ -248,6 +253,12 @@ class OrakelSim:
 
     def bild(self):
         lib().or_sim_bild(self.h)
+
+    def xpbd_bild(self, unterschritte: int, naht_nachgiebigkeit: float = 0.0) -> dict:
+        """One frame of the XPBD path (cpu.rs:226-245); returns its contact statistics."""
+        st = XpbdStatistik_c()
+        lib().or_sim_xpbd_bild(self.h, int(unterschritte), float(naht_nachgiebigkeit), C.byref(st))
+        return {"kontakte": st.kontakte, "haftend": st.haftend, "tiefe": st.tiefe, "tangential": st.tangential}
 
     def schritt(self, h: float):
         lib().or_sim_schritt(self.h, h)

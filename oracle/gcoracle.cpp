@@ -1173,6 +1173,192 @@ struct or_sim {
 };
 
 // ---------------------------------------------------------------------------
+// The XPBD path `CpuReferenz` (cpu.rs:76-246), the reference's second frame
+// loop, with the membrane's XPBD form `zuege` (kontinuum.rs:522-551)
+// ---------------------------------------------------------------------------
+struct Zug { double c; V3 grad[3]; double alpha; };  // kontinuum.rs:133-140
+
+// kontinuum.rs:526-551
+static void zuege(const Membran& m, const V3 p[3], Zug out[3]) {
+  V3 f1, f2;
+  f_(m, p, &f1, &f2);
+  V3 e = dehnung_aus(m, f1, f2);
+  double J[3][3][3];
+  jacobi_aus(m, f1, f2, J);
+  Basis B = basis(m);
+  auto grad = [&](V3 v, bool halb, V3 g[3]) {
+    for (int k = 0; k < 3; ++k)
+      for (int col = 0; col < 3; ++col) {
+        double s = 0.0;
+        for (int i = 0; i < 3; ++i) s += J[k][i][col] * v[i];
+        g[k][col] = halb ? 0.5 * s : s;
+      }
+  };
+  const double fl = m.flaeche;
+  auto alpha = [&](double k) { return k > 0.0 && fl > 0.0 ? 1.0 / (k * fl) : INFINITY; };
+  out[0].c = e[0];
+  grad(B.a, true, out[0].grad);
+  out[0].alpha = alpha(m.kette_n_m);
+  out[1].c = e[1];
+  grad(B.b, true, out[1].grad);
+  out[1].alpha = alpha(m.dehnung_n_m);
+  out[2].c = 2.0 * e[2];
+  grad(B.c, false, out[2].grad);
+  out[2].alpha = alpha(m.schub_n_m);
+}
+
+// cpu.rs:249-268: C = |p_i - p_j| - ruhe with p = x + delta + kappa; corrects kappa.
+static void xpbd_abstand(const std::vector<V3>& x, const std::vector<V3>& delta, std::vector<V3>& k, const std::vector<double>& w,
+                  std::array<uint32_t, 2> p, double ruhe, double alpha_h2) {
+  size_t i = p[0], j = p[1];
+  V3 d = add(add(sub(x[i], x[j]), sub(delta[i], delta[j])), sub(k[i], k[j]));
+  double l = norm(d);
+  double wsum = w[i] + w[j] + alpha_h2;
+  if (l < 1e-12 || wsum <= 0.0) return;
+  double dl = -(l - ruhe) / wsum;
+  V3 nrm = mul(d, 1.0 / l);
+  k[i] = add(k[i], mul(nrm, w[i] * dl));
+  k[j] = sub(k[j], mul(nrm, w[j] * dl));
+}
+
+// cpu.rs:310-333: C = theta - ruhe with p = x + delta + kappa; corrects kappa.
+static void xpbd_biegung(const std::vector<V3>& x, const std::vector<V3>& delta, std::vector<V3>& k, const std::vector<double>& w,
+                  std::array<uint32_t, 4> p, double ruhe, double alpha_h2) {
+  size_t p0 = p[0];
+  V3 q[4];
+  for (int a = 0; a < 4; ++a) {
+    size_t i = p[a];
+    q[a] = add(add(sub(x[i], x[p0]), sub(delta[i], delta[p0])), sub(k[i], k[p0]));
+  }
+  double theta;
+  V3 g[4];
+  if (!dieder(q, &theta, g)) return;
+  double nenner = alpha_h2;
+  for (int a = 0; a < 4; ++a) nenner += w[p[a]] * dot(g[a], g[a]);
+  if (nenner <= 0.0) return;
+  double dl = -winkel_fehler(theta, ruhe) / nenner;
+  for (int a = 0; a < 4; ++a) k[p[a]] = add(k[p[a]], mul(g[a], w[p[a]] * dl));
+}
+
+struct Xpbd {  // cpu.rs:76-86
+  std::vector<V3> delta, korr, last;
+  or_xpbd_statistik stat{};
+
+  // cpu.rs:94-223
+  void unterschritt(Szene& s, double h, double naht_alpha) {
+    size_t n = s.x.size();
+    V3 g = s.schwerkraft;
+    double daempf = fmax_(1.0 - s.daempfung * h, 0.0);
+    for (size_t i = 0; i < n; ++i) last[i] = mul(g, s.masse[i]);
+    for (size_t c = 0; c < s.abstand_punkte.size(); ++c) {
+      if (s.membran_kante[c]) continue;
+      size_t i = s.abstand_punkte[c][0], j = s.abstand_punkte[c][1];
+      V3 d = add(sub(s.x[j], s.x[i]), sub(delta[j], delta[i]));
+      double l = norm(d);
+      double zug = fmax_(l - s.abstand_ruhe[c], 0.0) / s.abstand_nachgiebigkeit[c];
+      if (l > 1e-12 && zug > 0.0) {
+        V3 f_ij = mul(d, zug / l);
+        last[i] = add(last[i], f_ij);
+        last[j] = sub(last[j], f_ij);
+      }
+    }
+    for (const auto& m : s.membran_elemente) {
+      V3 p[3];
+      for (int k = 0; k < 3; ++k) {
+        size_t q = m.punkte[k];
+        p[k] = add(add(s.x[q], delta[q]), korr[q]);
+      }
+      V3 fz[3];
+      krafte_zug(m.element, p, fz);
+      for (int k = 0; k < 3; ++k) last[m.punkte[k]] = add(last[m.punkte[k]], fz[k]);
+    }
+    for (size_t i = 0; i < n; ++i) {
+      korr[i] = {0.0, 0.0, 0.0};
+      if (s.w[i] > 0.0) {
+        s.v[i] = mul(add(s.v[i], mul(g, h)), daempf);
+        delta[i] = add(delta[i], mul(s.v[i], h));
+      }
+    }
+    double h2 = h * h;
+    for (size_t c = 0; c < s.abstand_punkte.size(); ++c) {
+      if (s.membran_kante[c]) continue;
+      xpbd_abstand(s.x, delta, korr, s.w, s.abstand_punkte[c], s.abstand_ruhe[c], s.abstand_nachgiebigkeit[c] / h2);
+    }
+    for (const auto& m : s.membran_elemente) {
+      V3 p[3];
+      for (int k = 0; k < 3; ++k) {
+        size_t q = m.punkte[k];
+        p[k] = add(add(s.x[q], delta[q]), korr[q]);
+      }
+      Zug z[3];
+      zuege(m.element, p, z);
+      for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(z[a].alpha)) continue;
+        double wsum = z[a].alpha / h2;
+        for (int k = 0; k < 3; ++k) wsum += s.w[m.punkte[k]] * dot(z[a].grad[k], z[a].grad[k]);
+        if (!(wsum > 0.0)) continue;
+        double dl = -z[a].c / wsum;
+        for (int k = 0; k < 3; ++k) {
+          size_t q = m.punkte[k];
+          korr[q] = add(korr[q], mul(z[a].grad[k], s.w[q] * dl));
+        }
+      }
+    }
+    for (size_t c = 0; c < s.naht_punkte.size(); ++c)
+      xpbd_abstand(s.x, delta, korr, s.w, s.naht_punkte[c], s.naht_ruhe[c], naht_alpha / h2);
+    for (size_t c = 0; c < s.biegung_punkte.size(); ++c)
+      xpbd_biegung(s.x, delta, korr, s.w, s.biegung_punkte[c], s.biegung_ruhe[c], s.biegung_nachgiebigkeit[c] / h2);
+    for (size_t i = 0; i < n; ++i) {
+      if (s.w[i] <= 0.0) continue;
+      for (const auto& k : s.koerper) {
+        V3 p = add(add(s.x[i], delta[i]), korr[i]);
+        Kontakt kt;
+        if (!k.kontakt(p, s.kollisionsabstand[i], &kt)) continue;
+        korr[i] = add(korr[i], mul(kt.normale, kt.tiefe));
+        V3 dx = sub(add(mul(s.v[i], h), korr[i]), mul(kt.geschwindigkeit, h));
+        V3 t = sub(dx, mul(kt.normale, dot(dx, kt.normale)));
+        double lt = norm(t);
+        double normallast = fmax_(-dot(last[i], kt.normale), 0.0);
+        double grenze = s.reibung[i] * fmax_(kt.tiefe, normallast * h2 * s.w[i]);
+        stat.kontakte += 1;
+        stat.tiefe += kt.tiefe;
+        stat.tangential += lt;
+        if (lt <= grenze) {
+          stat.haftend += 1;
+          korr[i] = sub(korr[i], t);
+        } else if (lt > 0.0) {
+          korr[i] = sub(korr[i], mul(t, grenze / lt));
+        }
+      }
+    }
+    for (size_t i = 0; i < n; ++i)
+      if (s.w[i] > 0.0) {
+        delta[i] = add(delta[i], korr[i]);
+        s.v[i] = add(s.v[i], mul(korr[i], 1.0 / h));
+      }
+  }
+
+  // cpu.rs:226-245
+  void bild(Szene& s, int unterschritte, double naht_alpha) {
+    int n = unterschritte > 1 ? unterschritte : 1;
+    double h = s.bild_dt / (double)n;
+    size_t np = s.x.size();
+    stat = or_xpbd_statistik{};
+    delta.assign(np, V3{0.0, 0.0, 0.0});
+    korr.assign(np, V3{0.0, 0.0, 0.0});
+    last.assign(np, V3{0.0, 0.0, 0.0});
+    std::vector<Koerper> anfang = s.koerper;
+    for (int k = 0; k < n; ++k) {
+      double weg = (double)(k + 1) * h;
+      for (size_t b = 0; b < s.koerper.size(); ++b) s.koerper[b] = anfang[b].verschoben(weg);
+      unterschritt(s, h, naht_alpha);
+    }
+    for (size_t i = 0; i < np; ++i) s.x[i] = add(s.x[i], delta[i]);
+    s.zeit += s.bild_dt;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // C API
 // ---------------------------------------------------------------------------
 static V3 v3(const double* p) { return {p[0], p[1], p[2]}; }
@@ -1327,6 +1513,11 @@ int32_t or_sim_dv_vorher(const or_sim* s, double* out) {
   return (int32_t)s->dv_vorher.size();
 }
 void or_sim_bild(or_sim* s) { s->bild(); }
+void or_sim_xpbd_bild(or_sim* s, int32_t unterschritte, double naht_nachgiebigkeit, or_xpbd_statistik* out) {
+  Xpbd x;
+  x.bild(s->s, unterschritte, naht_nachgiebigkeit);
+  if (out) *out = x.stat;
+}
 void or_sim_schritt(or_sim* s, double h) { s->schritt(h); }
 void or_sim_statistik(const or_sim* s, or_statistik* out) { *out = s->stat; }
 void or_sim_reset_statistik(or_sim* s) { s->stat = or_statistik{}; }

@@ -145,6 +145,12 @@ typedef struct or_statistik {
 
 typedef struct or_sim or_sim;
 
+/* Contacts of the last XPBD frame (cpu.rs:63-73). */
+typedef struct or_xpbd_statistik {
+  int64_t kontakte, haftend;
+  double tiefe, tangential;
+} or_xpbd_statistik;
+
 void    or_param_default(or_param* p);
 or_sim* or_sim_neu(const or_szene* s, const or_param* p);
 void    or_sim_frei(or_sim* s);
@@ -161,6 +167,10 @@ void    or_sim_setze_w(or_sim* s, const double* w);
 void    or_sim_setze_dv_vorher(or_sim* s, const double* dv, int32_t n); /* n = 0 clears the warm start */
 int32_t or_sim_dv_vorher(const or_sim* s, double* out);            /* returns its length (0 = none) */
 void    or_sim_bild(or_sim* s);
+/* One frame of the XPBD path `CpuReferenz::bild` (cpu.rs:226-245) on the
+ * sim's scene: `unterschritte` substeps of bild_dt / unterschritte, seams with
+ * compliance naht_nachgiebigkeit (szene.rs:545). */
+void    or_sim_xpbd_bild(or_sim* s, int32_t unterschritte, double naht_nachgiebigkeit, or_xpbd_statistik* out);
 void    or_sim_schritt(or_sim* s, double h);
 void    or_sim_statistik(const or_sim* s, or_statistik* out);
 void    or_sim_reset_statistik(or_sim* s);

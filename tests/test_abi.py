@@ -44,8 +44,9 @@ def test_keine_gpu_meldet_sich_laut():
 # header struct -> its ctypes mirror in garmentcraft_amd/_lib.py
 STRUKTUREN = {"gc_optionen": "Optionen", "gc_topologie": "Topologie", "gc_param": "Param", "gc_cg_lauf": "CgLauf_c",
               "gc_statistik": "Statistik_c", "gc_adjoint_stat": "AdjointStat_c", "gc_naht_eingabe": "NahtEingabe_c",
-              "gc_naht_ausgabe": "NahtAusgabe_c", "gc_netz_angaben": "NetzAngaben_c"}
-C_TYPEN = {"int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "double": ctypes.c_double, "int64_t": ctypes.c_int64,
+              "gc_naht_ausgabe": "NahtAusgabe_c", "gc_netz_angaben": "NetzAngaben_c",
+              "gc_xpbd_topologie": "XpbdTopologie", "gc_xpbd_statistik": "XpbdStatistik_c"}
+C_TYPEN = {"uint8_t": ctypes.c_uint8, "int32_t": ctypes.c_int32, "uint32_t": ctypes.c_uint32, "double": ctypes.c_double, "int64_t": ctypes.c_int64,
            "uint64_t": ctypes.c_uint64, "char": ctypes.c_char}
 
 
