@@ -76,10 +76,13 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lds-kb", type=int, default=0, help="LDS a PCG tile may fill (KiB); 0 = the library's choice")
     ap.add_argument("--ohne-selbstkollision", action="store_true", help="self-collision off (the plain drape)")
-    ap.add_argument("--workload", default="drape40k", choices=["drape40k", "stapel64", "adjoint20k", "gross250k"],
+    ap.add_argument("--workload", default="drape40k",
+                    choices=["drape40k", "stapel64", "adjoint20k", "gross250k", "netz40k", "xpbd40k"],
                     help="drape40k: the 40k-point drape; stapel64: 64 garments of 2k-60k points in one batch; "
                          "adjoint20k: gradient of a 60-step 20k-point drape rollout w.r.t. per-panel stiffnesses "
-                         "and friction; gross250k: the 250k-point drape, FP32 bulk against all-FP64")
+                         "and friction; gross250k: the 250k-point drape, FP32 bulk against all-FP64; "
+                         "netz40k: the 40k drape on the triangle-mesh body (contacts and path entries by mesh "
+                         "query); xpbd40k: XPBD frames (CpuReferenz) of the 40k drape")
     ap.add_argument("--vorlauf", type=int, default=120, help="adjoint20k: settling steps before the rollout starts")
     ap.add_argument("--fd-schritt", type=float, default=1e-4, help="adjoint20k: relative step of the central differences")
     ap.add_argument("--fd", type=int, default=0, help="adjoint20k: central differences of the FP64 forward for this "
@@ -97,13 +100,20 @@ def main() -> int:
         return adjoint_lauf(args)
     if args.workload == "gross250k":
         return gross_lauf(args)
+    if args.workload == "xpbd40k":
+        return xpbd_lauf(args)
 
     from garmentcraft_amd import szenen
     from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
 
     t0 = time.time()
-    sdf = szenen.koerper_sdf()
-    s, bericht = szenen.drape(sdf=sdf)
+    netz = args.workload == "netz40k"
+    if netz:
+        s, bericht = szenen.drape(mit_koerper=False)
+        s.koerper = szenen.koerper_netz(kante=0.01)
+        bericht["koerper_dreiecke"] = [int(k.dreiecke.shape[0]) for k in s.koerper]
+    else:
+        s, bericht = szenen.drape(sdf=szenen.koerper_sdf())
     print(f"[bench] scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     dt = s.param.bild_dt
     selbst = not args.ohne_selbstkollision
@@ -194,7 +204,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32+f64" if args.praezision == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": "drape: 28 woven panels, 24 seams, ellipsoid-cylinder grid SDF 256^3 at 3 mm, "
+        "config": {"workload": "drape: 28 woven panels, 24 seams, " +
+                               ("ellipsoid and cylinder as triangle meshes of 1 cm edges (" +
+                                " + ".join(str(t) for t in bericht["koerper_dreiecke"]) + " triangles), "
+                                if netz else "ellipsoid-cylinder grid SDF 256^3 at 3 mm, ") +
                                "friction 0.3, " + ("self-collision on, " if selbst else "self-collision off, ") +
                                "dt 1/120, rtol 1e-6",
                    "timed_steps": f"{args.warmup + 1}-{args.warmup + args.steps}",
@@ -210,6 +223,67 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out))
+    return 0
+
+
+def xpbd_lauf(args) -> int:
+    """XPBD frames (CpuReferenz::bild, cpu.rs:226-245) of the 40k drape on the
+    grid-SDF body: 20 substeps of a 1/60 s frame, each one round of distance,
+    membrane, seam and bending constraints in the reference's Gauss-Seidel
+    order, body contact with friction. Against the FP64 restatement on one
+    host core, from the same state."""
+    from garmentcraft_amd import szenen
+    from garmentcraft_amd.geraet import GeraetSzene, ImplizitParameter
+
+    steps = args.steps if args.steps != 300 else 60
+    warmup = args.warmup if args.warmup != 10 else 20
+    t0 = time.time()
+    s, bericht = szenen.drape(sdf=szenen.koerper_sdf())
+    s.param.bild_dt = 1.0 / 60.0
+    n = s.param.unterschritte
+    print(f"[bench] scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    g = GeraetSzene(s, ImplizitParameter(), praezision="fp64")
+    g.xpbd_topologie(s)
+    laeufe = g.xpbd_laeufe()
+    starts = 3 + 1 + laeufe["abstand"] + laeufe["membran_stufen"] + laeufe["naehte"] + laeufe["biegungen"] + 2
+    print(f"[bench] runs {laeufe}, {starts} launches per substep", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        g.xpbd_bild(n)
+    x_w, v_w = g.lage()
+    t_a = time.perf_counter()
+    kontakte = 0
+    for _ in range(steps):
+        kontakte += g.xpbd_bild(n)["kontakte"]
+    t_b = time.perf_counter()
+    wall_ms = (t_b - t_a) * 1e3
+    value = steps / (wall_ms / 1e3)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        o = O.OrakelSim(s, O.param())
+        o.setze_lage(x_w, v_w)
+        t_c = time.perf_counter()
+        n_cpu = 0
+        while True:
+            o.xpbd_bild(n)
+            n_cpu += 1
+            if time.perf_counter() - t_c > args.cpu_sekunden or n_cpu >= steps:
+                break
+        t_d = time.perf_counter()
+        cpu = {"value": round(n_cpu / (t_d - t_c), 4), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} XPBD frames of the same drape from the state after {warmup} warm-up frames, "
+                         f"FP64 C++ restatement of CpuReferenz::bild", "cpu": cpu_modell(),
+               "build": "g++ -O3 (oracle/Makefile)"}
+    out = {"metric": "XPBD frames/s @40k verts (CpuReferenz, 20 substeps of a 1/60 s frame)", "value": round(value, 3),
+           "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(wall_ms / steps, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "drape: 28 woven panels on the grid-SDF body, XPBD, 20 substeps per 1/60 s frame",
+                      "punkte": s.n, "laeufe": laeufe, "starts_je_unterschritt": starts,
+                      "us_je_unterschritt": round(wall_ms * 1e3 / (steps * n), 1),
+                      "us_je_start": round(wall_ms * 1e3 / (steps * n * starts), 2),
+                      "kontakte_je_bild": round(kontakte / steps, 1), "parallelism": "1 GPU"},
+           "roofline": None, "cpu_baseline": cpu}
     print(json.dumps(out))
     return 0
 

@@ -82,43 +82,60 @@ GC_HD void zuege(const Membran<double>& m, const V3<double> p[3], Zug out[3]) {
 
 }  // namespace xp
 
-// Load at the start of the substep: gravity (cpu.rs:99-101).
-__global__ void k_xp_last_start(int n, const double* __restrict__ masse, V3<double> g, V3<double>* __restrict__ last) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) last[i] = g * masse[i];
-}
+// The load at the start of the substep (cpu.rs:97-135): gravity, then edge
+// tension in constraint order, then membrane tension in element order. The
+// terms depend only on positions, so each is computed by its own thread and
+// every point gathers its terms in the reference's order (its distance
+// constraints ascending, then its elements ascending): the sum per point is
+// the reference's, without one launch per run or level.
 
-// Edge tension k max(0, L - L0) of one run of distance constraints
-// (cpu.rs:102-121).
-__global__ void k_xp_last_abstand(int lo, int hi, const uint32_t* __restrict__ ap, const double* __restrict__ ruhe,
-                                  const double* __restrict__ alpha, const V3<double>* __restrict__ x,
-                                  const V3<double>* __restrict__ delta, V3<double>* __restrict__ last) {
-  for (int c = lo + blockIdx.x * blockDim.x + threadIdx.x; c < hi; c += gridDim.x * blockDim.x) {
+// Edge tension k max(0, L - L0) per distance constraint (cpu.rs:102-121);
+// da[c] = 0 where the reference adds nothing.
+__global__ void k_xp_zug_abstand(int na, const uint32_t* __restrict__ ap, const double* __restrict__ ruhe,
+                                 const double* __restrict__ alpha, const V3<double>* __restrict__ x,
+                                 const V3<double>* __restrict__ delta, V3<double>* __restrict__ fa, uint8_t* __restrict__ da) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < na; c += gridDim.x * blockDim.x) {
     const uint32_t i = ap[2 * c], j = ap[2 * c + 1];
     const V3<double> d = (x[j] - x[i]) + (delta[j] - delta[i]);
     const double l = norm(d);
     const double zug = fmax(l - ruhe[c], 0.0) / alpha[c];
-    if (l > 1e-12 && zug > 0.0) {
-      const V3<double> f = d * (zug / l);
-      last[i] = last[i] + f;
-      last[j] = last[j] - f;
-    }
+    const bool an = l > 1e-12 && zug > 0.0;
+    da[c] = an ? 1 : 0;
+    if (an) fa[c] = d * (zug / l);
   }
 }
 
-// Membrane tension of one dependency level (cpu.rs:123-135).
-__global__ void k_xp_last_membran(int lo, int hi, const uint32_t* __restrict__ reihe, const uint32_t* __restrict__ mp,
-                                  const Membran<double>* __restrict__ mel, const V3<double>* __restrict__ x,
-                                  const V3<double>* __restrict__ delta, const V3<double>* __restrict__ korr,
-                                  V3<double>* __restrict__ last) {
-  for (int q = lo + blockIdx.x * blockDim.x + threadIdx.x; q < hi; q += gridDim.x * blockDim.x) {
-    const uint32_t t = reihe[q];
+// Membrane tension per element and corner (cpu.rs:123-135).
+__global__ void k_xp_zug_membran(int nm, const uint32_t* __restrict__ mp, const Membran<double>* __restrict__ mel,
+                                 const V3<double>* __restrict__ x, const V3<double>* __restrict__ delta,
+                                 const V3<double>* __restrict__ korr, V3<double>* __restrict__ fm) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nm; t += gridDim.x * blockDim.x) {
     const uint32_t* d = mp + 3 * (size_t)t;
     V3<double> p[3];
     for (int k = 0; k < 3; ++k) p[k] = (x[d[k]] + delta[d[k]]) + korr[d[k]];
     V3<double> f1, f2, kr[3], zug[3];
     spalten(mel[t], p[1] - p[0], p[2] - p[0], &f1, &f2);
     krafte(mel[t], f1, f2, kr, zug);
-    for (int k = 0; k < 3; ++k) last[d[k]] = last[d[k]] + zug[k];
+    for (int k = 0; k < 3; ++k) fm[3 * (size_t)t + k] = zug[k];
+  }
+}
+
+// Per point: gravity, then its distance terms (entry 2c + side: side 0 adds,
+// side 1 subtracts), then its membrane terms (entry 3t + corner).
+__global__ void k_xp_last(int n, const double* __restrict__ masse, V3<double> g, const uint32_t* __restrict__ a_off,
+                          const uint32_t* __restrict__ a_liste, const V3<double>* __restrict__ fa,
+                          const uint8_t* __restrict__ da, const uint32_t* __restrict__ m_off,
+                          const uint32_t* __restrict__ m_liste, const V3<double>* __restrict__ fm,
+                          V3<double>* __restrict__ last) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    V3<double> l = g * masse[i];
+    for (uint32_t e = a_off[i]; e < a_off[i + 1]; ++e) {
+      const uint32_t c = a_liste[e] >> 1;
+      if (!da[c]) continue;
+      l = (a_liste[e] & 1u) ? l - fa[c] : l + fa[c];
+    }
+    for (uint32_t e = m_off[i]; e < m_off[i + 1]; ++e) l = l + fm[m_liste[e]];
+    last[i] = l;
   }
 }
 

@@ -62,6 +62,10 @@ struct XpbdHost {
   int na = 0, nn = 0, nb = 0, nm = 0;
   std::vector<int> a_laeufe, n_laeufe, b_laeufe, m_stufen;  // run / level starts, last entry = count
   uint32_t *a_p = nullptr, *n_p = nullptr, *b_p = nullptr, *m_reihe = nullptr;
+  // per point: its distance constraints (2c + side) and membrane corners (3t + k), ascending
+  uint32_t *a_off = nullptr, *a_liste = nullptr, *m_off = nullptr, *m_liste = nullptr;
+  V3<double>*fa = nullptr, *fm = nullptr;
+  uint8_t* da = nullptr;
   double *a_ruhe = nullptr, *a_alpha = nullptr, *n_ruhe = nullptr, *b_ruhe = nullptr, *b_alpha = nullptr;
   double n_alpha = 0.0;
   V3<double> *delta = nullptr, *korr = nullptr, *last = nullptr;
@@ -2964,16 +2968,13 @@ static int xpbd_bild(gc_ctx* c, double bild_dt, int unterschritte, gc_xpbd_stati
       for (int q = 0; q < 3; ++q) c->koerper[b].k.punkt[q] = anfang[b].punkt[q] + anfang[b].geschwindigkeit[q] * weg;
     alle_param(c);
     if (sys_hoch<T>(c)) return 1;
-    hipLaunchKernelGGL(k_xp_last_start, dim3(gp), dim3(256), 0, st, c->n, s.masse, g, x.last);
-    for (size_t r = 0; r + 1 < x.a_laeufe.size(); ++r) {
-      const int lo = x.a_laeufe[r], hi = x.a_laeufe[r + 1];
-      hipLaunchKernelGGL(k_xp_last_abstand, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.a_p, x.a_ruhe, x.a_alpha, s.x, x.delta, x.last);
-    }
-    for (size_t r = 0; r + 1 < x.m_stufen.size(); ++r) {
-      const int lo = x.m_stufen[r], hi = x.m_stufen[r + 1];
-      hipLaunchKernelGGL(k_xp_last_membran, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.m_reihe, s.m_p, s.m_el64, s.x, x.delta,
-                         x.korr, x.last);
-    }
+    if (x.na > 0)
+      hipLaunchKernelGGL(k_xp_zug_abstand, zahl(0, x.na), dim3(256), 0, st, x.na, x.a_p, x.a_ruhe, x.a_alpha, s.x, x.delta, x.fa,
+                         x.da);
+    if (x.nm > 0)
+      hipLaunchKernelGGL(k_xp_zug_membran, zahl(0, x.nm), dim3(256), 0, st, x.nm, s.m_p, s.m_el64, s.x, x.delta, x.korr, x.fm);
+    hipLaunchKernelGGL(k_xp_last, dim3(gp), dim3(256), 0, st, c->n, s.masse, g, x.a_off, x.a_liste, x.fa, x.da, x.m_off,
+                       x.m_liste, x.fm, x.last);
     hipLaunchKernelGGL(k_xp_vorhersage, dim3(gp), dim3(256), 0, st, c->n, s.w, g, h, daempf, s.v, x.delta, x.korr);
     for (size_t r = 0; r + 1 < x.a_laeufe.size(); ++r) {
       const int lo = x.a_laeufe[r], hi = x.a_laeufe[r + 1];
@@ -3062,6 +3063,20 @@ int gc_xpbd_topology(gc_ctx* c, const gc_xpbd_topologie* t) {
   std::vector<uint32_t> reihe((size_t)x.nm);
   std::vector<int> frei(x.m_stufen.begin(), x.m_stufen.end() - 1);
   for (int e = 0; e < x.nm; ++e) reihe[frei[stufe[e]]++] = (uint32_t)e;
+  // per-point term lists for the load (cpu.rs:97-135)
+  std::vector<uint32_t> a_off((size_t)n + 1, 0u), m_off((size_t)n + 1, 0u);
+  for (size_t q = 0; q < ap.size(); ++q) a_off[ap[q] + 1] += 1;
+  for (size_t q = 0; q < 3 * (size_t)x.nm; ++q) m_off[c->m_p[q] + 1] += 1;
+  for (int i = 0; i < n; ++i) {
+    a_off[i + 1] += a_off[i];
+    m_off[i + 1] += m_off[i];
+  }
+  std::vector<uint32_t> a_liste(a_off[n]), m_liste(m_off[n]);
+  {
+    std::vector<uint32_t> fa_(a_off.begin(), a_off.end() - 1), fm_(m_off.begin(), m_off.end() - 1);
+    for (size_t q = 0; q < ap.size(); ++q) a_liste[fa_[ap[q]]++] = (uint32_t)q;  // q = 2c + side
+    for (size_t q = 0; q < 3 * (size_t)x.nm; ++q) m_liste[fm_[c->m_p[q]]++] = (uint32_t)q;  // q = 3t + corner
+  }
   x.n_alpha = t->naht_nachgiebigkeit;
   x.a_p = dalloc<uint32_t>(c, ap.size());
   x.a_ruhe = dalloc<double>(c, ar.size());
@@ -3076,6 +3091,17 @@ int gc_xpbd_topology(gc_ctx* c, const gc_xpbd_topologie* t) {
   x.korr = dalloc<V3<double>>(c, (size_t)n);
   x.last = dalloc<V3<double>>(c, (size_t)n);
   x.st = dalloc<XpStatDev>(c, 1);
+  x.a_off = dalloc<uint32_t>(c, a_off.size());
+  x.a_liste = dalloc<uint32_t>(c, a_liste.size());
+  x.m_off = dalloc<uint32_t>(c, m_off.size());
+  x.m_liste = dalloc<uint32_t>(c, m_liste.size());
+  x.fa = dalloc<V3<double>>(c, (size_t)x.na);
+  x.da = dalloc<uint8_t>(c, (size_t)x.na);
+  x.fm = dalloc<V3<double>>(c, 3 * (size_t)x.nm);
+  if (!x.a_off || !x.a_liste || !x.m_off || !x.m_liste || !x.fa || !x.da || !x.fm) return fehler(c, "hipMalloc failed (XPBD)");
+  if (hoch(c, x.a_off, a_off.data(), a_off.size()) || hoch(c, x.a_liste, a_liste.data(), a_liste.size()) ||
+      hoch(c, x.m_off, m_off.data(), m_off.size()) || hoch(c, x.m_liste, m_liste.data(), m_liste.size()))
+    return 1;
   if (!x.a_p || !x.a_ruhe || !x.a_alpha || !x.n_p || !x.n_ruhe || !x.b_p || !x.b_ruhe || !x.b_alpha || !x.m_reihe || !x.delta ||
       !x.korr || !x.last || !x.st)
     return fehler(c, "hipMalloc failed (XPBD)");
