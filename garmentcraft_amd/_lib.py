@@ -47,7 +47,7 @@ class Param(C.Structure):
         ("normalkraft_aus_last", C.c_int32),
         ("selbstkollision", C.c_int32), ("selbst_zusatzdicke", C.c_double), ("selbst_steifigkeit", C.c_double),
         ("selbst_stoss_iterationen", C.c_int32), ("selbst_schnitte_an_naehten", C.c_int32),
-        ("durchstoss_pruefen", C.c_int32),
+        ("durchstoss_pruefen", C.c_int32), ("koerperpunkt_durchgaenge", C.c_int32), ("koerperpunkt_tiefe", C.c_double),
     ]
 
 
@@ -86,6 +86,7 @@ class Statistik_c(C.Structure):
         ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
         ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64), ("durchstoesse", C.c_int64),
         ("cg_rundungsgrenze", C.c_int64), ("cg_neustarts", C.c_int64), ("selbst_listen_faktor", C.c_int64),
+        ("koerperpunkte", C.c_int64),
     ]
 
 

@@ -249,6 +249,8 @@ struct GridState {
   long long halten_abweichungen;  // replayed decisions that the state no longer allowed (taken as decided)
   long long rundungsgrenze;  // FP32 solves stopped at their rounding floor
   long long neustarts;       // FP32 true-residual restarts
+  long long koerperpunkte;   // cloth triangles lifted over body points
+  unsigned int kp_pass[2];   // lifts of the last two passes of body points against cloth triangles (pass d at d & 1)
   // result of the last solve
   int iterationen;
   int nicht_konv;
@@ -850,6 +852,7 @@ __global__ void k_reset(GridState* gs, Slot* slots) {
     gs->frei[1] = 0;
     gs->n_fest = 0;
   }
+  if (threadIdx.x < 2) gs->kp_pass[threadIdx.x] = 0;
   barriere_leeren(slots);
 }
 
@@ -1859,4 +1862,95 @@ __global__ void k_fest_zaehlen(const uint8_t* fl, int n, GridState* gs) {
   unsigned lokal = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) lokal += fl[i] == 1 ? 1u : 0u;
   if (lokal) atomicAdd(&gs->n_fest, lokal);
+}
+
+// ---------------------------------------------------------------------------
+// Body points against cloth triangles in the step (implizit.rs:1017-1026,
+// netzkoerper.rs:625-665): pass d of body b. A pass after one that lifted
+// nothing returns at once (the reference leaves the loop), so the launch
+// sequence stays fixed. Pass d counts its lifts in kp_pass[d & 1], which the
+// pass's first body clears while the other slot holds pass d - 1's count.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool kp_aus(const GridState* gs, int d) {
+  return d > 0 && __hip_atomic_load(&gs->kp_pass[(d - 1) & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+}
+
+// Per cloth triangle: the deepest stab of the body's points. A stabbed
+// triangle keeps its record at its index and joins the list (any order).
+template <typename T>
+__global__ void k_kp_stiche(const SysDev<T>* __restrict__ sp, int b, GridState* gs, int d, int erster, double tief,
+                            const uint32_t* __restrict__ dreiecke, int nt, Stich* __restrict__ st,
+                            uint32_t* __restrict__ liste, uint32_t* __restrict__ zahl) {
+  if (erster && blockIdx.x == 0 && threadIdx.x == 0) gs->kp_pass[d & 1] = 0u;
+  if (kp_aus(gs, d)) return;
+  const SysDev<T>& s = *sp;
+  const Koerper& k = s.koerper[b];
+  const V3<double> versatz = {k.punkt[0], k.punkt[1], k.punkt[2]};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    const uint32_t* q = dreiecke + 3 * (size_t)t;
+    Stich h;
+    h.tiefe = 0.0;
+    h.kir = 0;
+    h.da = 1u;
+    if (!nk_stich(*k.netz, versatz, tief, -1e-9, s.x[q[0]], s.x[q[1]], s.x[q[2]], &h)) continue;
+    st[t] = h;
+    liste[atomicAdd(zahl, 1u)] = (uint32_t)t;
+  }
+}
+
+// The lifts in cloth-triangle order (netzkoerper.rs:647-665): one block walks
+// the dependency levels (a level's triangles share no point, and each point
+// meets its triangles level by level in ascending order), each level over the
+// list of stabbed triangles, so every point receives its lifts in the
+// reference's sequence. Empties the list for the next pass.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_kp_heben(const SysDev<T>* __restrict__ sp, int b, GridState* gs, int d, int nstufen,
+                                                   const int* __restrict__ stufe, const uint32_t* __restrict__ dreiecke,
+                                                   const Stich* __restrict__ st, const uint32_t* __restrict__ liste,
+                                                   uint32_t* __restrict__ zahl) {
+  if (kp_aus(gs, d)) return;
+  const SysDev<T>& s = *sp;
+  const Koerper& k = s.koerper[b];
+  const NetzDev& N = *k.netz;
+  const V3<double> starr = {k.geschwindigkeit[0], k.geschwindigkeit[1], k.geschwindigkeit[2]};
+  const uint32_t m = *zahl;
+  unsigned gehoben = 0;
+  for (int l = 0; l < nstufen && m > 0; ++l) {
+    for (uint32_t q = threadIdx.x; q < m; q += blockDim.x) {
+      const uint32_t t = liste[q];
+      if (stufe[t] != l) continue;
+      const Stich h = st[t];
+      const uint32_t* idx = dreiecke + 3 * (size_t)t;
+      const double bary[3] = {h.bary.x, h.bary.y, h.bary.z};
+      double gew[3], nenner = 0.0;
+      for (int j = 0; j < 3; ++j) {
+        gew[j] = s.w[idx[j]];
+        nenner += bary[j] * bary[j] * gew[j];
+      }
+      if (nenner <= 0.0) continue;
+      const V3<double> vk = N.pv ? nk::lade3(N.pv, h.kir) : starr;
+      V3<double> summe = zero3<double>();
+      for (int j = 0; j < 3; ++j) summe = summe + s.v[idx[j]] * bary[j];
+      const double vn = fmin(dot(summe - vk, h.n), 0.0);
+      for (int j = 0; j < 3; ++j) {
+        const double f = bary[j] * gew[j] / nenner;
+        s.x[idx[j]] = s.x[idx[j]] + h.n * (f * h.tiefe);
+        s.v[idx[j]] = s.v[idx[j]] - h.n * (f * vn);
+      }
+      ++gehoben;
+    }
+    __syncthreads();
+  }
+  __shared__ unsigned summe_block;
+  if (threadIdx.x == 0) summe_block = 0;
+  __syncthreads();
+  if (gehoben) atomicAdd(&summe_block, gehoben);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *zahl = 0u;
+    if (summe_block) {
+      atomicAdd(&gs->kp_pass[d & 1], summe_block);
+      atomicAdd((unsigned long long*)&gs->koerperpunkte, (unsigned long long)summe_block);
+    }
+  }
 }

@@ -33,6 +33,8 @@ struct NetzDev {
   const double* pv;          // velocity per point of a deforming body, or null
   const uint32_t* anfang;    // cell z lists eintraege[anfang[z] .. anfang[z + 1]]
   const uint32_t* eintraege;
+  const uint32_t* p_anfang;  // the grid of the body's points on the same raster (one cell each)
+  const uint32_t* p_eintraege;
   double ursprung[3];
   double zelle;
   int64_t zahl[3];
@@ -557,4 +559,91 @@ __global__ void k_nk_backen_seiten(const NetzDev* __restrict__ gp, V3<double> or
       w[ix] = (float)(seite * band);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Body points against cloth triangles (netzkoerper.rs:625-736)
+// ---------------------------------------------------------------------------
+
+// The point grid (netzkoerper.rs:861-879): each body point in its one cell.
+__global__ void k_nk_punkte_zaehlen(const NetzDev* __restrict__ gp, const double* __restrict__ punkte, int n,
+                                    uint32_t* __restrict__ zahl_je_zelle) {
+  const NetzDev& g = *gp;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int64_t k[3];
+    nk::index(g, nk::lade3(punkte, i), k);
+    for (int a = 0; a < 3; ++a) k[a] = k[a] < 0 ? 0 : (k[a] > g.zahl[a] - 1 ? g.zahl[a] - 1 : k[a]);
+    atomicAdd(&zahl_je_zelle[(k[2] * g.zahl[1] + k[1]) * g.zahl[0] + k[0]], 1u);
+  }
+}
+__global__ void k_nk_punkte_fuellen(const NetzDev* __restrict__ gp, const double* __restrict__ punkte, int n,
+                                    uint32_t* __restrict__ cursor, uint32_t* __restrict__ roh) {
+  const NetzDev& g = *gp;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int64_t k[3];
+    nk::index(g, nk::lade3(punkte, i), k);
+    for (int a = 0; a < 3; ++a) k[a] = k[a] < 0 ? 0 : (k[a] > g.zahl[a] - 1 ? g.zahl[a] - 1 : k[a]);
+    roh[atomicAdd(&cursor[(k[2] * g.zahl[1] + k[1]) * g.zahl[0] + k[0]], 1u)] = (uint32_t)i;
+  }
+}
+
+// The deepest stab per cloth triangle (netzkoerper.rs:681-736, 641-646): every
+// body point in the cells of the triangle's box widened by max(tief, band),
+// cells z-major, each cell's points ascending; the line p + t n through the
+// triangle (Moeller-Trumbore) at -tief < t < band; a later stab replaces the
+// kept one only if deeper.
+struct Stich {
+  double tiefe;   // -t of the deepest stab
+  V3<double> n;   // the body point's normal
+  V3<double> bary;
+  uint32_t kir;   // the body point
+  uint32_t da;
+};
+
+__device__ __forceinline__ bool nk_stich(const NetzDev& N, V3<double> versatz, double tief, double band, V3<double> a,
+                                         V3<double> b, V3<double> c, Stich* aus) {
+  const double rand = fmax(tief, band);
+  V3<double> lo = a, hi = a;
+  for (const V3<double>& q : {b, c}) {
+    lo = {fmin(lo.x, q.x), fmin(lo.y, q.y), fmin(lo.z, q.z)};
+    hi = {fmax(hi.x, q.x), fmax(hi.y, q.y), fmax(hi.z, q.z)};
+  }
+  const V3<double> rr = {rand, rand, rand};
+  lo = (lo - versatz) - rr;
+  hi = (hi - versatz) + rr;
+  const V3<double> e1 = b - a, e2 = c - a;
+  int64_t r[3][2];
+  bool da = false;
+  if (!nk::bereich(N, lo, hi, r)) return false;
+  for (int64_t kz = r[2][0]; kz <= r[2][1]; ++kz)
+    for (int64_t ky = r[1][0]; ky <= r[1][1]; ++ky)
+      for (int64_t kx = r[0][0]; kx <= r[0][1]; ++kx) {
+        const int64_t z = (kz * N.zahl[1] + ky) * N.zahl[0] + kx;
+        const uint32_t e1_ = N.p_anfang[z + 1];
+        for (uint32_t e = N.p_anfang[z]; e < e1_; ++e) {
+          const uint32_t k = N.p_eintraege[e];
+          const V3<double> p = nk::lade3(N.punkte, k) + versatz;
+          const V3<double> n = nk::lade3(N.pn, k);
+          const V3<double> pv = cross(n, e2);
+          const double det = dot(e1, pv);
+          if (fabs(det) < 1e-18) continue;
+          const V3<double> tv = p - a;
+          const double bu = dot(tv, pv) / det;
+          if (!(bu >= 0.0 && bu <= 1.0)) continue;
+          const V3<double> qv = cross(tv, e1);
+          const double bv = dot(n, qv) / det;
+          if (bv < 0.0 || bu + bv > 1.0) continue;
+          const double t = dot(e2, qv) / det;
+          if (t > -tief && t < band) {
+            if (!da || -t > aus->tiefe) {
+              aus->tiefe = -t;
+              aus->n = n;
+              aus->bary = {1.0 - bu - bv, bu, bv};
+              aus->kir = k;
+            }
+            da = true;
+          }
+        }
+      }
+  return da;
 }

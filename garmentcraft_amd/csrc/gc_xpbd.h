@@ -194,6 +194,38 @@ __global__ void k_xp_membran(int lo, int hi, const uint32_t* __restrict__ reihe,
   }
 }
 
+// All membrane levels in one launch of one block: the levels follow one
+// another behind a workgroup barrier, which orders the block's global stores
+// before the next level's loads. For levels of at most a few hundred elements
+// this replaces a launch per level by a barrier per level; larger levels need
+// more than one CU.
+__global__ void __launch_bounds__(1024) k_xp_membran_stufen(int nstufen, const int* __restrict__ stufen,
+                                                            const uint32_t* __restrict__ reihe, const uint32_t* __restrict__ mp,
+                                                            const Membran<double>* __restrict__ mel, double h2,
+                                                            const V3<double>* __restrict__ x, const V3<double>* __restrict__ delta,
+                                                            V3<double>* __restrict__ korr, const double* __restrict__ w) {
+  for (int l = 0; l < nstufen; ++l) {
+    const int lo = stufen[l], hi = stufen[l + 1];
+    for (int q = lo + (int)threadIdx.x; q < hi; q += (int)blockDim.x) {
+      const uint32_t t = reihe[q];
+      const uint32_t* d = mp + 3 * (size_t)t;
+      V3<double> p[3];
+      for (int k = 0; k < 3; ++k) p[k] = (x[d[k]] + delta[d[k]]) + korr[d[k]];
+      xp::Zug z[3];
+      xp::zuege(mel[t], p, z);
+      for (int a = 0; a < 3; ++a) {
+        if (!isfinite(z[a].alpha)) continue;
+        double wsum = z[a].alpha / h2;
+        for (int k = 0; k < 3; ++k) wsum += w[d[k]] * dot(z[a].grad[k], z[a].grad[k]);
+        if (!(wsum > 0.0)) continue;
+        const double dl = -z[a].c / wsum;
+        for (int k = 0; k < 3; ++k) korr[d[k]] = korr[d[k]] + z[a].grad[k] * (w[d[k]] * dl);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // cpu.rs:310-333 for one run of bending constraints.
 __global__ void k_xp_biegung(int lo, int hi, const uint32_t* __restrict__ bp, const double* __restrict__ ruhe,
                              const double* __restrict__ alpha, double h2, const V3<double>* __restrict__ x,

@@ -42,15 +42,19 @@ struct Netzhost {
   double *punkte_d = nullptr, *fn_d = nullptr, *pn_d = nullptr, *kugel_d = nullptr, *pv_d = nullptr;
   uint32_t *dreiecke_d = nullptr, *nachbar_d = nullptr, *inz_anfang_d = nullptr, *inz_d = nullptr;
   uint32_t *anfang_d = nullptr, *zaehler_d = nullptr, *cursor_d = nullptr, *roh_d = nullptr, *eintraege_d = nullptr;
+  // the grid of body points (same raster), for body points against cloth triangles
+  uint32_t *p_anfang_d = nullptr, *p_roh_d = nullptr, *p_eintraege_d = nullptr;
   size_t nz_kap = 0, e_kap = 0;
   NetzDev* dev = nullptr;
   void frei() {
     for (void* q : {(void*)punkte_d, (void*)fn_d, (void*)pn_d, (void*)kugel_d, (void*)pv_d, (void*)dreiecke_d,
                     (void*)nachbar_d, (void*)inz_anfang_d, (void*)inz_d, (void*)anfang_d, (void*)zaehler_d,
-                    (void*)cursor_d, (void*)roh_d, (void*)eintraege_d, (void*)dev})
+                    (void*)cursor_d, (void*)roh_d, (void*)eintraege_d, (void*)p_anfang_d, (void*)p_roh_d,
+                    (void*)p_eintraege_d, (void*)dev})
       if (q) (void)hipFree(q);
     punkte_d = fn_d = pn_d = kugel_d = pv_d = nullptr;
     dreiecke_d = nachbar_d = inz_anfang_d = inz_d = anfang_d = zaehler_d = cursor_d = roh_d = eintraege_d = nullptr;
+    p_anfang_d = p_roh_d = p_eintraege_d = nullptr;
     dev = nullptr;
   }
 };
@@ -66,6 +70,10 @@ struct XpbdHost {
   uint32_t *a_off = nullptr, *a_liste = nullptr, *m_off = nullptr, *m_liste = nullptr;
   V3<double>*fa = nullptr, *fm = nullptr;
   uint8_t* da = nullptr;
+  // membrane levels in one single-block launch (levels of a few hundred
+  // elements), or one launch per level (large levels)
+  bool stufen_einzeln = false;
+  int* stufen_d = nullptr;
   double *a_ruhe = nullptr, *a_alpha = nullptr, *n_ruhe = nullptr, *b_ruhe = nullptr, *b_alpha = nullptr;
   double n_alpha = 0.0;
   V3<double> *delta = nullptr, *korr = nullptr, *last = nullptr;
@@ -102,6 +110,17 @@ struct gc_ctx {
   gc_param param{};
   std::vector<Koerperhost> koerper;
   XpbdHost xp;
+  // body points against cloth triangles (implizit.rs:1017-1026): the cloth
+  // triangles' dependency levels (a level's triangles share no point, each
+  // point meets its triangles in ascending order), the deepest stab per
+  // triangle and the list of stabbed triangles; built once passes are asked for
+  bool kp_bereit = false;
+  int kp_nt = 0, kp_nstufen = 0;
+  uint32_t* kp_dreiecke = nullptr;
+  int* kp_stufe = nullptr;
+  Stich* kp_stich = nullptr;
+  uint32_t* kp_liste = nullptr;
+  uint32_t* kp_zahl = nullptr;
   // device
   std::vector<void*> allocs;
   SysDev<double> sd{};
@@ -237,6 +256,37 @@ static void folge_neu(gc_ctx* c) {
     c->graph = nullptr;
   }
   ++c->folge;
+}
+
+// The tables of body points against cloth triangles, once the parameters ask
+// for passes and the topology holds the cloth triangles. Outside any step, so
+// that no allocation falls into a captured substep.
+static int kp_bereiten(gc_ctx* c) {
+  if (c->kp_bereit || !c->topo || c->param.koerperpunkt_durchgaenge <= 0 || c->sk_dreiecke_in.empty()) return 0;
+  const int nt = (int)(c->sk_dreiecke_in.size() / 3);
+  std::vector<int> stufe((size_t)nt), zuletzt((size_t)c->n, -1);
+  int stufen = 0;
+  for (int t = 0; t < nt; ++t) {
+    int l = 0;
+    for (int k = 0; k < 3; ++k) l = std::max(l, zuletzt[c->sk_dreiecke_in[3 * (size_t)t + k]] + 1);
+    stufe[t] = l;
+    for (int k = 0; k < 3; ++k) zuletzt[c->sk_dreiecke_in[3 * (size_t)t + k]] = l;
+    stufen = std::max(stufen, l + 1);
+  }
+  c->kp_dreiecke = dalloc<uint32_t>(c, c->sk_dreiecke_in.size());
+  c->kp_stufe = dalloc<int>(c, (size_t)nt);
+  c->kp_stich = dalloc<Stich>(c, (size_t)nt);
+  c->kp_liste = dalloc<uint32_t>(c, (size_t)nt);
+  c->kp_zahl = dalloc<uint32_t>(c, 1);  // cleared by dalloc, emptied by each lift pass
+  if (!c->kp_dreiecke || !c->kp_stufe || !c->kp_stich || !c->kp_liste || !c->kp_zahl)
+    return fehler(c, "hipMalloc failed (body points against cloth triangles)");
+  if (hoch(c, c->kp_dreiecke, c->sk_dreiecke_in.data(), c->sk_dreiecke_in.size()) || hoch(c, c->kp_stufe, stufe.data(), stufe.size()))
+    return 1;
+  HIPCK(c, hipStreamSynchronize(c->stream));
+  c->kp_nt = nt;
+  c->kp_nstufen = stufen;
+  c->kp_bereit = true;
+  return 0;
 }
 
 static void frei_alles(gc_ctx* c) {
@@ -1058,12 +1108,37 @@ static int launch_zwischen_pcg(gc_ctx* c) {
   return 0;
 }
 
+// Body points against cloth triangles after the update (implizit.rs:1017-1026):
+// per pass and mesh body in order, the deepest stab per triangle, then the
+// lifts level by level in one block over the stabbed triangles.
+template <typename T>
+static int launch_koerperpunkte(gc_ctx* c) {
+  if (!c->kp_bereit || c->param.koerperpunkt_durchgaenge <= 0) return 0;
+  std::vector<int> netze;
+  for (size_t b = 0; b < c->koerper.size(); ++b)
+    if (c->koerper[b].netz) netze.push_back((int)b);
+  if (netze.empty()) return 0;
+  const SysDev<T>* sp = dsys<T>(c);
+  for (int d = 0; d < c->param.koerperpunkt_durchgaenge; ++d)
+    for (size_t q = 0; q < netze.size(); ++q) {
+      hipLaunchKernelGGL(k_kp_stiche<T>, dim3(gitter(c->kp_nt)), dim3(BUILD_THREADS), 0, c->stream, sp, netze[q], c->gs, d,
+                         q == 0 ? 1 : 0, c->param.koerperpunkt_tiefe, (const uint32_t*)c->kp_dreiecke, c->kp_nt, c->kp_stich,
+                         c->kp_liste, c->kp_zahl);
+      hipLaunchKernelGGL(k_kp_heben<T>, dim3(1), dim3(1024), 0, c->stream, sp, netze[q], c->gs, d, c->kp_nstufen,
+                         (const int*)c->kp_stufe, (const uint32_t*)c->kp_dreiecke, (const Stich*)c->kp_stich,
+                         (const uint32_t*)c->kp_liste, c->kp_zahl);
+    }
+  HIPCK(c, hipGetLastError());
+  return 0;
+}
+
 template <typename T>
 static int launch_nach_pcg(gc_ctx* c) {
   const SysDev<T>& s = sys<T>(c);
   const SysDev<T>* sp = dsys<T>(c);
   hipLaunchKernelGGL(k_freigabe<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs, c->slots, 1);
   hipLaunchKernelGGL(k_update<T>, dim3(gitter(s.n)), dim3(BUILD_THREADS), 0, c->stream, sp, c->gs);
+  if (launch_koerperpunkte<T>(c)) return 1;
   if (s.selbst && launch_selbst_stoss<T>(c)) return 1;
   HIPCK(c, hipGetLastError());
   return 0;
@@ -2398,6 +2473,7 @@ int gc_set_topology(gc_ctx* c, const gc_topologie* t) {
   if (rc) return rc;
   HIPCK(c, hipStreamSynchronize(c->stream));
   c->topo = true;
+  if (kp_bereiten(c)) return 1;
   alle_param(c);
   return 0;
 }
@@ -2405,13 +2481,20 @@ int gc_set_topology(gc_ctx* c, const gc_topologie* t) {
 int gc_set_param(gc_ctx* c, const gc_param* p) {
   if (!c || !p) return 1;
   if (p->cg_max < 0) return fehler(c, "cg_max must be >= 0");
+  if (p->koerperpunkt_durchgaenge < 0) return fehler(c, "koerperpunkt_durchgaenge must be >= 0");
   const bool selbst_vorher = selbst_an(c);
   const int stoss_vorher = c->param.selbst_stoss_iterationen;
+  const int kp_vorher = c->param.koerperpunkt_durchgaenge;
+  const double kp_tiefe_vorher = c->param.koerperpunkt_tiefe;
   c->param = *p;
   c->hat_param = true;
-  // self-collision on or off and the impact iterations decide which kernels a
-  // substep launches: a captured graph of the old sequence is dropped
-  if (selbst_an(c) != selbst_vorher || (selbst_an(c) && p->selbst_stoss_iterationen != stoss_vorher)) folge_neu(c);
+  // self-collision on or off, the impact iterations and the passes of body
+  // points decide which kernels a substep launches (and with which depth): a
+  // captured graph of the old sequence is dropped
+  if (selbst_an(c) != selbst_vorher || (selbst_an(c) && p->selbst_stoss_iterationen != stoss_vorher) ||
+      p->koerperpunkt_durchgaenge != kp_vorher || !(p->koerperpunkt_tiefe == kp_tiefe_vorher))
+    folge_neu(c);
+  if (kp_bereiten(c)) return 1;
   if (c->sk_bereit) {
     // the kernels read the spring stiffness factor, the seam-cut switch and
     // the impact iterations from the device copy of the self-collision tables
@@ -2628,8 +2711,9 @@ static int netz_alloc(gc_ctx* c, X** p, size_t count) {
 }
 
 // Everything that depends on the points (netzkoerper.rs:257-291, 820-912):
-// face normals, spheres, point normals and the grid of triangle lists, built
-// on the device from the pose in h.punkte.
+// face normals, spheres, point normals, the grid of triangle lists and the
+// grid of body points (each cell's entries ascending), built on the device
+// from the pose in h.punkte.
 static int netz_pose(gc_ctx* c, Netzhost& h) {
   const double vol = netz_volumen(h);
   if (!(vol > 0.0))
@@ -2642,11 +2726,13 @@ static int netz_pose(gc_ctx* c, Netzhost& h) {
   HIPCK(c, hipMemcpyAsync(h.punkte_d, h.punkte.data(), h.punkte.size() * sizeof(double), hipMemcpyHostToDevice, st));
   if ((size_t)nz + 1 > h.nz_kap) {
     if (netz_alloc(c, &h.anfang_d, (size_t)nz + 1) || netz_alloc(c, &h.zaehler_d, (size_t)nz) ||
-        netz_alloc(c, &h.cursor_d, (size_t)nz))
+        netz_alloc(c, &h.cursor_d, (size_t)nz) || netz_alloc(c, &h.p_anfang_d, (size_t)nz + 1))
       return 1;
     h.nz_kap = (size_t)nz + 1;
   }
   h.bild.anfang = h.anfang_d;
+  h.bild.p_anfang = h.p_anfang_d;
+  h.bild.p_eintraege = h.p_eintraege_d;
   h.bild.pv = h.pv_d;
   HIPCK(c, hipMemcpyAsync(h.dev, &h.bild, sizeof(NetzDev), hipMemcpyHostToDevice, st));
   const unsigned gt = (unsigned)std::min<int64_t>(4096, (h.nt + 255) / 256), gp = (unsigned)std::min<int64_t>(4096, (h.n + 255) / 256);
@@ -2672,6 +2758,14 @@ static int netz_pose(gc_ctx* c, Netzhost& h) {
   if (gesamt > 0)
     hipLaunchKernelGGL(k_nk_ordnen, dim3((unsigned)std::min<uint32_t>(4096, (gesamt + 255) / 256)), dim3(256), 0, st, h.anfang_d,
                        nz, h.roh_d, h.eintraege_d);
+  // the point grid: count, scan, fill, order (netzkoerper.rs:862-879)
+  HIPCK(c, hipMemsetAsync(h.zaehler_d, 0, (size_t)nz * sizeof(uint32_t), st));
+  hipLaunchKernelGGL(k_nk_punkte_zaehlen, dim3(gp), dim3(256), 0, st, (const NetzDev*)h.dev, h.punkte_d, h.n, h.zaehler_d);
+  hipLaunchKernelGGL(k_nk_scan, dim3(1), dim3(1024), 0, st, h.zaehler_d, nz, h.p_anfang_d);
+  HIPCK(c, hipMemcpyAsync(h.cursor_d, h.p_anfang_d, (size_t)nz * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_nk_punkte_fuellen, dim3(gp), dim3(256), 0, st, (const NetzDev*)h.dev, h.punkte_d, h.n, h.cursor_d,
+                     h.p_roh_d);
+  hipLaunchKernelGGL(k_nk_ordnen, dim3(gp), dim3(256), 0, st, h.p_anfang_d, nz, h.p_roh_d, h.p_eintraege_d);
   HIPCK(c, hipGetLastError());
   HIPCK(c, hipStreamSynchronize(st));
   return 0;
@@ -2705,7 +2799,8 @@ int gc_add_netz(gc_ctx* c, const double* punkte, int32_t n_punkte, const uint32_
   const size_t n = (size_t)h->n, nt = (size_t)h->nt;
   if (netz_alloc(c, &h->punkte_d, 3 * n) || netz_alloc(c, &h->fn_d, 3 * nt) || netz_alloc(c, &h->pn_d, 3 * n) ||
       netz_alloc(c, &h->kugel_d, 4 * nt) || netz_alloc(c, &h->dreiecke_d, 3 * nt) || netz_alloc(c, &h->nachbar_d, 3 * nt) ||
-      netz_alloc(c, &h->inz_anfang_d, n + 1) || netz_alloc(c, &h->inz_d, inz.size()) || netz_alloc(c, &h->dev, 1)) {
+      netz_alloc(c, &h->inz_anfang_d, n + 1) || netz_alloc(c, &h->inz_d, inz.size()) || netz_alloc(c, &h->dev, 1) ||
+      netz_alloc(c, &h->p_roh_d, n) || netz_alloc(c, &h->p_eintraege_d, n)) {
     h->frei();
     return -1;
   }
@@ -2743,6 +2838,7 @@ int gc_add_netz(gc_ctx* c, const double* punkte, int32_t n_punkte, const uint32_
   }
   kh.netz = h;
   c->koerper.push_back(kh);
+  folge_neu(c);  // a mesh body adds the passes of body points to the substep
   alle_param(c);
   return (int)c->koerper.size() - 1;
 }
@@ -2981,10 +3077,15 @@ static int xpbd_bild(gc_ctx* c, double bild_dt, int unterschritte, gc_xpbd_stati
       hipLaunchKernelGGL(k_xp_abstand, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.a_p, x.a_ruhe, (const double*)x.a_alpha, 0.0, h2,
                          s.x, x.delta, x.korr, s.w);
     }
-    for (size_t r = 0; r + 1 < x.m_stufen.size(); ++r) {
-      const int lo = x.m_stufen[r], hi = x.m_stufen[r + 1];
-      hipLaunchKernelGGL(k_xp_membran, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.m_reihe, s.m_p, s.m_el64, h2, s.x, x.delta, x.korr,
-                         s.w);
+    if (x.stufen_einzeln) {
+      hipLaunchKernelGGL(k_xp_membran_stufen, dim3(1), dim3(1024), 0, st, (int)x.m_stufen.size() - 1, x.stufen_d, x.m_reihe, s.m_p,
+                         s.m_el64, h2, s.x, x.delta, x.korr, s.w);
+    } else {
+      for (size_t r = 0; r + 1 < x.m_stufen.size(); ++r) {
+        const int lo = x.m_stufen[r], hi = x.m_stufen[r + 1];
+        hipLaunchKernelGGL(k_xp_membran, zahl(lo, hi), dim3(256), 0, st, lo, hi, x.m_reihe, s.m_p, s.m_el64, h2, s.x, x.delta,
+                           x.korr, s.w);
+      }
     }
     for (size_t r = 0; r + 1 < x.n_laeufe.size(); ++r) {
       const int lo = x.n_laeufe[r], hi = x.n_laeufe[r + 1];
@@ -3099,6 +3200,14 @@ int gc_xpbd_topology(gc_ctx* c, const gc_xpbd_topologie* t) {
   x.da = dalloc<uint8_t>(c, (size_t)x.na);
   x.fm = dalloc<V3<double>>(c, 3 * (size_t)x.nm);
   if (!x.a_off || !x.a_liste || !x.m_off || !x.m_liste || !x.fa || !x.da || !x.fm) return fehler(c, "hipMalloc failed (XPBD)");
+  // one block walks all levels when they average at most 128 elements: its
+  // barrier then costs less than a launch per level. Larger levels need the
+  // whole chip (a substep of the 40k drape, 90 levels of ~900 elements,
+  // takes 3.5 ms with them in one block, 1.1 ms with a launch per level).
+  const int nstufen = (int)x.m_stufen.size() - 1;
+  x.stufen_einzeln = nstufen > 0 && x.nm <= 128 * nstufen;
+  x.stufen_d = dalloc<int>(c, x.m_stufen.size());
+  if (!x.stufen_d || hoch(c, x.stufen_d, x.m_stufen.data(), x.m_stufen.size())) return fehler(c, "hipMalloc failed (XPBD)");
   if (hoch(c, x.a_off, a_off.data(), a_off.size()) || hoch(c, x.a_liste, a_liste.data(), a_liste.size()) ||
       hoch(c, x.m_off, m_off.data(), m_off.size()) || hoch(c, x.m_liste, m_liste.data(), m_liste.size()))
     return 1;
@@ -3300,12 +3409,14 @@ int gc_read_statistik(gc_ctx* c, gc_statistik* out, int32_t reset) {
     out->cg_rundungsgrenze = g.rundungsgrenze;
     out->cg_neustarts = g.neustarts;
     out->selbst_listen_faktor = c->sk_mal;
+    out->koerperpunkte = g.koerperpunkte;
   }
   if (reset) {
     g.cg_iterationen = g.loesungen = g.nicht_konvergiert = g.kontakte = g.haftend = g.freigegeben = 0;
     g.naehe_pd = g.naehe_kk = g.selbst_stoesse = g.durchdringungen = g.gebiete = g.durchstoesse = 0;
     g.rundungsgrenze = 0;
     g.neustarts = 0;
+    g.koerperpunkte = 0;
     g.rest_max = 0.0;
     g.spins_max = 0;
     HIPCK(c, hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice));

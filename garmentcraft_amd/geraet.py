@@ -33,6 +33,8 @@ class ImplizitParameter:
     selbst_stoss_iterationen: int = 1
     selbst_schnitte_an_naehten: bool = False
     durchstoss_pruefen: bool = True
+    koerperpunkt_durchgaenge: int = 0
+    koerperpunkt_tiefe: float = 0.01
 
 
 @dataclass
@@ -153,6 +155,8 @@ class GeraetSzene:
         c.selbst_stoss_iterationen = p.selbst_stoss_iterationen
         c.selbst_schnitte_an_naehten = 1 if p.selbst_schnitte_an_naehten else 0
         c.durchstoss_pruefen = 1 if p.durchstoss_pruefen else 0
+        c.koerperpunkt_durchgaenge = p.koerperpunkt_durchgaenge
+        c.koerperpunkt_tiefe = p.koerperpunkt_tiefe
         self._ck(self.lib.gc_set_param(self.h, C.byref(c)), "gc_set_param")
 
     def add_koerper(self, kb) -> int:

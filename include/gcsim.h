@@ -102,6 +102,14 @@ typedef struct gc_param {
    * its entry (Koerpernetz::durchstoss, netzkoerper.rs:541-596; for the grid
    * SDF, the first outside -> inside crossing of the sampled distance). */
   int32_t durchstoss_pruefen;
+  /* Passes of body points against cloth triangles per step after the update
+   * (implizit.rs:477-481, 1017-1026; Koerpernetz::punkte_gegen_dreiecke,
+   * netzkoerper.rs:625-665): every cloth triangle under which a point of a
+   * mesh body lies at most koerperpunkt_tiefe (m) deep is lifted over the
+   * deepest such point, split over its corners by inverse mass; the passes
+   * end early after one that lifts nothing. 0 switches it off (default). */
+  int32_t koerperpunkt_durchgaenge;
+  double koerperpunkt_tiefe;  /* default 0.01 */
 } gc_param;
 
 /* CgLauf (implizit.rs:293-299). */
@@ -134,6 +142,7 @@ typedef struct gc_statistik {
   int64_t selbst_listen_faktor; /* capacity of the self-collision lists, in multiples of the default (a call
                                  * whose lists overflow is taken again with them doubled, up to 16; the
                                  * reference keeps them in growable vectors). Not reset. */
+  int64_t koerperpunkte;       /* cloth triangles lifted over body points (implizit.rs:550, 1022) */
 } gc_statistik;
 
 /* Construction. Replaces `GpuLoeser::neu() -> Result<Self, String>` (gpu.rs:1136):

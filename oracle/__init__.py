@@ -41,7 +41,7 @@ class Param_c(C.Structure):
         ("kontakt_rand", C.c_double), ("normalkraft_aus_last", C.c_int32), ("selbstkollision", C.c_int32),
         ("selbst_zusatzdicke", C.c_double), ("selbst_steifigkeit", C.c_double),
         ("selbst_stoss_iterationen", C.c_int32), ("selbst_schnitte_an_naehten", C.c_int32),
-        ("durchstoss_pruefen", C.c_int32),
+        ("durchstoss_pruefen", C.c_int32), ("koerperpunkt_durchgaenge", C.c_int32), ("koerperpunkt_tiefe", C.c_double),
     ]
 
 
@@ -51,6 +51,7 @@ class Statistik_c(C.Structure):
         ("kontakte", C.c_int64), ("haftend", C.c_int64), ("freigegeben", C.c_int64), ("rest_max", C.c_double),
         ("naehe_punkt_dreieck", C.c_int64), ("naehe_kante_kante", C.c_int64), ("selbst_stoesse", C.c_int64),
         ("durchdringungen", C.c_int64), ("durchdringungsgebiete", C.c_int64), ("durchstoesse", C.c_int64),
+        ("koerperpunkte", C.c_int64),
     ]
 
 

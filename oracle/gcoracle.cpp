@@ -1145,6 +1145,14 @@ struct or_sim {
         }
       }
     }
+    // implizit.rs:1017-1026: body points against cloth triangles, passes until one corrects nothing
+    for (int d = 0; d < param.koerperpunkt_durchgaenge; ++d) {
+      size_t zahl = 0;
+      for (const auto& k : s.koerper)
+        if (k.art == 2) zahl += k.netz->punkte_gegen_dreiecke(param.koerperpunkt_tiefe, k.versatz, k.geschwindigkeit, s.dreiecke, s.w, s.x, s.v);
+      stat.koerperpunkte += (int64_t)zahl;
+      if (zahl == 0) break;
+    }
     // implizit.rs:1030-1037: impacts on the path x_anfang -> x, held points x 1e9
     if (param.selbstkollision && hat_selbst) {
       std::vector<Stoss> st = finde_stoesse(netz, x_anfang, s.x, s.w, true, true);
@@ -1434,6 +1442,7 @@ void or_param_default(or_param* p) {  // implizit.rs:505-531
   p->schritte_je_bild = 5; p->cg_max = 50; p->cg_toleranz = 1e-5; p->naht_k = 500.0; p->kontakt_rand = 1e-4;
   p->normalkraft_aus_last = 0; p->selbstkollision = 0; p->selbst_zusatzdicke = 2.5e-3; p->selbst_steifigkeit = 1e-3;
   p->selbst_stoss_iterationen = 1; p->selbst_schnitte_an_naehten = 0; p->durchstoss_pruefen = 1;
+  p->koerperpunkt_durchgaenge = 0; p->koerperpunkt_tiefe = 0.01;
 }
 
 or_sim* or_sim_neu(const or_szene* d, const or_param* p) {

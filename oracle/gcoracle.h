@@ -133,6 +133,8 @@ typedef struct or_param {
   int32_t selbst_stoss_iterationen;
   int32_t selbst_schnitte_an_naehten;
   int32_t durchstoss_pruefen;     /* implizit.rs:476, default on */
+  int32_t koerperpunkt_durchgaenge; /* implizit.rs:478-479, default 0 (off) */
+  double koerperpunkt_tiefe;        /* implizit.rs:480-481, default 0.01 m */
 } or_param;
 
 typedef struct or_statistik {
@@ -141,6 +143,7 @@ typedef struct or_statistik {
   int64_t naehe_punkt_dreieck, naehe_kante_kante, selbst_stoesse;
   int64_t durchdringungen, durchdringungsgebiete;
   int64_t durchstoesse;
+  int64_t koerperpunkte;
 } or_statistik;
 
 typedef struct or_sim or_sim;

@@ -11,7 +11,8 @@ oracle's restatement of `Koerpernetz`.
 * A new pose (`nachfuehren`): the grown grid as the oracle's.
 * Whole steps on a drape falling onto the two-mesh body: contacts, releases,
   sticking and path entries equal per step, positions to 1e-9 m (FP64) and
-  1e-4 m (FP32) from the oracle's state.
+  1e-4 m (FP32) from the oracle's state; with the passes of body points
+  against cloth triangles on, the lifts per step equal as well.
 """
 import math
 
@@ -250,6 +251,40 @@ def test_drape_auf_netz_durchstoss_wie_orakel(drape_auf_netz):
         assert float(np.max(np.linalg.norm(xa - xb, axis=1))) <= 1e-9, k
     print("entries", summe)
     assert summe >= 5
+    g.close()
+
+
+@pytest.mark.parametrize("durchgaenge", [1, 3])
+def test_drape_auf_netz_koerperpunkte_wie_orakel(drape_auf_netz, durchgaenge):
+    """Body points against cloth triangles after the update (implizit.rs:
+    1017-1026, netzkoerper.rs:625-665): each step from the oracle's state with
+    the passes on, FP64; the lifted triangles per step equal, positions and
+    velocities to 1e-9 m and 1e-7 m/s. The drape lands on the ellipsoid in
+    steps 6-10, where its triangles sink between their points past the
+    body's vertices."""
+    s = drape_auf_netz
+    pp = dict(P, cg_toleranz=1e-10, koerperpunkt_durchgaenge=durchgaenge)
+    o = O.OrakelSim(s, O.param(**pp))
+    g = GeraetSzene(s, ImplizitParameter(**pp), praezision="fp64")
+    summe = 0
+    for k in range(16):
+        x, v = o.lage()
+        g.set_state(x, v)
+        g.set_dv_vorher(o.dv_vorher())
+        vor = o.statistik()
+        g.statistik(reset=True)
+        o.schritt(H)
+        g.schritte(H, 1)
+        nach, sg = o.statistik(), g.statistik()
+        for name in ("koerperpunkte", "kontakte", "freigegeben"):
+            assert nach[name] - vor[name] == sg[name], (k, name)
+        summe += nach["koerperpunkte"] - vor["koerperpunkte"]
+        xa, va = o.lage()
+        xb, vb = g.lage()
+        assert float(np.max(np.linalg.norm(xa - xb, axis=1))) <= 1e-9, k
+        assert float(np.max(np.linalg.norm(va - vb, axis=1))) <= 1e-7, k
+    print("lifts", durchgaenge, summe)
+    assert summe >= 40
     g.close()
 
 
