@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help="replay a captured HIP graph per step")
     ap.add_argument("--cpu-sekunden", type=float, default=25.0, help="time budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ohne-vergleich", action="store_true",
+                    help="stapel64: skip the sequential comparison (for counter runs)")
     ap.add_argument("--bloecke", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lds-kb", type=int, default=0, help="LDS a PCG tile may fill (KiB); 0 = the library's choice")
@@ -326,10 +328,12 @@ def stapel_lauf(args) -> int:
     st = GeraetStapel(A)
     st.schritte(dt, 1)
     wahl = st.autotune(dt)
-    nb, threads = st.bloecke()
     print(f"[bench] autotune: {wahl}", file=sys.stderr, flush=True)
-    if warmup > 1:
-        st.schritte_graph(dt, warmup - 1)
+    # the warm-up in chunks of 10 steps, so that it reports progress
+    for k in range(1, warmup, 10):
+        st.schritte_graph(dt, min(10, warmup - k))
+        print(f"[bench] warm-up steps 1-{min(k + 9, warmup - 1) + 1} done", file=sys.stderr, flush=True)
+    nb, threads = st.bloecke()
     for g in A:
         g.statistik(reset=True)
     t_a = time.perf_counter()
@@ -338,6 +342,8 @@ def stapel_lauf(args) -> int:
     wall_ms = (t_b - t_a) * 1e3
     value = steps / (wall_ms / 1e3)
     stat = [g.statistik() for g in A]
+    print("[bench] per system (points, blocks, CG iterations): "
+          + str([(int(p), int(b), int(q["cg_iterationen"])) for p, b, q in zip(punkte, nb, stat)]), file=sys.stderr, flush=True)
     iters = sum(q["cg_iterationen"] for q in stat)
     print(f"[bench] batch: {steps} graph steps in {wall_ms:.1f} ms, {iters} CG iterations in all, "
           f"max per system {max(q['cg_iterationen'] for q in stat)}", file=sys.stderr, flush=True)
@@ -354,23 +360,27 @@ def stapel_lauf(args) -> int:
         b = pcg_bytes(s, args.praezision)
         algo += q["cg_iterationen"] * b["iteration"] + q["loesungen"] * b["start"]
     achieved = algo / (ms_pcg * 1e-3) / 1e9
+    iters_e = sum(q["cg_iterationen"] for q in stat_e)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_pcg_stapel",
-                "ms_pcg_je_schritt": round(ms_pcg / steps, 3)}
+                "ms_pcg_je_schritt": round(ms_pcg / steps, 3),
+                "eager_schritte": {"cg_iterationen": iters_e, "bytes_algorithmisch": algo}}
     st.close()
 
     # the same systems, one after another, plain gc_step per system and step
-    B = kontexte()
-    for g in B:
-        g.schritte(dt, warmup)
-    t_c = time.perf_counter()
-    for _ in range(steps):
+    seq = None
+    if not args.ohne_vergleich:
+        B = kontexte()
         for g in B:
-            g.schritte(dt, 1)
-    t_d = time.perf_counter()
-    seq = steps / (t_d - t_c)
-    print(f"[bench] sequential: {steps} steps of {len(B)} systems in {(t_d - t_c) * 1e3:.1f} ms", file=sys.stderr,
-          flush=True)
+            g.schritte(dt, warmup)
+        t_c = time.perf_counter()
+        for _ in range(steps):
+            for g in B:
+                g.schritte(dt, 1)
+        t_d = time.perf_counter()
+        seq = steps / (t_d - t_c)
+        print(f"[bench] sequential: {steps} steps of {len(B)} systems in {(t_d - t_c) * 1e3:.1f} ms", file=sys.stderr,
+              flush=True)
     out = {
         "metric": "implicit steps/s @40k verts, PCG rtol 1e-6; batched/adjoint vs plain GPU vs CPU",
         "value": round(value, 3), "unit": "batched steps/s (64 garments per step)", "n_gpus": 1, "steps": steps,
@@ -382,7 +392,9 @@ def stapel_lauf(args) -> int:
                    "timed_steps": f"{warmup + 1}-{warmup + steps}",
                    "punkte": sum(punkte), "systeme": len(satz), "graph": True, "praezision": args.praezision,
                    "threads": threads, "bloecke": int(nb.sum()), "autotune_ms_loesung": round(wahl["ms_loesung"], 3),
-                   "sequentiell_steps_s": round(seq, 3), "stapel_gegen_sequentiell": round(value / seq, 2),
+                   "bloecke_je_system_max": int(nb.max()),
+                   "sequentiell_steps_s": round(seq, 3) if seq else None,
+                   "stapel_gegen_sequentiell": round(value / seq, 2) if seq else None,
                    "cg_iterationen_je_schritt": round(iters / steps, 1),
                    "kontakte_je_schritt": round(sum(q["kontakte"] for q in stat) / steps, 1),
                    "naehe_je_schritt": round(sum(q["naehe_punkt_dreieck"] + q["naehe_kante_kante"] for q in stat) / steps, 1),

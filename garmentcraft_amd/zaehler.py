@@ -6,9 +6,10 @@ dispatch).
     python -m garmentcraft_amd.zaehler --fetch F.db --write W.db \\
         --kopie-fetch KF.db --kopie-write KW.db --kernel k_pcg --letzte 80 --iterationen N
 
-sums the counters over the last `--letzte` dispatches of the kernel whose name
-starts with `--kernel` and divides by `--iterationen` (the PCG iterations
-those dispatches ran, from the bench line of the same run). The databases are
+sums the counters over the last `--letzte` dispatches of the kernel named
+`--kernel` (any template arguments, or exactly those given, as in
+k_pcg<double>) and divides by `--iterationen` (the PCG iterations those
+dispatches ran, from the bench line of the same run). The databases are
 rocprofv3's rocpd SQLite output (view counters_collection).
 """
 from __future__ import annotations
@@ -29,7 +30,10 @@ def werte(db: str, zaehler: str, kernel: str) -> list[float]:
             "order by dispatch_id", (zaehler,)).fetchall()
     finally:
         c.close()
-    return [float(v) for _, name, v in zeilen if name.split("(")[0].split("<")[0].split(" ")[-1] == kernel]
+    def passt(name: str) -> bool:
+        voll = name.split("(")[0].split(" ")[-1]  # with the template arguments, e.g. k_pcg<double>
+        return voll == kernel if "<" in kernel else voll.split("<")[0] == kernel
+    return [float(v) for _, name, v in zeilen if passt(name)]
 
 
 def faktor(db: str, zaehler: str) -> float:
