@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ohne-vergleich", action="store_true",
                     help="stapel64: skip the sequential comparison (for counter runs)")
+    ap.add_argument("--stapel-eager", action="store_true",
+                    help="stapel64: eager batched steps instead of graph replays (for counter runs)")
     ap.add_argument("--bloecke", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lds-kb", type=int, default=0, help="LDS a PCG tile may fill (KiB); 0 = the library's choice")
@@ -329,15 +331,18 @@ def stapel_lauf(args) -> int:
     st.schritte(dt, 1)
     wahl = st.autotune(dt)
     print(f"[bench] autotune: {wahl}", file=sys.stderr, flush=True)
+    # graph replays by default; --stapel-eager for counter runs (the profiler's
+    # counter collection does not take graph launches)
+    schritt = st.schritte if args.stapel_eager else st.schritte_graph
     # the warm-up in chunks of 10 steps, so that it reports progress
     for k in range(1, warmup, 10):
-        st.schritte_graph(dt, min(10, warmup - k))
+        schritt(dt, min(10, warmup - k))
         print(f"[bench] warm-up steps 1-{min(k + 9, warmup - 1) + 1} done", file=sys.stderr, flush=True)
     nb, threads = st.bloecke()
     for g in A:
         g.statistik(reset=True)
     t_a = time.perf_counter()
-    st.schritte_graph(dt, steps)
+    schritt(dt, steps)
     t_b = time.perf_counter()
     wall_ms = (t_b - t_a) * 1e3
     value = steps / (wall_ms / 1e3)
@@ -390,7 +395,7 @@ def stapel_lauf(args) -> int:
                                "x10^U(-0.5,0.5), friction U(0.1,0.5), shared 256^3 SDF, "
                                + ("self-collision on, " if selbst else "self-collision off, ") + "dt 1/120, rtol 1e-6",
                    "timed_steps": f"{warmup + 1}-{warmup + steps}",
-                   "punkte": sum(punkte), "systeme": len(satz), "graph": True, "praezision": args.praezision,
+                   "punkte": sum(punkte), "systeme": len(satz), "graph": not args.stapel_eager, "praezision": args.praezision,
                    "threads": threads, "bloecke": int(nb.sum()), "autotune_ms_loesung": round(wahl["ms_loesung"], 3),
                    "bloecke_je_system_max": int(nb.max()),
                    "sequentiell_steps_s": round(seq, 3) if seq else None,
