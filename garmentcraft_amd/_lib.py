@@ -161,6 +161,8 @@ SIGNATUREN = [
     ("gc_batch_bloecke", C.c_int, [C.c_void_p, I32, C.POINTER(C.c_int32)]),
     ("gc_batch_zeiten", C.c_int, [C.c_void_p, D, D]),
     ("gc_adjoint_step", C.c_int, [C.c_void_p, C.c_double]),
+    ("gc_step_aufnehmen", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
+    ("gc_adjoint_step_aus", C.c_int, [C.c_void_p, C.c_double, C.c_int32]),
     ("gc_adjoint_set", C.c_int, [C.c_void_p, D, D]),
     ("gc_adjoint_get", C.c_int, [C.c_void_p, D, D]),
     ("gc_adjoint_gradient", C.c_int, [C.c_void_p, D, D, D, D, D, C.c_int32]),

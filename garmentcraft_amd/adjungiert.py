@@ -16,7 +16,8 @@ drape.
   adjoint are summed back onto theta by the chain rule.
 * :class:`Rollout` runs n steps from a fixed start state, saving the state
   every ``alle`` steps in device memory; its reverse pass re-runs each
-  segment from its checkpoint and steps the adjoint back through it.
+  segment from its checkpoint, keeping each step's solutions, and steps the
+  adjoint back through it with those solutions.
 * :class:`RolloutFunktion` wraps a rollout as a ``torch.autograd.Function``
   (theta -> final positions) for checks with torch.
 """
@@ -89,13 +90,17 @@ class PanelMaterial:
 class Rollout:
     """n steps of length h from the scene's state on a device context, with a
     checkpoint every ``alle`` steps (slots of gc_zustand_sichern) and the
-    adjoint pass back through them."""
+    adjoint pass back through them. With ``wiedergabe`` (the default) the
+    reverse pass's segment re-run records each step's solutions and the
+    adjoint step replays them (gc_step_aufnehmen, gc_adjoint_step_aus), the
+    same gradient to the bit with one forward solve less per step."""
 
     def __init__(self, g: GeraetSzene, s: Szene, material: PanelMaterial, h: float, schritte: int, alle: int = 10,
-                 x0=None, v0=None, dv_vorher=None):
+                 x0=None, v0=None, dv_vorher=None, wiedergabe: bool = True):
         if schritte < 1 or alle < 1:
             raise ValueError("schritte and alle must be >= 1")
         self.g, self.s, self.material, self.h, self.schritte, self.alle = g, s, material, h, schritte, alle
+        self.wiedergabe = wiedergabe
         g.set_state(s.x if x0 is None else x0, s.v if v0 is None else v0)
         g.set_dv_vorher(dv_vorher)
         g.sichern(START_SLOT)
@@ -141,13 +146,22 @@ class Rollout:
         for seg in reversed(range(segmente)):
             n0, n1 = seg * self.alle, min((seg + 1) * self.alle, self.schritte)
             g.laden(seg)
+            # the segment re-run keeps each step's solutions with the state
+            # before it, so that the adjoint step does not solve it again (the
+            # segment's last step is taken only by its adjoint step)
             for k in range(n0, n1):
-                g.sichern(TEMP_SLOT + k - n0)
-                if k + 1 < n1:
-                    g.schritte(self.h, 1)
+                if k + 1 < n1 and self.wiedergabe:
+                    g.schritt_aufnehmen(self.h, TEMP_SLOT + k - n0)
+                else:
+                    g.sichern(TEMP_SLOT + k - n0)
+                    if k + 1 < n1:
+                        g.schritte(self.h, 1)
             for k in reversed(range(n0, n1)):
-                g.laden(TEMP_SLOT + k - n0)
-                g.adjoint_schritt(self.h)
+                if k + 1 < n1 and self.wiedergabe:
+                    g.adjoint_schritt_aus(self.h, TEMP_SLOT + k - n0)
+                else:
+                    g.laden(TEMP_SLOT + k - n0)
+                    g.adjoint_schritt(self.h)
         self.statistik = g.adjoint_statistik()
         return self.material.zurueck(g.adjoint_gradient(self.s, reset=True))
 

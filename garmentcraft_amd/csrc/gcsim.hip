@@ -203,8 +203,18 @@ struct gc_ctx {
     V3<double>* dvv = nullptr;
     uint32_t* folge = nullptr;   // self-collision push-spring sequence per triangle (selbstkollision.rs:1024-1026)
     bool hat_vorher = false;
+    // the solutions of the step taken from this state (gc_step_aufnehmen):
+    // dv after solve 1 and solve 2 in the bulk type, and for a refined FP32
+    // solve its FP64 solution after each
+    void* dv_loes[2] = {nullptr, nullptr};
+    V3<double>* dv64_loes[2] = {nullptr, nullptr};
+    bool hat_loesung = false;
   };
   std::map<int, Zustand> zustaende;
+  // while a step records its solutions into a slot, or takes them from one
+  // instead of solving (set only for the duration of that step)
+  Zustand* aufnahme = nullptr;
+  const Zustand* wiedergabe = nullptr;
   // recorded decisions of steps by slot (gc_entscheidungen)
   std::map<int, uint16_t*> entscheidungen;
   // the state at the start of the current gc_step / gc_step_graph call, with
@@ -1144,17 +1154,40 @@ static int launch_nach_pcg(gc_ctx* c) {
   return 0;
 }
 
+// Solve k (0, 1) of a substep, or its recorded result instead (c->wiedergabe);
+// with c->aufnahme the result is kept. The solution is dv, and for a refined
+// FP32 solve also its FP64 solution dv64, which the release pass and the
+// update read.
+template <typename T>
+static int loesung(gc_ctx* c, int k) {
+  const SysDev<T>& s = sys<T>(c);
+  const size_t n = (size_t)s.n;
+  if (c->wiedergabe) {
+    HIPCK(c, hipMemcpyAsync(s.dv, c->wiedergabe->dv_loes[k], n * sizeof(V3<T>), hipMemcpyDeviceToDevice, c->stream));
+    if (s.dv64)
+      HIPCK(c, hipMemcpyAsync(s.dv64, c->wiedergabe->dv64_loes[k], n * sizeof(V3<double>), hipMemcpyDeviceToDevice, c->stream));
+  } else if (launch_pcg<T>(c, k)) {
+    return 1;
+  }
+  if (c->aufnahme) {
+    HIPCK(c, hipMemcpyAsync(c->aufnahme->dv_loes[k], s.dv, n * sizeof(V3<T>), hipMemcpyDeviceToDevice, c->stream));
+    if (s.dv64)
+      HIPCK(c, hipMemcpyAsync(c->aufnahme->dv64_loes[k], s.dv64, n * sizeof(V3<double>), hipMemcpyDeviceToDevice, c->stream));
+  }
+  return 0;
+}
+
 template <typename T>
 // ev: four events (before/after each PCG launch) or nullptr
 static int launch_schritt(gc_ctx* c, int hat_vorher, const hipEvent_t* ev) {
   hipStream_t st = c->stream;
   if (launch_vor_pcg<T>(c, hat_vorher)) return 1;
   if (ev) HIPCK(c, hipEventRecord(ev[0], st));
-  if (launch_pcg<T>(c, 0)) return 1;
+  if (loesung<T>(c, 0)) return 1;
   if (ev) HIPCK(c, hipEventRecord(ev[1], st));
   if (launch_zwischen_pcg<T>(c)) return 1;
   if (ev) HIPCK(c, hipEventRecord(ev[2], st));
-  if (launch_pcg<T>(c, 1)) return 1;
+  if (loesung<T>(c, 1)) return 1;
   if (ev) HIPCK(c, hipEventRecord(ev[3], st));
   return launch_nach_pcg<T>(c);
 }
@@ -3686,6 +3719,42 @@ int gc_batch_zeiten(gc_batch* b, double* ms_schritt, double* ms_pcg) {
 int gc_adjoint_step(gc_ctx* c, double h) {
   if (!c) return 1;
   return c->opt.praezision == GC_FP32 ? adj_schritt<float>(c, h) : adj_schritt<double>(c, h);
+}
+
+int gc_step_aufnehmen(gc_ctx* c, double h, int32_t slot) {
+  if (!c || !c->topo) return 1;
+  const bool neu = c->zustaende.find(slot) == c->zustaende.end();
+  gc_ctx::Zustand& z = c->zustaende[slot];
+  z.hat_loesung = false;
+  if (zustand_sichern_in(c, z)) {
+    if (neu && !z.x) c->zustaende.erase(slot);
+    return 1;
+  }
+  const size_t n = (size_t)c->n;
+  const bool fp32 = c->opt.praezision == GC_FP32;
+  for (int k = 0; k < 2; ++k) {
+    if (!z.dv_loes[k]) z.dv_loes[k] = fp32 ? (void*)dalloc<V3<float>>(c, n) : (void*)dalloc<V3<double>>(c, n);
+    if (fp32 && c->sf.dv64 && !z.dv64_loes[k]) z.dv64_loes[k] = dalloc<V3<double>>(c, n);
+    if (!z.dv_loes[k] || (fp32 && c->sf.dv64 && !z.dv64_loes[k])) return fehler(c, "hipMalloc failed (recorded solutions)");
+  }
+  c->aufnahme = &z;
+  const int rc = gc_step(c, h, 1);
+  c->aufnahme = nullptr;
+  z.hat_loesung = rc == 0;
+  return rc;
+}
+
+int gc_adjoint_step_aus(gc_ctx* c, double h, int32_t slot) {
+  if (!c || !c->topo) return 1;
+  auto it = c->zustaende.find(slot);
+  if (it == c->zustaende.end() || !it->second.hat_loesung)
+    return fehler(c, "gc_adjoint_step_aus: no step recorded in this slot (gc_step_aufnehmen)");
+  if (zustand_laden_aus(c, it->second)) return 1;
+  alle_param(c);
+  c->wiedergabe = &it->second;
+  const int rc = c->opt.praezision == GC_FP32 ? adj_schritt<float>(c, h) : adj_schritt<double>(c, h);
+  c->wiedergabe = nullptr;
+  return rc;
 }
 
 int gc_adjoint_set(gc_ctx* c, const double* gx, const double* gv) {

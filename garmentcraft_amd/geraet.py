@@ -455,6 +455,14 @@ class GeraetSzene:
     def adjoint_schritt(self, h: float) -> None:
         self._ck(self.lib.gc_adjoint_step(self.h, h), "gc_adjoint_step")
 
+    def schritt_aufnehmen(self, h: float, slot: int) -> None:
+        """One step after saving the state into slot, keeping its solutions (gc_step_aufnehmen)."""
+        self._ck(self.lib.gc_step_aufnehmen(self.h, h, int(slot)), "gc_step_aufnehmen")
+
+    def adjoint_schritt_aus(self, h: float, slot: int) -> None:
+        """The adjoint step from slot's state with its recorded solutions (gc_adjoint_step_aus)."""
+        self._ck(self.lib.gc_adjoint_step_aus(self.h, h, int(slot)), "gc_adjoint_step_aus")
+
     def adjoint_gradient(self, s: Szene, reset: bool = False) -> dict:
         """Accumulated parameter gradients: per input edge, seam, bending
         constraint, membrane (E_k, E_s, G) and point (friction)."""

@@ -404,6 +404,16 @@ int gc_batch_zeiten(gc_batch* b, double* ms_schritt, double* ms_pcg);
  * forward saving states (gc_zustand_sichern), then for each step from the
  * last: gc_zustand_laden of the state before it, gc_adjoint_step. */
 int gc_adjoint_step(gc_ctx* ctx, double h);
+/* A rollout's reverse pass re-runs each segment from its checkpoint to get the
+ * states before its steps; gc_step_aufnehmen takes such a step (as
+ * gc_step(ctx, h, 1)) after saving the state into slot, and keeps the step's
+ * two solutions with it. gc_adjoint_step_aus loads that state and runs
+ * gc_adjoint_step with the recorded solutions in place of the forward's two
+ * PCG solves (the same dv, to the bit), so the step's forward is not solved
+ * a second time. The slot is valid for the same h, parameters and bodies as
+ * the recording. */
+int gc_step_aufnehmen(gc_ctx* ctx, double h, int32_t slot);
+int gc_adjoint_step_aus(gc_ctx* ctx, double h, int32_t slot);
 /* Seed / read the adjoint state: dL/dx and dL/dv (double[3n] each; NULL sets zero). */
 int gc_adjoint_set(gc_ctx* ctx, const double* gx, const double* gv);
 int gc_adjoint_get(gc_ctx* ctx, double* gx, double* gv);

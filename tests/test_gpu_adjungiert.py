@@ -132,6 +132,26 @@ def test_checkpoint_abstand_aendert_nichts(aufbau, fd):
     assert np.array_equal(gr[0], gr[1]) and np.array_equal(gr[0], gr[2])
 
 
+@pytest.mark.parametrize("praezision", ["fp64", "fp32"])
+def test_wiedergabe_wie_neu_geloest(aufbau, fd, praezision):
+    """The reverse pass with the segment re-run's solutions replayed
+    (gc_step_aufnehmen, gc_adjoint_step_aus) gives the gradient and the start
+    state's adjoint of the pass that solves every step again, to the bit."""
+    a = aufbau
+    out = []
+    for wiedergabe in (True, False):
+        g, r = geraet_rollout(a, praezision, tol=1e-8, alle=SCHRITTE)
+        r.wiedergabe = wiedergabe
+        x = r.vorwaerts(a["th0"])
+        d = r.rueckwaerts(verlust_gradient(x, fd["ziel"]))
+        out.append((d, *r.anfangsgradient(), r.statistik["cg_iterationen"], r.statistik["abweichungen"]))
+        g.close()
+    print(praezision, "adjoint CG iterations", out[0][3], out[1][3], "deviations", out[0][4], out[1][4])
+    for u, w in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(u, w)
+    assert out[0][4] == 0 and out[1][4] == 0
+
+
 def test_torch_autograd_wie_direkt(aufbau, fd):
     import torch
     a = aufbau
