@@ -81,6 +81,9 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lds-kb", type=int, default=0, help="LDS a PCG tile may fill (KiB); 0 = the library's choice")
     ap.add_argument("--ohne-selbstkollision", action="store_true", help="self-collision off (the plain drape)")
+    ap.add_argument("--koerperpunkte", type=int, default=0,
+                    help="netz40k: passes of body points against cloth triangles per step (0, the reference's "
+                         "default, switches them off)")
     ap.add_argument("--workload", default="drape40k",
                     choices=["drape40k", "stapel64", "adjoint20k", "gross250k", "netz40k", "xpbd40k"],
                     help="drape40k: the 40k-point drape; stapel64: 64 garments of 2k-60k points in one batch; "
@@ -127,7 +130,8 @@ def main() -> int:
     print(f"[bench] scene {bericht} built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     dt = s.param.bild_dt
     selbst = not args.ohne_selbstkollision
-    param = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0, selbstkollision=selbst)
+    param = ImplizitParameter(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0, selbstkollision=selbst,
+                              koerperpunkt_durchgaenge=args.koerperpunkte)
     g = GeraetSzene(s, param, praezision=args.praezision, bloecke=args.bloecke, threads=args.threads,
                     lds_kb=args.lds_kb)
     print(f"[bench] device: {g.name()}", file=sys.stderr, flush=True)
@@ -182,7 +186,7 @@ def main() -> int:
     if not args.no_cpu:
         import oracle as O
         o = O.OrakelSim(s, O.param(schritte_je_bild=1, cg_max=10000, cg_toleranz=1e-6, naht_k=500.0,
-                                   selbstkollision=1 if selbst else 0))
+                                   selbstkollision=1 if selbst else 0, koerperpunkt_durchgaenge=args.koerperpunkte))
         o.setze_lage(x_w, v_w)
         # warm start of the oracle: the device's dv of the last warm-up step is
         # not exported; start from h*g like a fresh solver, which costs the CPU
@@ -229,6 +233,8 @@ def main() -> int:
                    "ms_geraet_je_schritt": round(ms_gesamt / args.steps, 3) if ms_gesamt == ms_gesamt else None,
                    "naehe_je_schritt": round((st["naehe_punkt_dreieck"] + st["naehe_kante_kante"]) / args.steps, 1),
                    "stoesse": st["selbst_stoesse"], "durchdringungen": st["durchdringungen"],
+                   **({"koerperpunkt_durchgaenge": args.koerperpunkte, "koerperpunkte": st["koerperpunkte"]}
+                      if args.koerperpunkte else {}),
                    "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
