@@ -3918,6 +3918,36 @@ extern "C" int gc_phasen_bloecke(gc_ctx* c, double* out, int32_t n, int32_t rese
 }
 #endif
 
+#ifdef GC_PHASEN
+}  // extern "C"
+// out (4 x n): per tile its first row, its end row, the spring list entries
+// of its rows after the last rebuild and its listed elements, so that the
+// per-block times of one-tile blocks can be set against the tile's work.
+template <typename T>
+static int phasen_kacheln(gc_ctx* c, int32_t* out, int32_t n) {
+  const SysDev<T>& s = sys<T>(c);
+  const int nt = std::min<int>(n, s.n_kacheln);
+  std::vector<int32_t> kz(2 * (size_t)s.n_kacheln);
+  std::vector<uint32_t> el((size_t)s.n_kacheln + 1), fe((size_t)s.n + 1, 0u);
+  if (hipMemcpy(kz.data(), s.kachel_zeilen, kz.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(el.data(), s.kachel_el_off, el.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return -1;
+  if (s.selbst && s.fe_off && hipMemcpy(fe.data(), s.fe_off, fe.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  for (int t = 0; t < nt; ++t) {
+    out[4 * t] = kz[2 * t];
+    out[4 * t + 1] = kz[2 * t + 1];
+    out[4 * t + 2] = (int32_t)(fe[kz[2 * t + 1]] - fe[kz[2 * t]]);
+    out[4 * t + 3] = (int32_t)(el[t + 1] - el[t]);
+  }
+  return nt;
+}
+extern "C" {
+int gc_phasen_kacheln(gc_ctx* c, int32_t* out, int32_t n) {
+  if (!c || !out) return -1;
+  return c->opt.praezision == GC_FP32 ? phasen_kacheln<float>(c, out, n) : phasen_kacheln<double>(c, out, n);
+}
+#endif
+
 int gc_entscheidungen_abweichungen(gc_ctx* c, int64_t* anzahl, int32_t reset) {
   if (!c || !anzahl) return 1;
   GridState g;

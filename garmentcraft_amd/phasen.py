@@ -46,6 +46,8 @@ def laden():
     L.gc_phasen_lesen.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     L.gc_phasen_bloecke.restype = C.c_int
     L.gc_phasen_bloecke.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+    L.gc_phasen_kacheln.restype = C.c_int
+    L.gc_phasen_kacheln.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     return L
 
 
@@ -86,8 +88,25 @@ def main(argv=None) -> None:
         for k, name in enumerate(["A-C", "update", "wait_pAp", "wait_rz"]):
             q = np.percentile(b[:, k], [0, 50, 100]) if len(b) else [0.0, 0.0, 0.0]
             pro_block[name] = {"min": round(float(q[0]), 3), "median": round(float(q[1]), 3), "max": round(float(q[2]), 3)}
+        # the slowest one-tile blocks against their tiles' work (spring list
+        # entries after the window's last rebuild, listed elements, rows)
+        kach = np.zeros(4 * 1024, dtype=np.int32)
+        nk = L.gc_phasen_kacheln(g.h, kach.ctypes.data, 1024)
+        kach = kach.reshape(1024, 4)[:max(nk, 0)]
+        arbeit = {}
+        if nk > 0 and len(b) == nk:
+            ac = b[:, 0]
+            reihe = np.argsort(ac)[::-1]
+            arbeit = {
+                "slowest": [{"block": int(r), "A-C_us": round(float(ac[r]), 2), "springs": int(kach[r, 2]),
+                             "elements": int(kach[r, 3]), "rows": int(kach[r, 1] - kach[r, 0])} for r in reihe[:8]],
+                "median_springs": float(np.median(kach[:, 2])), "median_elements": float(np.median(kach[:, 3])),
+                "corr_A-C_springs": round(float(np.corrcoef(ac, kach[:, 2])[0, 1]), 3) if kach[:, 2].std() > 0 else None,
+                "corr_A-C_elements": round(float(np.corrcoef(ac, kach[:, 3])[0, 1]), 3),
+            }
         print(json.dumps({
             "steps": f"{lo + 1}-{hi}", "iterations": n, "cg_iterations": st["cg_iterationen"],
+            "tiles": arbeit,
             "block0_us": {k: round(float(us[i]), 3) for i, k in enumerate(NAMEN)},
             "block0_sum_us": round(float(us[:6].sum()), 3),
             "leader_partial_to_result_us": round(float(us[6]), 3),
