@@ -174,9 +174,18 @@ __device__ __forceinline__ bool sk_bereich_ok(const Quader& q, double zelle) {  
     if (!(isfinite(q.lo[a]) && isfinite(q.hi[a]) && (q.hi[a] - q.lo[a]) <= 64.0 * zelle)) return false;
   return true;
 }
-__device__ __forceinline__ uint32_t sk_streue(int64_t i, int64_t j, int64_t k, uint32_t n) {  // :274-278
+// u mod n by multiplication (Lemire, Kaser and Kurz, "Faster remainder by
+// direct computation", 2019): with m = ceil(2^64 / n), u mod n is the high
+// 64 bits of (m u mod 2^64) n, exactly, for every 32-bit u and n >= 1 (n = 1
+// gives m = 0 and the remainder 0). The hash below takes its remainder this
+// way, a few integer multiplies instead of the software division.
+__device__ __forceinline__ uint64_t sk_rest_m(uint32_t n) { return ~0ull / (uint64_t)n + 1ull; }
+__device__ __forceinline__ uint32_t sk_rest(uint32_t u, uint64_t m, uint32_t n) {
+  return (uint32_t)__umul64hi(m * (uint64_t)u, (uint64_t)n);
+}
+__device__ __forceinline__ uint32_t sk_streue(int64_t i, int64_t j, int64_t k, uint32_t n, uint64_t m) {  // :274-278
   const uint32_t h = ((uint32_t)i * 0x8DA6B343u) ^ ((uint32_t)j * 0xD8163841u) ^ ((uint32_t)k * 0xCB1AB31Fu);
-  return h % n;
+  return sk_rest(h, m, n);
 }
 __device__ __forceinline__ void sk_quader_rein(Quader& q, D3 p, double m) {  // one point of quader_punkte (:347-357)
   for (int a = 0; a < 3; ++a) {
@@ -346,8 +355,8 @@ __device__ __forceinline__ bool sk_zellen(const Quader& q, double zelle, int64_t
 // An entry with more than SK_KLEIN_ZELLEN cells is left to k_sk_gross; the
 // counting pass (zaehlen) puts it on the list of such entries.
 template <typename F>
-__device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, uint32_t e,
-                                            bool zaehlen, F f) {
+__device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q, double zelle, uint32_t eimer, uint64_t m,
+                                            uint32_t e, bool zaehlen, F f) {
   int64_t lo[3], hi[3];
   if (!sk_zellen(q, zelle, lo, hi)) return;
   const int nc = (int)((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
@@ -363,11 +372,11 @@ __device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q,
   const int ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
   int i = 0, j = 0, k = 0;
   for (int c = 0; c < nc; ++c) {
-    const uint32_t h = sk_streue(lo[0] + i, lo[1] + j, lo[2] + k, eimer);
+    const uint32_t h = sk_streue(lo[0] + i, lo[1] + j, lo[2] + k, eimer, m);
     bool neu = true;
     int i2 = 0, j2 = 0, k2 = 0;
     for (int d = 0; d < c && neu; ++d) {
-      neu = sk_streue(lo[0] + i2, lo[1] + j2, lo[2] + k2, eimer) != h;
+      neu = sk_streue(lo[0] + i2, lo[1] + j2, lo[2] + k2, eimer, m) != h;
       if (++k2 == nz) {
         k2 = 0;
         if (++j2 == ny) { j2 = 0; ++i2; }
@@ -384,16 +393,18 @@ __device__ __forceinline__ void sk_je_eimer(const SelbstDev& s, const Quader& q,
 __global__ void k_sk_eimer_zaehlen(const SelbstDev* __restrict__ sp, int anzahl, const double* zelle_p, uint32_t eimer) {
   const SelbstDev& s = *sp;
   const double zelle = *zelle_p;
+  const uint64_t m = sk_rest_m(eimer);
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < anzahl; e += gridDim.x * blockDim.x)
-    sk_je_eimer(s, s.quader[e], zelle, eimer, (uint32_t)e, true, [&](uint32_t h) { atomicAdd(&s.eimer_fuell[h], 1u); });
+    sk_je_eimer(s, s.quader[e], zelle, eimer, m, (uint32_t)e, true, [&](uint32_t h) { atomicAdd(&s.eimer_fuell[h], 1u); });
 }
 
 __global__ void k_sk_eimer_fuellen(const SelbstDev* __restrict__ sp, int anzahl, const double* zelle_p, uint32_t eimer) {
   const SelbstDev& s = *sp;
   const double zelle = *zelle_p;
   if (s.zaehler[SK_Z_EINTRAEGE] > s.kapazitaet_eintraege) return;
+  const uint64_t m = sk_rest_m(eimer);
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < anzahl; e += gridDim.x * blockDim.x)
-    sk_je_eimer(s, s.quader[e], zelle, eimer, (uint32_t)e, false, [&](uint32_t h) {
+    sk_je_eimer(s, s.quader[e], zelle, eimer, m, (uint32_t)e, false, [&](uint32_t h) {
       const uint32_t pos = atomicAdd(&s.eimer_fuell[h], 1u);
       s.eintraege_roh[pos] = (uint32_t)e;
     });
@@ -412,6 +423,7 @@ __global__ void __launch_bounds__(1024) k_sk_gross(const SelbstDev* __restrict__
   if (fuellen && s.zaehler[SK_Z_EINTRAEGE] > s.kapazitaet_eintraege) return;
   const uint32_t ng = min(s.zaehler[SK_Z_GROSS], s.kapazitaet_gross);
   const uint32_t woerter = (eimer + 31u) / 32u;
+  const uint64_t m = sk_rest_m(eimer);
   for (uint32_t g = 0; g < ng; ++g) {
     const uint32_t e = s.gross[g];
     int64_t lo[3], hi[3];
@@ -422,7 +434,7 @@ __global__ void __launch_bounds__(1024) k_sk_gross(const SelbstDev* __restrict__
       __hip_atomic_store(&s.eimer_bits[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-      const uint32_t h = sk_streue(lo[0] + c / (ny * nz), lo[1] + (c / nz) % ny, lo[2] + c % nz, eimer);
+      const uint32_t h = sk_streue(lo[0] + c / (ny * nz), lo[1] + (c / nz) % ny, lo[2] + c % nz, eimer, m);
       const uint32_t bit = 1u << (h & 31u);
       if (atomicOr(&s.eimer_bits[h >> 5], bit) & bit) continue;
       const uint32_t pos = atomicAdd(&s.eimer_fuell[h], 1u);
@@ -484,11 +496,12 @@ __device__ __forceinline__ void sk_welle_je_eintrag(const SelbstDev& s, const Qu
   const int nx = (int)(hi[0] - lo[0] + 1), ny = (int)(hi[1] - lo[1] + 1), nz = (int)(hi[2] - lo[2] + 1);
   const int nc = nx * ny * nz;
   const int nyz = ny * nz;
+  const uint64_t mr = sk_rest_m(eimer);
   for (int c0 = 0; c0 < nc; c0 += 64) {
     const int c = c0 + lane;
     uint32_t anf = 0, n = 0;
     if (c < nc) {
-      const uint32_t h = sk_streue(lo[0] + c / nyz, lo[1] + (c / nz) % ny, lo[2] + c % nz, eimer);
+      const uint32_t h = sk_streue(lo[0] + c / nyz, lo[1] + (c / nz) % ny, lo[2] + c % nz, eimer, mr);
       uint32_t u = s.eimer_anfang[h];
       const uint32_t b = s.eimer_anfang[h + 1];
       uint32_t o = b;
