@@ -288,6 +288,26 @@ def test_drape_auf_netz_koerperpunkte_wie_orakel(drape_auf_netz, durchgaenge):
     g.close()
 
 
+def test_drape_auf_netz_koerperpunkte_graph_wie_eager(drape_auf_netz):
+    """The passes of body points in a captured substep: 16 graph replays equal
+    16 eager steps to the bit, lifts included (each pass clears its own
+    counter and list, so replays start clean)."""
+    s = drape_auf_netz
+    pp = dict(P, cg_toleranz=1e-8, koerperpunkt_durchgaenge=2)
+    a = GeraetSzene(s, ImplizitParameter(**pp), praezision="fp32")
+    b = GeraetSzene(s, ImplizitParameter(**pp), praezision="fp32")
+    a.schritte(H, 16)
+    b.schritte_graph(H, 16)
+    xa, va = a.lage()
+    xb, vb = b.lage()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    ka, kb = a.statistik()["koerperpunkte"], b.statistik()["koerperpunkte"]
+    print("lifts eager", ka, "graph", kb)
+    assert ka == kb and ka >= 40
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("ausschnitt", [False, True], ids=["ganz", "angeschnitten"])
 def test_sdf_backen_wie_orakel(ausschnitt):
     """gc_netz_sdf_backen: nodes within the band carry the oracle's signed
