@@ -266,8 +266,8 @@ struct GridState {
   // the all-reduce's leader (block 0): ticks from its own partial to the
   // published sums (its wait for the other blocks included), and the count
   unsigned long long leiter[2];
-  // per block (rank r at 4 r): ticks of stages A-C, of the update, and of the
-  // waits in the two all-reduces (null: not recorded)
+  // per block (rank r at 6 r): ticks of stages A, B and C, of the update, and
+  // of the waits in the two all-reduces (null: not recorded)
   unsigned long long* pro_block;
 #endif
 };
@@ -1589,10 +1589,12 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
         }
         unsigned long long* pb = gs->pro_block;
         if (pb) {
-          pb[4 * rang] += tp[3] - tp[0];
-          pb[4 * rang + 1] += tp[5] - tp[4];
-          pb[4 * rang + 2] += tp[4] - tp[3];
-          pb[4 * rang + 3] += tp[6] - tp[5];
+          pb[6 * rang] += tp[1] - tp[0];
+          pb[6 * rang + 1] += tp[2] - tp[1];
+          pb[6 * rang + 2] += tp[3] - tp[2];
+          pb[6 * rang + 3] += tp[5] - tp[4];
+          pb[6 * rang + 4] += tp[4] - tp[3];
+          pb[6 * rang + 5] += tp[6] - tp[5];
         }
       }
 #endif

@@ -3895,25 +3895,25 @@ extern "C" int gc_phasen_lesen(gc_ctx* c, double* us7, int32_t reset) {
 #endif
 
 #ifdef GC_PHASEN
-// out (4 x n, microseconds summed over the recorded iterations): per block,
-// stages A-C, update, wait in the p.Ap all-reduce, wait in the r.z one. The
-// first call (out null) turns the record on.
+// out (6 x n, microseconds summed over the recorded iterations): per block,
+// stages A, B and C, update, wait in the p.Ap all-reduce, wait in the r.z
+// one. The first call (out null) turns the record on.
 extern "C" int gc_phasen_bloecke(gc_ctx* c, double* out, int32_t n, int32_t reset) {
   if (!c) return -1;
   GridState g;
   if (hipMemcpy(&g, c->gs, sizeof(GridState), hipMemcpyDeviceToHost) != hipSuccess) return -1;
   if (!g.pro_block) {
-    if (hipMalloc((void**)&g.pro_block, 4 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
-    if (hipMemset(g.pro_block, 0, 4 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
+    if (hipMalloc((void**)&g.pro_block, 6 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
+    if (hipMemset(g.pro_block, 0, 6 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
     if (hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice) != hipSuccess) return -1;
   }
   if (!out) return 0;
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->opt.geraet) != hipSuccess || khz <= 0) return -1;
-  std::vector<unsigned long long> h(4 * GC_MAX_BLOECKE);
+  std::vector<unsigned long long> h(6 * GC_MAX_BLOECKE);
   if (hipMemcpy(h.data(), g.pro_block, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess) return -1;
-  for (int k = 0; k < 4 * std::min<int>(n, GC_MAX_BLOECKE); ++k) out[k] = (double)h[k] / ((double)khz * 1e-3);
-  if (reset && hipMemset(g.pro_block, 0, 4 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
+  for (int k = 0; k < 6 * std::min<int>(n, GC_MAX_BLOECKE); ++k) out[k] = (double)h[k] / ((double)khz * 1e-3);
+  if (reset && hipMemset(g.pro_block, 0, 6 * GC_MAX_BLOECKE * sizeof(unsigned long long)) != hipSuccess) return -1;
   return 0;
 }
 #endif

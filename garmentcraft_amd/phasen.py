@@ -67,7 +67,7 @@ def main(argv=None) -> None:
     if L.gc_phasen_bloecke(g.h, None, 0, 0) != 0:
         raise _lib.GcFehler("gc_phasen_bloecke")
     us = np.zeros(7)
-    bl = np.zeros(4 * 1024)
+    bl = np.zeros(6 * 1024)
     fertig = 0
     for lo, hi in fenster:
         if lo > fertig:
@@ -81,11 +81,13 @@ def main(argv=None) -> None:
         L.gc_phasen_bloecke(g.h, bl.ctypes.data, 1024, 1)
         st = g.statistik()
         ms, pcg = g.zeiten()
-        b = bl.reshape(1024, 4) / max(n, 1)
-        aktiv = b[:, 0] > 0
-        b = b[aktiv]
+        b6 = bl.reshape(1024, 6) / max(n, 1)
+        aktiv = b6[:, 0] > 0
+        b6 = b6[aktiv]
+        # columns: A-C, update, waits (as before), then A, B, C alone
+        b = np.column_stack([b6[:, 0] + b6[:, 1] + b6[:, 2], b6[:, 3], b6[:, 4], b6[:, 5], b6[:, 0], b6[:, 1], b6[:, 2]])
         pro_block = {}
-        for k, name in enumerate(["A-C", "update", "wait_pAp", "wait_rz"]):
+        for k, name in enumerate(["A-C", "update", "wait_pAp", "wait_rz", "A", "B", "C"]):
             q = np.percentile(b[:, k], [0, 50, 100]) if len(b) else [0.0, 0.0, 0.0]
             pro_block[name] = {"min": round(float(q[0]), 3), "median": round(float(q[1]), 3), "max": round(float(q[2]), 3)}
         # the slowest one-tile blocks against their tiles' work (spring list
@@ -98,7 +100,8 @@ def main(argv=None) -> None:
             ac = b[:, 0]
             reihe = np.argsort(ac)[::-1]
             arbeit = {
-                "slowest": [{"block": int(r), "A-C_us": round(float(ac[r]), 2), "springs": int(kach[r, 2]),
+                "slowest": [{"block": int(r), "A-C_us": round(float(ac[r]), 2), "A_B_C_us": [round(float(b[r, k]), 2) for k in (4, 5, 6)],
+                             "springs": int(kach[r, 2]),
                              "elements": int(kach[r, 3]), "rows": int(kach[r, 1] - kach[r, 0])} for r in reihe[:8]],
                 "median_springs": float(np.median(kach[:, 2])), "median_elements": float(np.median(kach[:, 3])),
                 "corr_A-C_springs": round(float(np.corrcoef(ac, kach[:, 2])[0, 1]), 3) if kach[:, 2].std() > 0 else None,
