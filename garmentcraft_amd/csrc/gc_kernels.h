@@ -263,45 +263,42 @@ struct GridState {
   // all-reduce r.z) and the number of iterations they cover
   unsigned long long phasen[6];
   unsigned long long phasen_it;
-  // the all-reduce's leader (block 0): ticks from its own partial to the
-  // published sums (its wait for the other blocks included), and the count
-  unsigned long long leiter[2];
+  // block 0 in the all-reduces: ticks from its own partial to the summed
+  // result (its wait for the other blocks included), and the count
+  unsigned long long warte0[2];
   // per block (rank r at 6 r): ticks of stages A, B and C, of the update, and
   // of the waits in the two all-reduces (null: not recorded)
   unsigned long long* pro_block;
 #endif
 };
 
+// One block's partial of a grid all-reduce: value k as two 64-bit words
+// (2k: low half, 2k + 1: high half), each tagged with the barrier's epoch in
+// its upper 32 bits. One 64-byte line per block.
 #define GC_MAX_BLOECKE 1024
 struct alignas(64) Slot {
-  double v[4];
-  unsigned epoche;  // the barrier whose partial v holds (the all-reduce's check)
+  unsigned long long wort[8];
 };
 
-// A system's slot array (GC_SLOTS entries): the partials of the grid
-// all-reduce, one slot per block for each epoch parity, then one result line
-// per group of GC_GRUPPE blocks, each on a 128-byte line of its own.
-#define GC_GRUPPE 8
-#define GC_BARR_GRUPPEN (GC_MAX_BLOECKE / GC_GRUPPE)
-#define GC_BARR_BASIS (2 * GC_MAX_BLOECKE)
-#define GC_SLOTS (GC_BARR_BASIS + 2 * GC_BARR_GRUPPEN)
-__device__ __forceinline__ unsigned long long* barr_ergebnis(Slot* slots, unsigned g) {
-  return (unsigned long long*)&slots[GC_BARR_BASIS + 2 * g];
-}
-// Clears the partials and result lines (one block; a fill kernel's work, no
-// memset): an all-reduce reads a slot as this barrier's once it carries the
-// barrier's epoch, and the epoch starts anew at 1 after each clear.
+// A system's slot array (GC_SLOTS entries): one slot per block for each
+// epoch parity.
+#define GC_SLOTS (2 * GC_MAX_BLOECKE)
+// Clears the partials (one block; a fill kernel's work, no memset): an
+// all-reduce reads a slot as this barrier's once it carries the barrier's
+// epoch, and the epoch starts anew at 1 after each clear.
 __device__ __forceinline__ void barriere_leeren(Slot* slots) {
   for (unsigned k = threadIdx.x; k < 16 * GC_SLOTS; k += blockDim.x) ((unsigned*)slots)[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
-// Grid-wide all-reduce. Each block writes its partial into its slot; block 0
-// (the leader) waits for all nb partials, sums them in one fixed order and
-// publishes the sums, so every block holds the identical value
-// (deterministic, equal to the bit across blocks and across a batch's
-// systems of the same launch shape). Slots are double-buffered by epoch
-// parity.
+// Grid-wide all-reduce. Each block writes its partial into its slot, then
+// polls the partials of all nb blocks and sums them itself, in one fixed
+// order, so every block holds the identical value (deterministic, equal to
+// the bit across blocks and across a batch's systems of the same launch
+// shape). Slots are double-buffered by epoch parity: a block writes the
+// partial of barrier e + 2 into the slot of barrier e only after barrier
+// e + 1, which needs every block's partial of e + 1, so no block still reads
+// the slot for barrier e then.
 //
 // Block sums are deterministic: an xor butterfly inside each wave (every lane
 // ends with the same value, IEEE addition being commutative), then the wave
@@ -310,34 +307,28 @@ __device__ __forceinline__ void barriere_leeren(Slot* slots) {
 // batched launch). Wave 0 alone arrives, waits and hands the result to the
 // block through LDS (two workgroup barriers per grid barrier).
 //
-// Every value travels as two 64-bit words, (epoch << 32) | low half and
-// (epoch << 32) | high half. A 64-bit access is single-copy atomic, so a
-// reader that finds this barrier's epoch in every word of a slot holds this
-// barrier's value, with no counter or flag to order against it: a block's
-// arrival IS its partial's store, and the leader's publication IS the
-// result. The critical path is the last partial's store, the leader's poll
-// that sees it, the leader's stores of the result and the polls that see
-// them (where a counter tree adds two atomic round trips and every block
-// reading all partials contends on their few lines). The leader writes the
-// result into one line per group of GC_GRUPPE consecutive ranks, which that
-// group's blocks poll. A slot holding a later epoch than the barrier's is
-// stale (left from another launch): bit 31 of the result's epoch half then
-// marks the all-reduce failed. Slots and result lines are cleared before each
-// launch that starts the epoch anew.
+// A 64-bit access is single-copy atomic, so a reader that finds this
+// barrier's epoch in every word of a slot holds that block's partial for
+// this barrier, with no counter or flag to order against it: a block's
+// arrival IS its partial's store. The critical path is the last partial's
+// store and the polls that see it. (Where one leader block sums and
+// publishes the result for the others to poll, the result's store and a
+// second poll follow: one more trip through the level that is coherent
+// across the XCDs.) A slot holding a later epoch than the barrier's is stale
+// (left from another launch): the all-reduce then fails and the block says
+// why in gs. Slots are cleared before each launch that starts the epoch
+// anew.
 //
 // Release: each block writes back its stores (the L2 write-back to the
 // coherent level) before its partial. There is no acquire (no L2
 // invalidation): everything a block reads that another block wrote in this
-// launch (partials, results, dv / z / p of points it does not own) is read
-// with agent-scope loads, which the program order after the observed result
+// launch (partials, dv / z / p of points it does not own) is read with
+// agent-scope loads, which the program order after the observed partials
 // puts after every writer's write-back.
 // VOLL = false: the caller's stores before this all-reduce are not read by
 // other blocks until after the next full one, so the block's partial alone
 // is published; the L2 write-back of the release is left to that next
 // all-reduce.
-__device__ __forceinline__ unsigned long long* barr_teil(Slot* buf, unsigned r) {
-  return (unsigned long long*)&buf[r];
-}
 __device__ __forceinline__ unsigned long long barr_wort(double x, int haelfte, unsigned long long marke) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(x);
   return marke | (haelfte ? (b >> 32) : (b & 0xffffffffull));
@@ -363,7 +354,6 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
   __shared__ int s_fehler;
   if (wave == 0) {
     int fehler = 0;
-    const unsigned ng = (nb + GC_GRUPPE - 1) / GC_GRUPPE;
     const unsigned long long marke = (unsigned long long)epoch << 32;
     // the block's partial: word 2k + h holds half h of value k (the waves'
     // sums in wave order); lane l stores word l mod 2 NV, so that no branch
@@ -376,100 +366,61 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
 #ifdef GC_PHASEN
     const unsigned long long t_an = wall_clock64();
 #endif
-    __hip_atomic_store(barr_teil(buf, rang) + kw, barr_wort(a, kw & 1, marke), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&buf[rang].wort[kw], barr_wort(a, kw & 1, marke), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the nb partials, summed in the fixed order (lane l takes l, l + 64,
+    // ..., four slots per round, then the wave's butterfly); each lane waits
+    // for its own slots
     double acc[NV];
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
     unsigned spins = 0;
-    if (rang == 0) {
-      // the leader: the nb partials, summed in the fixed order (lane l takes
-      // l, l + 64, ..., four slots per round, then the wave's butterfly);
-      // each lane waits for its own slots
-      bool spaet = false;  // a slot of a later barrier: stale
-      for (unsigned b0 = lane; b0 < nb && !fehler; b0 += 256) {
-        unsigned long long x[4][2 * NV];
-        for (;;) {
-          bool da = true;
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const bool gibt = b0 + 64u * u < nb;
-#pragma unroll
-            for (int k = 0; k < 2 * NV; ++k) {
-              x[u][k] = gibt ? __hip_atomic_load(barr_teil(buf, b0 + 64u * u) + k, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT)
-                             : marke;
-              const unsigned e = (unsigned)(x[u][k] >> 32);
-              da = da && e == epoch;
-              spaet = spaet || (e > epoch && e != 0u);
-            }
-          }
-          if (da || spaet) break;
-          if (++spins > (1u << 22)) { fehler = 1; break; }
-          if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            fehler = 1;
-            break;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (b0 + 64u * u < nb)
-#pragma unroll
-            for (int k = 0; k < NV; ++k)
-              acc[k] += __longlong_as_double((long long)((x[u][2 * k + 1] << 32) | (x[u][2 * k] & 0xffffffffull)));
-      }
-      if (__ballot(fehler == 1) != 0ull) fehler = 1;
-      else if (__ballot(spaet) != 0ull) fehler = 2;
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
-      // publish: lane q < ng writes group q's line (every word carries the
-      // epoch, so the words' order does not matter); a failed all-reduce
-      // publishes too, with bit 31 set, so that no block waits for it
-      const unsigned long long m = ((unsigned long long)(epoch | (fehler ? 0x80000000u : 0u))) << 32;
-      for (unsigned q = lane; q < ng; q += 64) {
-        unsigned long long* z = barr_ergebnis(slots, q);
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-          __hip_atomic_store(z + 2 * k, barr_wort(acc[k], 0, m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_store(z + 2 * k + 1, barr_wort(acc[k], 1, m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
-    } else {
-      // lane l reads word l mod 2 NV of the group's line until every word
-      // carries this barrier's epoch
-      const unsigned long long* z = barr_ergebnis(slots, rang / GC_GRUPPE);
-      unsigned long long w = 0;
+    bool spaet = false;  // a slot of a later barrier: stale
+    for (unsigned b0 = lane; b0 < nb && !fehler; b0 += 256) {
+      unsigned long long x[4][2 * NV];
       for (;;) {
-        w = __hip_atomic_load(z + kw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const bool da = ((unsigned)(w >> 32) & 0x7fffffffu) == epoch;
-        if (__ballot(!da) == 0ull) break;
+        bool da = true;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool gibt = b0 + 64u * u < nb;
+#pragma unroll
+          for (int k = 0; k < 2 * NV; ++k) {
+            x[u][k] = gibt ? __hip_atomic_load(&buf[b0 + 64u * u].wort[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : marke;
+            const unsigned e = (unsigned)(x[u][k] >> 32);
+            da = da && e == epoch;
+            spaet = spaet || (e > epoch && e != 0u);
+          }
+        }
+        if (da || spaet) break;
         if (++spins > (1u << 22)) { fehler = 1; break; }
         if ((spins & 63u) == 0u && __hip_atomic_load(&gs->fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           fehler = 1;
           break;
         }
       }
-      // the leader failed (and says why in gs)
-      if (!fehler && __ballot((w >> 63) != 0ull) != 0ull) fehler = 3;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const unsigned long long lo = __shfl(w & 0xffffffffull, 2 * k, 64);
-        const unsigned long long hi = __shfl(w & 0xffffffffull, 2 * k + 1, 64);
-        acc[k] = __longlong_as_double((long long)((hi << 32) | lo));
-      }
+      for (int u = 0; u < 4; ++u)
+        if (b0 + 64u * u < nb)
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            acc[k] += __longlong_as_double((long long)((x[u][2 * k + 1] << 32) | (x[u][2 * k] & 0xffffffffull)));
     }
+    if (__ballot(fehler == 1) != 0ull) fehler = 1;
+    else if (__ballot(spaet) != 0ull) fehler = 2;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o, 64);
 #ifdef GC_PHASEN
     if (lane == 0 && rang == 0) {
-      gs->leiter[0] += wall_clock64() - t_an;
-      gs->leiter[1] += 1ull;
+      gs->warte0[0] += wall_clock64() - t_an;
+      gs->warte0[1] += 1ull;
     }
 #endif
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) spins = max(spins, (unsigned)__shfl_xor((int)spins, o, 64));
     if (lane == 0) {
       for (int k = 0; k < NV; ++k) aus[k] = acc[k];
-      if (fehler == 1 || fehler == 2)
+      if (fehler)
         __hip_atomic_store(&gs->fehler, (unsigned)fehler, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_max(&gs->spins_max, spins, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_fehler = fehler;

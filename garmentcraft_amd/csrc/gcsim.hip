@@ -3869,8 +3869,8 @@ int gc_entscheidungen(gc_ctx* c, int32_t modus, int32_t slot) {
 // the p.Ap all-reduce, the update and the r.z all-reduce, over the
 // iterations since the last reset; returns the iteration count.
 // us7: the six phases of a PCG iteration seen by block 0 (stage A, stage B,
-// stage C, all-reduce p.Ap, update, all-reduce r.z), then the leader's time
-// from its partial to the published sums per all-reduce, in microseconds;
+// stage C, all-reduce p.Ap, update, all-reduce r.z), then block 0's time
+// from its partial to the summed result per all-reduce, in microseconds;
 // returns the iterations they cover
 extern "C" int gc_phasen_lesen(gc_ctx* c, double* us7, int32_t reset) {
   if (!c || !us7) return -1;
@@ -3881,10 +3881,10 @@ extern "C" int gc_phasen_lesen(gc_ctx* c, double* us7, int32_t reset) {
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->opt.geraet) != hipSuccess || khz <= 0) return -1;
   const double it = (double)std::max<unsigned long long>(g.phasen_it, 1ull);
   for (int k = 0; k < 6; ++k) us6[k] = (double)g.phasen[k] / ((double)khz * 1e-3) / it;
-  us7[6] = (double)g.leiter[0] / ((double)khz * 1e-3) / (double)std::max<unsigned long long>(g.leiter[1], 1ull);
+  us7[6] = (double)g.warte0[0] / ((double)khz * 1e-3) / (double)std::max<unsigned long long>(g.warte0[1], 1ull);
   if (reset) {
     for (int k = 0; k < 6; ++k) g.phasen[k] = 0;
-    for (int k = 0; k < 2; ++k) g.leiter[k] = 0;
+    for (int k = 0; k < 2; ++k) g.warte0[k] = 0;
     const unsigned long long n = g.phasen_it;
     g.phasen_it = 0;
     if (hipMemcpy(c->gs, &g, sizeof(GridState), hipMemcpyHostToDevice) != hipSuccess) return -1;

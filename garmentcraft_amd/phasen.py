@@ -12,8 +12,8 @@ clock at the phase boundaries of every iteration in every block:
   ar_rz   grid all-reduce of r . z (with the release's L2 write-back)
 
 Block 0's split is summed in GridState; every block's split goes into a
-per-block record (stages A-C, update and the two waits). The leader's time
-from its own partial to the published sums is read too.
+per-block record (stages A-C, update and the two waits). Block 0's time
+from its own partial to the summed result is read too.
 
     python -m garmentcraft_amd.phasen [--fenster 5:25 --fenster 240:260]
 
@@ -112,7 +112,7 @@ def main(argv=None) -> None:
             "tiles": arbeit,
             "block0_us": {k: round(float(us[i]), 3) for i, k in enumerate(NAMEN)},
             "block0_sum_us": round(float(us[:6].sum()), 3),
-            "leader_partial_to_result_us": round(float(us[6]), 3),
+            "block0_partial_to_result_us": round(float(us[6]), 3),
             "us_per_iteration": round(pcg * 1e3 / max(st["cg_iterationen"], 1), 3),
             "blocks": int(aktiv.sum()), "per_block_us": pro_block,
             "proximity_pairs": st["naehe_punkt_dreieck"] + st["naehe_kante_kante"],

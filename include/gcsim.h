@@ -294,7 +294,7 @@ int gc_solve(gc_ctx* ctx, double* dv, gc_cg_lauf* lauf);
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid it): gc_solve without any allocation. The grid state and the
  * reduction and barrier slots of its PCG launch (up to 1024 blocks, about
- * 164 KiB) live in a static device array, one per device, instead of the
+ * 128.3 KiB) live in a static device array, one per device, instead of the
  * context's buffers; everything else is the context's, set up before. The
  * warm start and the solution are copied straight between dv and the
  * device's solution vector. Calls that overlap on one device race on that
