@@ -536,11 +536,22 @@ __global__ void k_feder_saetze(const SysDev<T>* __restrict__ sp) {
 }
 
 // Row i's springs from the terms of feder_terme (in LDS from entry e0 of the
-// tile on, up to fe_lds_kap of them), added in the order of feder_zeile; four
-// terms are loaded before their sums.
+// tile on, up to fe_lds_kap of them), added in the order of feder_zeile; eight
+// (FP32) or four terms are loaded before their sums.
 template <typename T>
 __device__ __forceinline__ V3<T> feder_summe_ab(const SysDev<T>& s, uint32_t e, uint32_t b, V3<T> acc, const V3<T>* lds, uint32_t e0) {
   auto term = [&](uint32_t q) { return q - e0 < s.fe_lds_kap ? lds[q - e0] : lg(gm(s.fe_t) + (q)); };
+  // the order of the additions is the entries' order either way; the FP64
+  // kernel has no registers to spare for eight
+  if constexpr (sizeof(T) == 4) {
+    for (; e + 8 <= b; e += 8) {
+      V3<T> t[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) t[k] = term(e + k);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc = acc + t[k];
+    }
+  }
   for (; e + 4 <= b; e += 4) {
     V3<T> t[4];
 #pragma unroll

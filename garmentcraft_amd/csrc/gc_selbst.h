@@ -951,7 +951,7 @@ __global__ void k_sk_federn(const SelbstDev* __restrict__ sp) {
 // Approach damping in list order (:762-781): each pair reads the velocities
 // the pairs before it left, so the pairs are applied one after another by one
 // thread. The block stages them in chunks: all threads load a chunk's pair
-// records (mu, masses, weights, normal, held flags) and put the velocities
+// records (mu, the points' mass shares, weights, normal, held flags) and put the velocities
 // of the points they touch into an LDS hash table keyed by point; thread 0
 // then applies the chunk's pairs in order on the table (the arithmetic and
 // its order are the sequential loop's), and the block writes the touched
@@ -964,7 +964,8 @@ __global__ void __launch_bounds__(1024) k_sk_daempfung(const SelbstDev* __restri
   __shared__ D3 tab_v[SK_DP_TAB];
   __shared__ D3 p_n[SK_DP_CHUNK];
   __shared__ double p_g[SK_DP_CHUNK][4];
-  __shared__ double p_mu[SK_DP_CHUNK], p_m1[SK_DP_CHUNK], p_m2[SK_DP_CHUNK];
+  __shared__ double p_mu[SK_DP_CHUNK], p_anteil[SK_DP_CHUNK][4];
+  __shared__ uint8_t p_massiv[SK_DP_CHUNK];
   __shared__ uint16_t p_pos[SK_DP_CHUNK][4];
   __shared__ uint8_t p_fest[SK_DP_CHUNK];
   const uint32_t nn = s.zaehler[SK_Z_NAEHE];
@@ -1002,9 +1003,11 @@ __global__ void __launch_bounds__(1024) k_sk_daempfung(const SelbstDev* __restri
         else m2 -= g * s.masse[p.punkte[q]];
         if (s.w[p.punkte[q]] <= 0.0) fest |= (uint8_t)(1u << q);
       }
+      // each point's share of the correction: the quotients of the sequential
+      // loop, taken here so that thread 0's loop holds no division
+      for (int q = 0; q < 4; ++q) p_anteil[pr][q] = p_g[pr][q] > 0.0 ? m2 / (m1 + m2) : m1 / (m1 + m2);
       p_mu[pr] = mu;
-      p_m1[pr] = m1;
-      p_m2[pr] = m2;
+      p_massiv[pr] = m1 + m2 > 0.0 ? 1 : 0;
       p_n[pr] = p.normale;
       p_fest[pr] = fest;
     }
@@ -1015,18 +1018,15 @@ __global__ void __launch_bounds__(1024) k_sk_daempfung(const SelbstDev* __restri
     if (threadIdx.x == 0)
       for (uint32_t pr = 0; pr < anzahl; ++pr) {
         const D3 n = p_n[pr];
-        const double m1 = p_m1[pr], m2 = p_m2[pr];
         D3 dv = zero3<double>();
         for (int q = 0; q < 4; ++q) dv = dv + tab_v[p_pos[pr][q]] * p_g[pr][q];
         const double an = dot(dv, n) * SK_ANNAEHERUNG;
-        if (an < 0.0 && m1 + m2 > 0.0) {
+        if (an < 0.0 && p_massiv[pr]) {
           const D3 u = n * an + (dv - n * an) * p_mu[pr];
           for (int q = 0; q < 4; ++q) {
-            const double g = p_g[pr][q];
             if (p_fest[pr] & (1u << q)) continue;
-            const double anteil = g > 0.0 ? m2 / (m1 + m2) : m1 / (m1 + m2);
             const uint16_t e = p_pos[pr][q];
-            tab_v[e] = tab_v[e] - u * (g * anteil);
+            tab_v[e] = tab_v[e] - u * (p_g[pr][q] * p_anteil[pr][q]);
           }
         }
       }
