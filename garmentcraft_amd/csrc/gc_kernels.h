@@ -334,6 +334,14 @@ __device__ __forceinline__ unsigned long long barr_wort(double x, int haelfte, u
   return marke | (haelfte ? (b >> 32) : (b & 0xffffffffull));
 }
 
+// A workgroup barrier that orders LDS only: global loads still in flight stay
+// in flight across it (a __syncthreads waits for them).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int NV, bool VOLL = true>
 __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot* slots, GridState* gs, unsigned& epoch,
                                                unsigned rang, unsigned nb) {
@@ -346,8 +354,11 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
   if (lane == 0)
     for (int k = 0; k < NV; ++k) sh[wave * NV + k] = v[k];
   // every wave's stores and wave sums are in; sh's previous readers (wave 0
-  // below, the broadcast readers at the end) finished before this barrier
-  __syncthreads();
+  // below, the broadcast readers at the end) finished before this barrier.
+  // VOLL = false publishes no global stores here, so its barriers order LDS
+  // only, and the loads the caller issued before it stay in flight.
+  if constexpr (VOLL) __syncthreads();
+  else lds_sync();
   ++epoch;
   Slot* buf = slots + (epoch & 1u) * GC_MAX_BLOECKE;
   double* aus = sh + 16 * NV;
@@ -426,7 +437,8 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
       s_fehler = fehler;
     }
   }
-  __syncthreads();
+  if constexpr (VOLL) __syncthreads();
+  else lds_sync();
   if (s_fehler) return false;
   for (int k = 0; k < NV; ++k) v[k] = aus[k];
   return true;
@@ -1466,6 +1478,11 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       const bool erste = it == it_start;
       // ap = S(A p), ca = p . ap (implizit.rs:401-405)
       double ca[1] = {0.0};
+      // one row per thread (FP32): the row's p and A p stay in registers for
+      // the update, whose other operands are loaded at the end of stage C, so
+      // that their loads run during the p.Ap all-reduce
+      V3<T> u_p = zero3<T>(), u_ap = zero3<T>(), u_dv = zero3<T>(), u_r = zero3<T>();
+      T u_pinv[9];
       for (int t = rang; t < s.n_kacheln; t += nb) {
         const KachelK kk = konst(t);
         const int lo = kk.lo, hi = kk.hi;
@@ -1498,6 +1515,12 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
             sg(pn + zr, pi);
             sg(gm(s.ap) + (zr), ap);
             ca[0] += (double)dot(pi, ap);
+            u_p = pi;
+            u_ap = ap;
+            u_dv = lg(gm(s.dv) + (zr));
+            u_r = lg(gm(s.r) + (zr));
+#pragma unroll
+            for (int q = 0; q < 9; ++q) u_pinv[q] = gm(s.pinv)[9 * (size_t)zr + q];
           }
         } else {
           for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
@@ -1523,6 +1546,16 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       // dv += p alpha, r -= ap alpha, z = P r, neu = r . z (implizit.rs:410-417)
       double neu[1] = {0.0};
       const T ta = T(alpha);
+      if (zeile_reg) {
+        if (hat_zeile) {
+          sg(gm(s.dv) + (zr), u_dv + u_p * ta);
+          const V3<T> ri = u_r - u_ap * ta;
+          sg(gm(s.r) + (zr), ri);
+          const V3<T> zi = m3_mal(u_pinv, ri);
+          sg(gm(s.z) + (zr), zi);
+          neu[0] += (double)dot(ri, zi);
+        }
+      } else
       for (int t = rang; t < s.n_kacheln; t += nb) {
         int lo, hi;
         if (eine) {
