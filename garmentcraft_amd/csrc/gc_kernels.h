@@ -1464,8 +1464,9 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     bool neu_starten = false;
     while (delta > ziel && it < s.cg_max) {
 #ifdef GC_PHASEN
+      // thread 0's stamps, in LDS: registers would be held by every lane
       const bool uhr = threadIdx.x == 0;
-      unsigned long long tp[7];
+      __shared__ unsigned long long tp[7];
       if (uhr) tp[0] = wall_clock64();
 #define GC_PHASE(k) \
   if (uhr) tp[k] = wall_clock64();
