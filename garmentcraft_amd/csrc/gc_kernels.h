@@ -1459,6 +1459,14 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
     double beta = 0.0;
     int cur = 0;
     const int it_start = it;
+    // one row per thread (FP32): the row's dv and r stay in registers from
+    // here to the end of the loop (no other block reads them), A p is never
+    // stored, and an iteration stores only p and z, which other blocks read
+    V3<T> u_dv = zero3<T>(), u_r = zero3<T>();
+    if (hat_zeile) {
+      u_dv = lg(gm(s.dv) + (zr));
+      u_r = lg(gm(s.r) + (zr));
+    }
     double dmin = delta;
     int it_min = it;
     bool neu_starten = false;
@@ -1480,9 +1488,9 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       // ap = S(A p), ca = p . ap (implizit.rs:401-405)
       double ca[1] = {0.0};
       // one row per thread (FP32): the row's p and A p stay in registers for
-      // the update, whose other operands are loaded at the end of stage C, so
-      // that their loads run during the p.Ap all-reduce
-      V3<T> u_p = zero3<T>(), u_ap = zero3<T>(), u_dv = zero3<T>(), u_r = zero3<T>();
+      // the update, whose preconditioner block is loaded at the end of stage
+      // C, so that the load runs during the p.Ap all-reduce
+      V3<T> u_p = zero3<T>(), u_ap = zero3<T>();
       T u_pinv[9];
       for (int t = rang; t < s.n_kacheln; t += nb) {
         const KachelK kk = konst(t);
@@ -1514,12 +1522,9 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
             const V3<T> hp = s.selbst ? feder_summe_ab(s, r_fa, r_fb, hk, fterm, fe0) : hk;
             const V3<T> ap = filter_fl(r_fl, r_n, hp * th2 + pi * r_md);
             sg(pn + zr, pi);
-            sg(gm(s.ap) + (zr), ap);
             ca[0] += (double)dot(pi, ap);
             u_p = pi;
             u_ap = ap;
-            u_dv = lg(gm(s.dv) + (zr));
-            u_r = lg(gm(s.r) + (zr));
 #pragma unroll
             for (int q = 0; q < 9; ++q) u_pinv[q] = gm(s.pinv)[9 * (size_t)zr + q];
           }
@@ -1549,9 +1554,9 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
       const T ta = T(alpha);
       if (zeile_reg) {
         if (hat_zeile) {
-          sg(gm(s.dv) + (zr), u_dv + u_p * ta);
+          u_dv = u_dv + u_p * ta;
           const V3<T> ri = u_r - u_ap * ta;
-          sg(gm(s.r) + (zr), ri);
+          u_r = ri;
           const V3<T> zi = m3_mal(u_pinv, ri);
           sg(gm(s.z) + (zr), zi);
           neu[0] += (double)dot(ri, zi);
@@ -1608,6 +1613,15 @@ __device__ __forceinline__ void pcg_loesen(const SysDev<T>& s, GridState* gs, Sl
           break;
         }
       }
+    }
+    if (hat_zeile) {
+      sg(gm(s.dv) + (zr), u_dv);
+      sg(gm(s.r) + (zr), u_r);
+    }
+    // a restart reads the dv of other blocks' points: publish it first
+    if (!verf && neu_starten && neustarts < 3) {
+      double nix[1] = {0.0};
+      if (!grid_allreduce<1>(nix, sh, slots, gs, epoch, (unsigned)rang, (unsigned)nb)) return;
     }
     if (verf) break;  // the round ends; k_verf checks it in FP64
     if (!neu_starten) break;
