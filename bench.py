@@ -25,7 +25,7 @@ WURZEL = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, WURZEL)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
-TRAFFIC_JE_ITERATION = 5338409
+TRAFFIC_JE_ITERATION = 3585606
 TRAFFIC_EINHEIT = "B per PCG iteration (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB x 1024; steps 241-250, 136 x 1024)"
 
 
