@@ -334,8 +334,10 @@ __device__ __forceinline__ unsigned long long barr_wort(double x, int haelfte, u
   return marke | (haelfte ? (b >> 32) : (b & 0xffffffffull));
 }
 
-// A workgroup barrier that orders LDS only: global loads still in flight stay
-// in flight across it (a __syncthreads waits for them).
+// A workgroup barrier that orders LDS only. On gfx950 (no threadgroup split)
+// a __syncthreads emits the same s_barrier with no wait on global memory
+// either; this form states that nothing global is ordered here, so loads
+// issued before it stay in flight until their first use.
 __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
@@ -356,7 +358,8 @@ __device__ __forceinline__ bool grid_allreduce(double (&v)[NV], double* sh, Slot
   // every wave's stores and wave sums are in; sh's previous readers (wave 0
   // below, the broadcast readers at the end) finished before this barrier.
   // VOLL = false publishes no global stores here, so its barriers order LDS
-  // only, and the loads the caller issued before it stay in flight.
+  // only; with no release fence either, wave 0 stores its partial without
+  // waiting for loads the caller issued before it.
   if constexpr (VOLL) __syncthreads();
   else lds_sync();
   ++epoch;
